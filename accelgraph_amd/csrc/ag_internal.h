@@ -1,0 +1,91 @@
+// ag_internal.h -- shared definitions of the HIP runtime and kernels (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "accelgraph.h"
+#include "../host/ag_plan.h"
+
+#define AG_MAX_EVENTS 2
+
+struct ag_device {
+    int ordinal;
+    hipStream_t stream;
+    uint64_t run_timeout_ms;
+    uint32_t poll_us;
+    uint32_t live_graphs;   // graphs uploaded on this device and not yet freed
+};
+
+// Device-resident graph: the plan arrays in HBM plus launch geometry.
+struct ag_dgraph {
+    ag_device *dev;
+    ag_plan_params params;
+    uint32_t n, m;
+    uint32_t bin_row[AG_NBINS + 1];
+    uint64_t bin_edges[AG_NBINS];
+    uint32_t bin_blocks[AG_NBINS];
+    uint32_t slot_blk0[AG_NSLOTS + 1];
+    uint32_t total_blocks;
+    uint32_t n_tiles, n_hubs, n_segs;
+    uint64_t device_bytes;
+    // HBM arrays
+    uint32_t *d2o;       // [n]
+    uint32_t *h_d2o;     // [n] host copy of the device-row map
+    uint32_t *off;       // [n+1]
+    uint32_t *src;       // [m]
+    float *w;            // [m] or null
+    uint32_t *wq;        // [m] FloatToFixed32(w), built on first fixed SpMV, or user codes
+    int wq_user;         // wq came from ag_graph_set_fixed_weights
+    uint32_t *outdeg;    // [n]
+    uint32_t *tile_row;  // [n_tiles+1]
+    uint32_t *seg_row, *seg_lo, *seg_hi;  // [n_segs]
+    uint32_t *hub_seg0;  // [n_hubs+1]
+    uint32_t *hub_ticket;  // [n_hubs] zeroed; each hub's last segment resets its counter
+    uint64_t *hub_part;    // [n_segs] 8-byte partial sums (float or u64 bits)
+    // scratch for permuted vectors (x, y) of SpMV / gather, sized lazily
+    void *vec_a;
+    void *vec_b;
+    size_t vec_bytes;
+};
+
+// Geometry passed by value to the pull kernels.
+struct PullGeom {
+    const uint32_t *off;
+    const uint32_t *src;
+    const uint32_t *tile_row;
+    const uint32_t *seg_row;
+    const uint32_t *seg_lo;
+    const uint32_t *seg_hi;
+    const uint32_t *hub_seg0;
+    uint32_t *hub_ticket;
+    uint64_t *hub_part;
+    uint32_t n;
+    uint32_t bin_row[AG_NBINS + 1];
+    uint32_t slot_blk0[AG_NSLOTS + 1];
+};
+
+// Device-side iteration state of a PageRank session.
+struct PrState {
+    uint32_t iter;          // iterations executed so far
+    uint32_t done;          // set by the last block of the stopping iteration
+    uint32_t stop_iter;     // index of the stopping iteration
+    uint32_t tickets;       // block counter of the running launch (reset by its last block)
+    int32_t refine_iter;    // first iteration that writes f32 contributions (-1: none yet)
+    uint32_t last_active;
+    double last_err;
+};
+
+int ag_set_error(int status, const char *fmt, ...);
+int ag_hip_fail(hipError_t e, const char *what, const char *file, int line);
+int ag_wait_stream(ag_device *dev);
+
+#define AG_HIP(call)                                                   \
+    do {                                                               \
+        hipError_t ag_e_ = (call);                                     \
+        if (ag_e_ != hipSuccess)                                       \
+            return ag_hip_fail(ag_e_, #call, __FILE__, __LINE__);      \
+    } while (0)
+
+PullGeom ag_make_geom(const ag_dgraph *g);
+int ag_ensure_vec(ag_dgraph *g, size_t bytes);

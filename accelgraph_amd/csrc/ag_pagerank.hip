@@ -1,0 +1,870 @@
+// ag_pagerank.hip -- PageRank pull on the GPU: one fused launch per iteration.
+//
+// Reference host loop (02_capi_graph/src/algorithms/capi/pageRank.c:942-1109), per iteration:
+//   contrib[v] = out_degree[v] ? rank[v] / out_degree[v] : 0          (:1026-1033)
+//   next[v]    = sum over in-edges (u->v) of contrib[u]                (FPGA gather, :1041-1046)
+//   rank[v]    = base_pr + damp * next[v]; err += |delta| / N;
+//   active    += |delta| >= epsilon                                    (:1055-1069)
+//   stop when active == 0                                              (:1072-1074)
+// and finally rank[v] /= N with an FP64 sum (:1079-1085).
+//
+// Here one launch does the gather, the damping update, the L1/active reduction and the next
+// iteration's contribution for every vertex; `next` never exists in memory. Contributions
+// ping-pong between two buffers by iteration parity. The launch's last workgroup (ticket
+// counter) reduces the per-workgroup partials in a fixed order, records the iteration's
+// error/active count, and sets a done flag that turns every later launch into a no-op, so
+// many iterations can be queued without a host round trip per iteration.
+//
+// Fixed-point variants (pageRank.c:1276-1990) gather u64/u32/u16/u8 contributions into an
+// exact u64 sum (wraps modulo 2^64) and convert with the formats of fixedPoint.h.
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "ag_internal.h"
+#include "ag_fixed_dev.h"
+#include "ag_pull_kernel.h"
+
+namespace {
+
+// ---------------------------------------------------------------- 16-bit storage
+
+struct bf16s {
+    uint16_t bits;
+};
+
+__device__ __forceinline__ float ld_f32(float v) { return v; }
+__device__ __forceinline__ float ld_f32(bf16s v) { return __uint_as_float((uint32_t)v.bits << 16); }
+__device__ __forceinline__ float ld_f32(_Float16 v) { return (float)v; }
+
+template <typename T>
+__device__ __forceinline__ T st_from_f32(float x);
+template <>
+__device__ __forceinline__ float st_from_f32<float>(float x) { return x; }
+template <>
+__device__ __forceinline__ bf16s st_from_f32<bf16s>(float x)
+{
+    // round to nearest even; inputs are finite
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return bf16s{(uint16_t)(u >> 16)};
+}
+template <>
+__device__ __forceinline__ _Float16 st_from_f32<_Float16>(float x) { return (_Float16)x; }
+
+// ---------------------------------------------------------------- iteration control
+
+struct PrCtl {
+    PrState *st;
+    double *part_err;      // [gridDim.x]
+    uint32_t *part_act;    // [gridDim.x]
+    double *hist_err;      // [max_iterations]
+    uint32_t *hist_act;    // [max_iterations]
+    uint32_t stop_rule;
+    uint32_t mixed;        // 16-bit storage with f32 refinement
+    double l1_tol;
+    double refine_l1;
+};
+
+__device__ __forceinline__ void pr_epilogue(const PrCtl &C, uint32_t iter, Red red)
+{
+    __shared__ double s_e[AG_WG / AG_WAVE];
+    __shared__ uint32_t s_a[AG_WG / AG_WAVE];
+    __shared__ uint32_t s_last;
+    const uint32_t tid = threadIdx.x;
+    double e = group_reduce<double, AG_WAVE>(red.err);
+    uint32_t a = group_reduce<uint32_t, AG_WAVE>(red.active);
+    if ((tid & (AG_WAVE - 1)) == 0) {
+        s_e[tid / AG_WAVE] = e;
+        s_a[tid / AG_WAVE] = a;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double be = s_e[0];
+        uint32_t ba = s_a[0];
+        for (uint32_t k = 1; k < AG_WG / AG_WAVE; k++) {
+            be += s_e[k];
+            ba += s_a[k];
+        }
+        __hip_atomic_store(&C.part_err[blockIdx.x], be, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&C.part_act[blockIdx.x], ba, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t t = __hip_atomic_fetch_add(&C.st->tickets, 1u, __ATOMIC_ACQ_REL,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+        s_last = (t == gridDim.x - 1) ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!s_last)
+        return;
+    double te = 0.0;
+    uint32_t ta = 0;
+    for (uint32_t i = tid; i < gridDim.x; i += AG_WG) {
+        te += __hip_atomic_load(&C.part_err[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ta += __hip_atomic_load(&C.part_act[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    te = group_reduce<double, AG_WAVE>(te);
+    ta = group_reduce<uint32_t, AG_WAVE>(ta);
+    __syncthreads();
+    if ((tid & (AG_WAVE - 1)) == 0) {
+        s_e[tid / AG_WAVE] = te;
+        s_a[tid / AG_WAVE] = ta;
+    }
+    __syncthreads();
+    if (tid != 0)
+        return;
+    double E = s_e[0];
+    uint32_t A = s_a[0];
+    for (uint32_t k = 1; k < AG_WG / AG_WAVE; k++) {
+        E += s_e[k];
+        A += s_a[k];
+    }
+    PrState *st = C.st;
+    C.hist_err[iter] = E;
+    C.hist_act[iter] = A;
+    bool can_stop = true;
+    if (C.mixed) {
+        int32_t ri = st->refine_iter;
+        if (ri < 0) {
+            // switch to f32 contributions once the L1 delta is small enough, or as soon as
+            // the 16-bit iteration stops improving
+            bool stalled = iter > 0 && E >= st->last_err;
+            if (E <= C.refine_l1 || stalled)
+                st->refine_iter = (int32_t)iter + 1;
+            can_stop = false;
+        } else {
+            can_stop = (int32_t)iter >= ri + 1;
+        }
+    }
+    bool stop = false;
+    if (C.stop_rule == AG_STOP_ACTIVE)
+        stop = (A == 0);
+    else if (C.stop_rule == AG_STOP_L1)
+        stop = (E <= C.l1_tol);
+    if (can_stop && stop) {
+        st->done = 1;
+        st->stop_iter = iter;
+    }
+    st->last_err = E;
+    st->last_active = A;
+    st->iter = iter + 1;
+    __hip_atomic_store(&st->tickets, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------- FP32 policy
+
+template <typename RT, typename WT>
+struct PrF32Pol {
+    using Acc = float;
+    const RT *cin;
+    WT *cout;
+    float *rank;
+    const uint32_t *outdeg;
+    float base, damp;
+    double dn, eps;
+
+    __device__ __forceinline__ float edge(uint32_t, uint32_t s) const { return ld_f32(cin[s]); }
+    __device__ __forceinline__ void finish(uint32_t r, float sum, Red &red) const
+    {
+        const float nw = __fadd_rn(base, __fmul_rn(damp, sum));
+        const float old = rank[r];
+        rank[r] = nw;
+        const double e = (double)fabsf(__fsub_rn(nw, old));
+        red.err += __ddiv_rn(e, dn);
+        red.active += (e >= eps) ? 1u : 0u;
+        const uint32_t od = outdeg[r];
+        cout[r] = st_from_f32<WT>(od ? __fdiv_rn(nw, __uint2float_rn(od)) : 0.0f);
+    }
+};
+
+struct PrF32Args {
+    void *c32[2];      // f32 contribution buffers (ping-pong)
+    void *c16[2];      // 16-bit contribution buffers (mixed storage only)
+    float *rank;
+    const uint32_t *outdeg;
+    float base, damp;
+    double dn, eps;
+};
+
+template <typename HT>
+__device__ __forceinline__ void pr_f32_iteration(const PullGeom &G, const PrCtl &C,
+                                                 const PrF32Args &A, bool mixed)
+{
+    __shared__ float s_vals[AG_TILE_NNZ_MAX];
+    __shared__ float s_wave[AG_WG / AG_WAVE];
+    const PrState *st = C.st;
+    if (*(volatile const uint32_t *)&st->done)
+        return;
+    const uint32_t iter = *(volatile const uint32_t *)&st->iter;
+    const uint32_t rd = iter & 1u, wr = rd ^ 1u;
+    Red red{0.0, 0u};
+    if (!mixed) {
+        PrF32Pol<float, float> p{(const float *)A.c32[rd], (float *)A.c32[wr], A.rank, A.outdeg,
+                                 A.base, A.damp, A.dn, A.eps};
+        pull_body(G, p, red, s_vals, s_wave);
+    } else {
+        const int32_t ri = *(volatile const int32_t *)&st->refine_iter;
+        const bool wf32 = ri >= 0 && (int32_t)iter >= ri;
+        const bool rf32 = ri >= 0 && (int32_t)iter >= ri + 1;
+        if (rf32) {
+            PrF32Pol<float, float> p{(const float *)A.c32[rd], (float *)A.c32[wr], A.rank,
+                                     A.outdeg, A.base, A.damp, A.dn, A.eps};
+            pull_body(G, p, red, s_vals, s_wave);
+        } else if (wf32) {
+            PrF32Pol<HT, float> p{(const HT *)A.c16[rd], (float *)A.c32[wr], A.rank, A.outdeg,
+                                  A.base, A.damp, A.dn, A.eps};
+            pull_body(G, p, red, s_vals, s_wave);
+        } else {
+            PrF32Pol<HT, HT> p{(const HT *)A.c16[rd], (HT *)A.c16[wr], A.rank, A.outdeg, A.base,
+                               A.damp, A.dn, A.eps};
+            pull_body(G, p, red, s_vals, s_wave);
+        }
+    }
+    pr_epilogue(C, iter, red);
+}
+
+__global__ __launch_bounds__(AG_WG) void pr_f32_kernel(PullGeom G, PrCtl C, PrF32Args A)
+{
+    pr_f32_iteration<float>(G, C, A, false);
+}
+__global__ __launch_bounds__(AG_WG) void pr_bf16_kernel(PullGeom G, PrCtl C, PrF32Args A)
+{
+    pr_f32_iteration<bf16s>(G, C, A, true);
+}
+__global__ __launch_bounds__(AG_WG) void pr_f16_kernel(PullGeom G, PrCtl C, PrF32Args A)
+{
+    pr_f32_iteration<_Float16>(G, C, A, true);
+}
+
+// ---------------------------------------------------------------- fixed-point policy
+
+template <int BITS>
+struct FixT;
+template <> struct FixT<64> { using T = uint64_t; };
+template <> struct FixT<32> { using T = uint32_t; };
+template <> struct FixT<16> { using T = uint16_t; };
+template <> struct FixT<8> { using T = uint8_t; };
+
+template <int BITS>
+__device__ __forceinline__ typename FixT<BITS>::T fix_encode(float quotient)
+{
+    if constexpr (BITS == 64)
+        return ag_dev_double_to_fixed64((double)quotient);
+    else if constexpr (BITS == 32)
+        return ag_dev_float_to_fixed32(quotient);
+    else if constexpr (BITS == 16)
+        return ag_dev_float_to_fixed16(quotient);
+    else
+        return ag_dev_float_to_fixed8(quotient);
+}
+
+template <int BITS>
+__device__ __forceinline__ float fix_update(float base, float damp, uint64_t q)
+{
+    if constexpr (BITS == 64) {
+        // float + float * double is evaluated in double and stored to float (pageRank.c:1400)
+        return __double2float_rn(__dadd_rn((double)base, __dmul_rn((double)damp,
+                                                                   ag_dev_fixed64_to_double(q))));
+    } else {
+        float f;
+        if constexpr (BITS == 32)
+            f = ag_dev_fixed32_to_float(q);
+        else if constexpr (BITS == 16)
+            f = ag_dev_fixed16_to_float(q);
+        else
+            f = ag_dev_fixed8_to_float(q);
+        return __fadd_rn(base, __fmul_rn(damp, f));
+    }
+}
+
+template <int BITS>
+struct PrFixPol {
+    using Acc = uint64_t;
+    using CT = typename FixT<BITS>::T;
+    const CT *cin;
+    CT *cout;
+    float *rank;
+    const uint32_t *outdeg;
+    float base, damp;
+    double dn, eps;
+
+    __device__ __forceinline__ uint64_t edge(uint32_t, uint32_t s) const { return (uint64_t)cin[s]; }
+    __device__ __forceinline__ void finish(uint32_t r, uint64_t q, Red &red) const
+    {
+        const float nw = fix_update<BITS>(base, damp, q);
+        const float old = rank[r];
+        rank[r] = nw;
+        const double e = (double)fabsf(__fsub_rn(nw, old));
+        red.err += __ddiv_rn(e, dn);
+        red.active += (e >= eps) ? 1u : 0u;
+        const uint32_t od = outdeg[r];
+        cout[r] = od ? fix_encode<BITS>(__fdiv_rn(nw, __uint2float_rn(od))) : (CT)0;
+    }
+};
+
+template <int BITS>
+__global__ __launch_bounds__(AG_WG) void pr_fixed_kernel(PullGeom G, PrCtl C, PrF32Args A)
+{
+    __shared__ uint64_t s_vals[AG_TILE_NNZ_MAX];
+    __shared__ uint64_t s_wave[AG_WG / AG_WAVE];
+    using CT = typename FixT<BITS>::T;
+    const PrState *st = C.st;
+    if (*(volatile const uint32_t *)&st->done)
+        return;
+    const uint32_t iter = *(volatile const uint32_t *)&st->iter;
+    const uint32_t rd = iter & 1u, wr = rd ^ 1u;
+    Red red{0.0, 0u};
+    PrFixPol<BITS> p{(const CT *)A.c32[rd], (CT *)A.c32[wr], A.rank, A.outdeg, A.base, A.damp,
+                     A.dn, A.eps};
+    pull_body(G, p, red, s_vals, s_wave);
+    pr_epilogue(C, iter, red);
+}
+
+// ---------------------------------------------------------------- init / finalise
+
+// kind: 0 = f32 contributions, 1 = bf16, 2 = f16, 64/32/16/8 = fixed point
+__global__ void pr_init_kernel(uint32_t n, uint32_t kind, uint32_t init, float base,
+                               const float *user_orig, const uint32_t *d2o,
+                               const uint32_t *outdeg, float *rank, void *c0)
+{
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n)
+        return;
+    float x = (init == AG_INIT_REFERENCE) ? base : (init == AG_INIT_UNIFORM) ? 1.0f
+                                                                           : user_orig[d2o[r]];
+    rank[r] = x;
+    uint32_t od = outdeg[r];
+    float q = od ? __fdiv_rn(x, __uint2float_rn(od)) : 0.0f;
+    switch (kind) {
+    case 0: ((float *)c0)[r] = q; break;
+    case 1: ((bf16s *)c0)[r] = st_from_f32<bf16s>(q); break;
+    case 2: ((_Float16 *)c0)[r] = st_from_f32<_Float16>(q); break;
+    case 64: ((uint64_t *)c0)[r] = od ? ag_dev_double_to_fixed64((double)q) : 0ull; break;
+    case 32: ((uint32_t *)c0)[r] = od ? ag_dev_float_to_fixed32(q) : 0u; break;
+    case 16: ((uint16_t *)c0)[r] = od ? ag_dev_float_to_fixed16(q) : (uint16_t)0; break;
+    default: ((uint8_t *)c0)[r] = od ? ag_dev_float_to_fixed8(q) : (uint8_t)0; break;
+    }
+}
+
+// out[d2o[r]] = rank[r] / N (pageRank.c:1083); per-workgroup FP64 partial sums
+__global__ __launch_bounds__(AG_WG) void pr_finalize_kernel(uint32_t n, const float *rank,
+                                                             const uint32_t *d2o, float *out_orig,
+                                                             float *scaled_orig, double *part)
+{
+    __shared__ double s_wave[AG_WG / AG_WAVE];
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (r < n) {
+        float x = rank[r];
+        float y = __fdiv_rn(x, __uint2float_rn(n));
+        out_orig[d2o[r]] = y;
+        if (scaled_orig)
+            scaled_orig[d2o[r]] = x;
+        v = (double)y;
+    }
+    v = block_reduce<double>(v, s_wave);
+    if (threadIdx.x == 0)
+        part[blockIdx.x] = v;
+}
+
+__global__ void sum_partials_kernel(const double *part, uint32_t count, double *out)
+{
+    __shared__ double s_wave[AG_WG / AG_WAVE];
+    double v = 0.0;
+    for (uint32_t i = threadIdx.x; i < count; i += AG_WG)
+        v += part[i];
+    v = block_reduce<double>(v, s_wave);
+    if (threadIdx.x == 0)
+        *out = v;
+}
+
+__global__ void scaled_copy_kernel(uint32_t n, const float *rank, const uint32_t *d2o, float *out)
+{
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n)
+        out[d2o[r]] = rank[r];
+}
+
+} // namespace
+
+// ---------------------------------------------------------------- host side
+
+// damp = 0.85f and base_pr = 1.0f - damp, both in float (pageRank.h:13, pageRank.c:57-58)
+static inline float pr_damp(void) { return 0.85f; }
+static inline float pr_base(void)
+{
+    volatile float d = 0.85f;
+    return 1.0f - d;
+}
+
+struct ag_pr_session {
+    ag_dgraph *g;
+    ag_pr_options o;
+    uint32_t kind_code;         // 0 f32, 1 bf16, 2 f16, 64/32/16/8
+    size_t celem;               // bytes per stored contribution
+    PrState *st;                // device
+    PrState *st_host;           // pinned
+    double *part_err;
+    uint32_t *part_act;
+    double *hist_err;
+    uint32_t *hist_act;
+    float *rank;
+    void *c[2];
+    void *c16[2];
+    double *fin_part;
+    double *fin_sum;
+    float *user;
+    uint32_t enqueued;
+    hipEvent_t *ev;             // per-launch events when record_history (max_iterations + 1)
+    hipEvent_t step_ev[2];
+    uint32_t step_launches;
+    uint32_t ev_count;
+    struct timespec t0;
+};
+
+extern "C" void ag_pr_options_default(ag_pr_options *o)
+{
+    memset(o, 0, sizeof(*o));
+    o->kind = AG_KIND_F32;
+    o->max_iterations = 20;     // the reference CLI default (accel-graph.c:285-329)
+    o->epsilon = 1e-4;
+    o->stop_rule = AG_STOP_ACTIVE;
+    o->l1_tolerance = 1e-6;
+    o->init = AG_INIT_REFERENCE;
+    o->storage = AG_STORE_F32;
+    o->refine_l1 = 1e-3;
+    o->chunk = 0;
+    o->record_history = 0;
+}
+
+static void session_free(ag_pr_session *s)
+{
+    if (!s)
+        return;
+    hipSetDevice(s->g->dev->ordinal);
+    hipStreamSynchronize(s->g->dev->stream);
+    void *ptrs[] = {s->st, s->part_err, s->part_act, s->hist_err, s->hist_act, s->rank, s->c[0],
+                    s->c[1], s->c16[0], s->c16[1], s->fin_part, s->fin_sum, s->user};
+    for (void *p : ptrs)
+        if (p)
+            hipFree(p);
+    if (s->st_host)
+        hipHostFree(s->st_host);
+    if (s->ev) {
+        for (uint32_t i = 0; i < s->ev_count; i++)
+            hipEventDestroy(s->ev[i]);
+        free(s->ev);
+    }
+    if (s->step_ev[0])
+        hipEventDestroy(s->step_ev[0]);
+    if (s->step_ev[1])
+        hipEventDestroy(s->step_ev[1]);
+    free(s);
+}
+
+static int salloc(ag_pr_session *s, void **p, size_t bytes)
+{
+    AG_HIP(hipMalloc(p, bytes ? bytes : 8));
+    (void)s;
+    return AG_OK;
+}
+
+#define AG_TRY(x)                   \
+    do {                            \
+        int ag_st_ = (x);           \
+        if (ag_st_ != AG_OK) {      \
+            session_free(s);        \
+            return ag_st_;          \
+        }                           \
+    } while (0)
+
+extern "C" int ag_pr_begin(ag_dgraph *g, const ag_pr_options *opts, ag_pr_session **out)
+{
+    if (!g || !out)
+        return ag_set_error(AG_ERR_INVALID, "graph or out is null");
+    *out = NULL;
+    ag_pr_options o;
+    if (opts)
+        o = *opts;
+    else
+        ag_pr_options_default(&o);
+    uint32_t kc;
+    size_t celem;
+    switch (o.kind) {
+    case AG_KIND_F32:
+        if (o.storage == AG_STORE_F32) { kc = 0; celem = 4; }
+        else if (o.storage == AG_STORE_BF16) { kc = 1; celem = 4; }
+        else if (o.storage == AG_STORE_F16) { kc = 2; celem = 4; }
+        else return ag_set_error(AG_ERR_INVALID, "unknown storage %u", o.storage);
+        break;
+    case AG_KIND_FIXED64: kc = 64; celem = 8; break;
+    case AG_KIND_FIXED32: kc = 32; celem = 4; break;
+    case AG_KIND_FIXED16: kc = 16; celem = 2; break;
+    case AG_KIND_FIXED8: kc = 8; celem = 1; break;
+    default: return ag_set_error(AG_ERR_INVALID, "unknown kind %u", o.kind);
+    }
+    if (o.kind != AG_KIND_F32 && o.storage != AG_STORE_F32)
+        return ag_set_error(AG_ERR_UNSUPPORTED, "16-bit storage applies to the FP32 kind only");
+    if (o.stop_rule > AG_STOP_NONE)
+        return ag_set_error(AG_ERR_INVALID, "unknown stop rule %u", o.stop_rule);
+    if (o.init > AG_INIT_USER || (o.init == AG_INIT_USER && !o.init_ranks && g->n))
+        return ag_set_error(AG_ERR_INVALID, "bad init mode or missing init_ranks");
+    if (!(o.epsilon >= 0.0))
+        return ag_set_error(AG_ERR_INVALID, "epsilon must be >= 0");
+
+    AG_HIP(hipSetDevice(g->dev->ordinal));
+    ag_pr_session *s = (ag_pr_session *)calloc(1, sizeof(ag_pr_session));
+    if (!s)
+        return ag_set_error(AG_ERR_OOM, "host allocation failed");
+    s->g = g;
+    s->o = o;
+    s->kind_code = kc;
+    s->celem = celem;
+    clock_gettime(CLOCK_MONOTONIC, &s->t0);
+    hipStream_t stream = g->dev->stream;
+    const uint32_t n = g->n;
+    const uint32_t maxit = o.max_iterations ? o.max_iterations : 1;
+    const uint32_t nblk = g->total_blocks ? g->total_blocks : 1;
+    const uint32_t fblk = (n + AG_WG - 1) / AG_WG ? (n + AG_WG - 1) / AG_WG : 1;
+
+    AG_TRY(salloc(s, (void **)&s->st, sizeof(PrState)));
+    {
+        hipError_t e = hipHostMalloc((void **)&s->st_host, sizeof(PrState), hipHostMallocDefault);
+        if (e != hipSuccess) {
+            session_free(s);
+            return ag_hip_fail(e, "hipHostMalloc", __FILE__, __LINE__);
+        }
+    }
+    AG_TRY(salloc(s, (void **)&s->part_err, (size_t)nblk * sizeof(double)));
+    AG_TRY(salloc(s, (void **)&s->part_act, (size_t)nblk * sizeof(uint32_t)));
+    AG_TRY(salloc(s, (void **)&s->hist_err, (size_t)maxit * sizeof(double)));
+    AG_TRY(salloc(s, (void **)&s->hist_act, (size_t)maxit * sizeof(uint32_t)));
+    AG_TRY(salloc(s, (void **)&s->rank, (size_t)n * sizeof(float)));
+    AG_TRY(salloc(s, &s->c[0], (size_t)n * celem));
+    AG_TRY(salloc(s, &s->c[1], (size_t)n * celem));
+    if (kc == 1 || kc == 2) {
+        AG_TRY(salloc(s, &s->c16[0], (size_t)n * 2));
+        AG_TRY(salloc(s, &s->c16[1], (size_t)n * 2));
+    }
+    AG_TRY(salloc(s, (void **)&s->fin_part, (size_t)fblk * sizeof(double)));
+    AG_TRY(salloc(s, (void **)&s->fin_sum, sizeof(double)));
+
+    PrState init_st;
+    memset(&init_st, 0, sizeof(init_st));
+    init_st.refine_iter = -1;
+    *s->st_host = init_st;
+    {
+        hipError_t e = hipMemcpyAsync(s->st, s->st_host, sizeof(PrState), hipMemcpyHostToDevice, stream);
+        if (e != hipSuccess) {
+            session_free(s);
+            return ag_hip_fail(e, "state upload", __FILE__, __LINE__);
+        }
+    }
+    if (o.init == AG_INIT_USER && n) {
+        AG_TRY(salloc(s, (void **)&s->user, (size_t)n * sizeof(float)));
+        hipError_t e = hipMemcpyAsync(s->user, o.init_ranks, (size_t)n * sizeof(float),
+                                      hipMemcpyHostToDevice, stream);
+        if (e != hipSuccess) {
+            session_free(s);
+            return ag_hip_fail(e, "init ranks upload", __FILE__, __LINE__);
+        }
+    }
+    if (n) {
+        void *c0 = (kc == 1 || kc == 2) ? s->c16[0] : s->c[0];
+        pr_init_kernel<<<(n + AG_WG - 1) / AG_WG, AG_WG, 0, stream>>>(
+            n, kc, o.init, pr_base(), s->user, g->d2o, g->outdeg, s->rank, c0);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            session_free(s);
+            return ag_hip_fail(e, "pr_init_kernel launch", __FILE__, __LINE__);
+        }
+    }
+    if (o.record_history) {
+        s->ev_count = maxit + 1;
+        s->ev = (hipEvent_t *)calloc(s->ev_count, sizeof(hipEvent_t));
+        if (!s->ev) {
+            session_free(s);
+            return ag_set_error(AG_ERR_OOM, "host allocation failed");
+        }
+        for (uint32_t i = 0; i < s->ev_count; i++) {
+            hipError_t e = hipEventCreate(&s->ev[i]);
+            if (e != hipSuccess) {
+                s->ev_count = i;
+                session_free(s);
+                return ag_hip_fail(e, "hipEventCreate", __FILE__, __LINE__);
+            }
+        }
+    }
+    {
+        hipError_t e = hipEventCreate(&s->step_ev[0]);
+        if (e == hipSuccess)
+            e = hipEventCreate(&s->step_ev[1]);
+        if (e != hipSuccess) {
+            session_free(s);
+            return ag_hip_fail(e, "hipEventCreate", __FILE__, __LINE__);
+        }
+    }
+    *out = s;
+    return AG_OK;
+}
+
+static int launch_iteration(ag_pr_session *s)
+{
+    ag_dgraph *g = s->g;
+    hipStream_t stream = g->dev->stream;
+    PullGeom G = ag_make_geom(g);
+    PrCtl C;
+    C.st = s->st;
+    C.part_err = s->part_err;
+    C.part_act = s->part_act;
+    C.hist_err = s->hist_err;
+    C.hist_act = s->hist_act;
+    C.stop_rule = s->o.stop_rule;
+    C.mixed = (s->kind_code == 1 || s->kind_code == 2) ? 1u : 0u;
+    C.l1_tol = s->o.l1_tolerance;
+    C.refine_l1 = s->o.refine_l1;
+    PrF32Args A;
+    A.c32[0] = s->c[0];
+    A.c32[1] = s->c[1];
+    A.c16[0] = s->c16[0];
+    A.c16[1] = s->c16[1];
+    A.rank = s->rank;
+    A.outdeg = g->outdeg;
+    A.base = pr_base();
+    A.damp = pr_damp();
+    A.dn = (double)g->n;
+    A.eps = s->o.epsilon;
+    dim3 grid(g->total_blocks), block(AG_WG);
+    switch (s->kind_code) {
+    case 0: pr_f32_kernel<<<grid, block, 0, stream>>>(G, C, A); break;
+    case 1: pr_bf16_kernel<<<grid, block, 0, stream>>>(G, C, A); break;
+    case 2: pr_f16_kernel<<<grid, block, 0, stream>>>(G, C, A); break;
+    case 64: pr_fixed_kernel<64><<<grid, block, 0, stream>>>(G, C, A); break;
+    case 32: pr_fixed_kernel<32><<<grid, block, 0, stream>>>(G, C, A); break;
+    case 16: pr_fixed_kernel<16><<<grid, block, 0, stream>>>(G, C, A); break;
+    default: pr_fixed_kernel<8><<<grid, block, 0, stream>>>(G, C, A); break;
+    }
+    AG_HIP(hipGetLastError());
+    return AG_OK;
+}
+
+static uint32_t session_maxit(const ag_pr_session *s)
+{
+    return s->o.max_iterations ? s->o.max_iterations : 0;
+}
+
+extern "C" int ag_pr_step(ag_pr_session *s, uint32_t iterations)
+{
+    if (!s)
+        return ag_set_error(AG_ERR_INVALID, "session is null");
+    AG_HIP(hipSetDevice(s->g->dev->ordinal));
+    hipStream_t stream = s->g->dev->stream;
+    const uint32_t maxit = session_maxit(s);
+    s->step_launches = 0;
+    if (s->g->total_blocks == 0)
+        return AG_OK;
+    AG_HIP(hipEventRecord(s->step_ev[0], stream));
+    for (uint32_t i = 0; i < iterations && s->enqueued < maxit; i++) {
+        if (s->ev)
+            AG_HIP(hipEventRecord(s->ev[s->enqueued], stream));
+        int st = launch_iteration(s);
+        if (st != AG_OK)
+            return st;
+        s->enqueued++;
+        s->step_launches++;
+    }
+    if (s->ev)
+        AG_HIP(hipEventRecord(s->ev[s->enqueued], stream));
+    AG_HIP(hipEventRecord(s->step_ev[1], stream));
+    return AG_OK;
+}
+
+static void fill_result(const ag_pr_session *s, const PrState *st, ag_pr_result *r)
+{
+    const uint32_t maxit = session_maxit(s);
+    memset(r, 0, sizeof(*r));
+    if (s->g->n == 0) {
+        // no vertex is ever active: the reference loop breaks in its first iteration
+        bool stops = s->o.stop_rule != AG_STOP_NONE && maxit > 0;
+        r->iterations = stops ? 0 : maxit;
+        r->executed = stops ? 1 : maxit;
+        r->refine_iteration = -1;
+        return;
+    }
+    if (st->done) {
+        r->iterations = st->stop_iter;
+        r->executed = st->stop_iter + 1;
+    } else {
+        r->iterations = st->iter;
+        r->executed = st->iter;
+    }
+    r->active_vertices = st->last_active;
+    r->error_total = st->last_err;
+    r->refine_iteration = st->refine_iter;
+}
+
+extern "C" int ag_pr_poll(ag_pr_session *s, ag_pr_result *res)
+{
+    if (!s)
+        return ag_set_error(AG_ERR_INVALID, "session is null");
+    AG_HIP(hipSetDevice(s->g->dev->ordinal));
+    AG_HIP(hipMemcpyAsync(s->st_host, s->st, sizeof(PrState), hipMemcpyDeviceToHost,
+                          s->g->dev->stream));
+    int st = ag_wait_stream(s->g->dev);
+    if (st != AG_OK)
+        return st;
+    if (res)
+        fill_result(s, s->st_host, res);
+    return AG_OK;
+}
+
+extern "C" int ag_pr_last_step_kernel_ms(ag_pr_session *s, double *avg_ms, uint32_t *launches)
+{
+    if (!s || !avg_ms)
+        return ag_set_error(AG_ERR_INVALID, "null argument");
+    *avg_ms = 0.0;
+    if (launches)
+        *launches = s->step_launches;
+    if (s->step_launches == 0)
+        return AG_OK;
+    int st = ag_wait_stream(s->g->dev);
+    if (st != AG_OK)
+        return st;
+    float ms = 0.0f;
+    AG_HIP(hipEventElapsedTime(&ms, s->step_ev[0], s->step_ev[1]));
+    *avg_ms = (double)ms / (double)s->step_launches;
+    return AG_OK;
+}
+
+extern "C" int ag_pr_history(ag_pr_session *s, uint32_t cap, uint32_t *active, double *error,
+                             float *iter_ms, uint32_t *rows)
+{
+    if (!s || !rows)
+        return ag_set_error(AG_ERR_INVALID, "null argument");
+    AG_HIP(hipSetDevice(s->g->dev->ordinal));
+    ag_pr_result r;
+    int st = ag_pr_poll(s, &r);
+    if (st != AG_OK)
+        return st;
+    uint32_t k = r.executed < cap ? r.executed : cap;
+    if (s->g->n == 0)
+        k = 0;
+    *rows = k;
+    if (k == 0)
+        return AG_OK;
+    if (active)
+        AG_HIP(hipMemcpy(active, s->hist_act, (size_t)k * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (error)
+        AG_HIP(hipMemcpy(error, s->hist_err, (size_t)k * sizeof(double), hipMemcpyDeviceToHost));
+    if (iter_ms) {
+        for (uint32_t i = 0; i < k; i++) {
+            float ms = 0.0f;
+            if (s->ev && i + 1 < s->ev_count && i + 1 <= s->enqueued)
+                AG_HIP(hipEventElapsedTime(&ms, s->ev[i], s->ev[i + 1]));
+            iter_ms[i] = ms;
+        }
+    }
+    return AG_OK;
+}
+
+extern "C" int ag_pr_read_scaled(ag_pr_session *s, float *scaled_out)
+{
+    if (!s || (!scaled_out && s->g->n))
+        return ag_set_error(AG_ERR_INVALID, "null argument");
+    ag_dgraph *g = s->g;
+    if (g->n == 0)
+        return AG_OK;
+    AG_HIP(hipSetDevice(g->dev->ordinal));
+    int st = ag_ensure_vec(g, (size_t)g->n * sizeof(uint64_t));
+    if (st != AG_OK)
+        return st;
+    scaled_copy_kernel<<<(g->n + AG_WG - 1) / AG_WG, AG_WG, 0, g->dev->stream>>>(
+        g->n, s->rank, g->d2o, (float *)g->vec_a);
+    AG_HIP(hipGetLastError());
+    AG_HIP(hipMemcpyAsync(scaled_out, g->vec_a, (size_t)g->n * sizeof(float),
+                          hipMemcpyDeviceToHost, g->dev->stream));
+    return ag_wait_stream(g->dev);
+}
+
+extern "C" int ag_pr_end(ag_pr_session *s, float *ranks_out, ag_pr_result *res)
+{
+    if (!s)
+        return ag_set_error(AG_ERR_INVALID, "session is null");
+    ag_dgraph *g = s->g;
+    int st = AG_OK;
+    hipError_t e = hipSetDevice(g->dev->ordinal);
+    if (e != hipSuccess)
+        st = ag_hip_fail(e, "hipSetDevice", __FILE__, __LINE__);
+    ag_pr_result r;
+    if (st == AG_OK)
+        st = ag_pr_poll(s, &r);
+    double sum = 0.0;
+    if (st == AG_OK && g->n) {
+        st = ag_ensure_vec(g, (size_t)g->n * sizeof(uint64_t));
+        const uint32_t fblk = (g->n + AG_WG - 1) / AG_WG;
+        if (st == AG_OK) {
+            pr_finalize_kernel<<<fblk, AG_WG, 0, g->dev->stream>>>(
+                g->n, s->rank, g->d2o, (float *)g->vec_a, NULL, s->fin_part);
+            sum_partials_kernel<<<1, AG_WG, 0, g->dev->stream>>>(s->fin_part, fblk, s->fin_sum);
+            e = hipGetLastError();
+            if (e == hipSuccess && ranks_out)
+                e = hipMemcpyAsync(ranks_out, g->vec_a, (size_t)g->n * sizeof(float),
+                                   hipMemcpyDeviceToHost, g->dev->stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(&s->st_host->last_err, s->fin_sum, sizeof(double),
+                                   hipMemcpyDeviceToHost, g->dev->stream);
+            if (e != hipSuccess)
+                st = ag_hip_fail(e, "finalize", __FILE__, __LINE__);
+            if (st == AG_OK)
+                st = ag_wait_stream(g->dev);
+            if (st == AG_OK)
+                sum = s->st_host->last_err;
+        }
+    }
+    if (st == AG_OK && res) {
+        *res = r;
+        res->rank_sum = sum;
+        float ms = 0.0f;
+        if (s->ev && s->enqueued > 0) {
+            if (hipEventElapsedTime(&ms, s->ev[0], s->ev[s->enqueued]) != hipSuccess)
+                ms = 0.0f;
+        }
+        res->loop_ms = ms;
+        struct timespec t1;
+        clock_gettime(CLOCK_MONOTONIC, &t1);
+        res->total_ms = (double)(t1.tv_sec - s->t0.tv_sec) * 1e3 +
+                        (double)(t1.tv_nsec - s->t0.tv_nsec) * 1e-6;
+    }
+    session_free(s);
+    return st;
+}
+
+extern "C" int ag_pagerank_pull(ag_dgraph *g, const ag_pr_options *opts, float *ranks_out,
+                                ag_pr_result *res)
+{
+    ag_pr_options o;
+    if (opts)
+        o = *opts;
+    else
+        ag_pr_options_default(&o);
+    ag_pr_session *s = NULL;
+    int st = ag_pr_begin(g, &o, &s);
+    if (st != AG_OK)
+        return st;
+    const uint32_t maxit = o.max_iterations;
+    const uint32_t chunk = o.chunk ? o.chunk : 8;
+    ag_pr_result r;
+    memset(&r, 0, sizeof(r));
+    while (s->enqueued < maxit && g->total_blocks) {
+        uint32_t k = maxit - s->enqueued < chunk ? maxit - s->enqueued : chunk;
+        st = ag_pr_step(s, k);
+        if (st == AG_OK)
+            st = ag_pr_poll(s, &r);
+        if (st != AG_OK) {
+            session_free(s);
+            return st;
+        }
+        if (s->st_host->done)
+            break;
+    }
+    return ag_pr_end(s, ranks_out, res);
+}

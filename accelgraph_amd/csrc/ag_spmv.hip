@@ -1,0 +1,245 @@
+// ag_spmv.hip -- SpMV pull and the raw pull gather on the GPU.
+//
+// SpMV pull (02_capi_graph/src/algorithms/capi/SPMV.c:622-751): y[v] = sum over in-edges
+// (u->v) of w(u,v) * x[u]. The FPGA FloatPoint datapath multiplies in binary32 and then
+// accumulates in stream order (cu_SPMV/CSR/PULL/FloatPoint/cu/cu_sum_kernel_control.sv:
+// 345-367); the FixedPoint datapath adds (u32)(((u64)x * w) >> SCALEF) into a u64 sum with
+// SCALEF = 16 (FixedPoint/cu/cu_sum_kernel_control.sv:155-164, FixedPoint/pkg/
+// globals_cu_pkg.sv:37). Each call computes y = A x once per repetition (the RTL overwrites
+// y every run). Unweighted graphs use the constant weight 0.0001f (SPMV.c:1206).
+//
+// The raw gather is the sum kernel itself (cu_PageRank/CSR/PULL/*/cu/cu_sum_kernel_control.sv)
+// on caller-supplied per-vertex values; it exists so the gather arithmetic can be checked
+// against the reference's kernel-level known-answer fixtures.
+#include <string.h>
+
+#include "ag_internal.h"
+#include "ag_fixed_dev.h"
+#include "ag_pull_kernel.h"
+
+namespace {
+
+template <bool WEIGHTED>
+struct SpmvF32Pol {
+    using Acc = float;
+    const float *x;     // device order
+    const float *w;     // [m] or null
+    float *y;           // device order
+    __device__ __forceinline__ float edge(uint32_t j, uint32_t s) const
+    {
+        return __fmul_rn(x[s], WEIGHTED ? w[j] : 0.0001f);
+    }
+    __device__ __forceinline__ void finish(uint32_t r, float sum, Red &) const { y[r] = sum; }
+};
+
+template <bool WEIGHTED>
+struct SpmvFixPol {
+    using Acc = uint64_t;
+    const uint32_t *xq;
+    const uint32_t *wq;
+    uint32_t wconst;
+    uint64_t *y;
+    __device__ __forceinline__ uint64_t edge(uint32_t j, uint32_t s) const
+    {
+        const uint64_t wide = (uint64_t)xq[s] * (uint64_t)(WEIGHTED ? wq[j] : wconst);
+        return (uint64_t)(uint32_t)(wide >> 16);
+    }
+    __device__ __forceinline__ void finish(uint32_t r, uint64_t sum, Red &) const { y[r] = sum; }
+};
+
+template <typename T>
+__global__ void permute_in_kernel(uint32_t n, const T *orig, const uint32_t *d2o, T *dev)
+{
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n)
+        dev[r] = orig[d2o[r]];
+}
+
+template <typename T>
+__global__ void permute_out_kernel(uint32_t n, const T *dev, const uint32_t *d2o, T *orig)
+{
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n)
+        orig[d2o[r]] = dev[r];
+}
+
+__global__ void quantize_weights_kernel(uint32_t m, const float *w, uint32_t *wq)
+{
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < m)
+        wq[j] = ag_dev_float_to_fixed32(w[j]);
+}
+
+template <typename VT, typename A>
+struct Gather {
+    using Acc = A;
+    const VT *v;
+    A *out;
+    __device__ __forceinline__ A edge(uint32_t, uint32_t s) const { return (A)v[s]; }
+    __device__ __forceinline__ void finish(uint32_t r, A sum, Red &) const { out[r] = sum; }
+};
+
+} // namespace
+
+// Timed launches of one pull policy `reps` times; *avg_ms is the mean launch duration from
+// HIP events on the device stream.
+template <class P>
+static int run_pull(ag_dgraph *g, const P &pol, uint32_t reps, double *avg_ms)
+{
+    hipStream_t stream = g->dev->stream;
+    PullGeom G = ag_make_geom(g);
+    hipEvent_t e0 = NULL, e1 = NULL;
+    if (avg_ms) {
+        AG_HIP(hipEventCreate(&e0));
+        AG_HIP(hipEventCreate(&e1));
+        AG_HIP(hipEventRecord(e0, stream));
+    }
+    for (uint32_t i = 0; i < reps && g->total_blocks; i++)
+        pull_kernel<P><<<g->total_blocks, AG_WG, 0, stream>>>(G, pol);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && avg_ms)
+        e = hipEventRecord(e1, stream);
+    int st = (e == hipSuccess) ? ag_wait_stream(g->dev) : ag_hip_fail(e, "pull launch", __FILE__, __LINE__);
+    if (st == AG_OK && avg_ms) {
+        float ms = 0.0f;
+        e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess)
+            st = ag_hip_fail(e, "hipEventElapsedTime", __FILE__, __LINE__);
+        *avg_ms = reps ? (double)ms / reps : 0.0;
+    }
+    if (e0)
+        hipEventDestroy(e0);
+    if (e1)
+        hipEventDestroy(e1);
+    return st;
+}
+
+extern "C" int ag_spmv_pull(ag_dgraph *g, const float *x, float *y, uint32_t reps, double *avg_ms)
+{
+    if (!g || (g->n && (!x || !y)))
+        return ag_set_error(AG_ERR_INVALID, "null argument");
+    if (avg_ms)
+        *avg_ms = 0.0;
+    const uint32_t n = g->n;
+    if (n == 0)
+        return AG_OK;
+    if (reps == 0)
+        reps = 1;
+    AG_HIP(hipSetDevice(g->dev->ordinal));
+    int st = ag_ensure_vec(g, (size_t)n * sizeof(uint64_t));
+    if (st != AG_OK)
+        return st;
+    hipStream_t stream = g->dev->stream;
+    float *xo = (float *)g->vec_a;                    // x, original order (then y original)
+    float *xd = (float *)g->vec_b;                    // x, device order
+    float *yd = (float *)((char *)g->vec_b + (size_t)n * sizeof(float));  // y, device order
+    const uint32_t blk = (n + AG_WG - 1) / AG_WG;
+    AG_HIP(hipMemcpyAsync(xo, x, (size_t)n * sizeof(float), hipMemcpyHostToDevice, stream));
+    permute_in_kernel<float><<<blk, AG_WG, 0, stream>>>(n, xo, g->d2o, xd);
+    AG_HIP(hipGetLastError());
+    if (g->w)
+        st = run_pull(g, SpmvF32Pol<true>{xd, g->w, yd}, reps, avg_ms);
+    else
+        st = run_pull(g, SpmvF32Pol<false>{xd, NULL, yd}, reps, avg_ms);
+    if (st != AG_OK)
+        return st;
+    permute_out_kernel<float><<<blk, AG_WG, 0, stream>>>(n, yd, g->d2o, xo);
+    AG_HIP(hipGetLastError());
+    AG_HIP(hipMemcpyAsync(y, xo, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, stream));
+    return ag_wait_stream(g->dev);
+}
+
+static uint32_t host_fixed32(float x)
+{
+    double s = (double)x * 65536.0 + 0.5;
+    if (!(s >= 1.0))
+        return 0;
+    if (s >= 4294967296.0)
+        return 0xFFFFFFFFu;
+    return (uint32_t)s;
+}
+
+extern "C" int ag_spmv_pull_fixed(ag_dgraph *g, const uint32_t *xq, uint64_t *yq, uint32_t reps,
+                                  double *avg_ms)
+{
+    if (!g || (g->n && (!xq || !yq)))
+        return ag_set_error(AG_ERR_INVALID, "null argument");
+    if (avg_ms)
+        *avg_ms = 0.0;
+    const uint32_t n = g->n;
+    if (n == 0)
+        return AG_OK;
+    if (reps == 0)
+        reps = 1;
+    AG_HIP(hipSetDevice(g->dev->ordinal));
+    int st = ag_ensure_vec(g, (size_t)n * sizeof(uint64_t) * 2);
+    if (st != AG_OK)
+        return st;
+    hipStream_t stream = g->dev->stream;
+    if (g->w && !g->wq && g->m) {
+        AG_HIP(hipMalloc((void **)&g->wq, (size_t)g->m * sizeof(uint32_t)));
+        g->device_bytes += (size_t)g->m * sizeof(uint32_t);
+        quantize_weights_kernel<<<(g->m + AG_WG - 1) / AG_WG, AG_WG, 0, stream>>>(g->m, g->w, g->wq);
+        AG_HIP(hipGetLastError());
+    }
+    uint32_t *xo = (uint32_t *)g->vec_a;
+    uint64_t *yo = (uint64_t *)g->vec_a;              // reused after x is permuted
+    uint32_t *xd = (uint32_t *)g->vec_b;
+    uint64_t *yd = (uint64_t *)((char *)g->vec_b + (size_t)n * sizeof(uint64_t));
+    const uint32_t blk = (n + AG_WG - 1) / AG_WG;
+    AG_HIP(hipMemcpyAsync(xo, xq, (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    permute_in_kernel<uint32_t><<<blk, AG_WG, 0, stream>>>(n, xo, g->d2o, xd);
+    AG_HIP(hipGetLastError());
+    if (g->wq)
+        st = run_pull(g, SpmvFixPol<true>{xd, g->wq, 0u, yd}, reps, avg_ms);
+    else
+        st = run_pull(g, SpmvFixPol<false>{xd, NULL, host_fixed32(0.0001f), yd}, reps, avg_ms);
+    if (st != AG_OK)
+        return st;
+    permute_out_kernel<uint64_t><<<blk, AG_WG, 0, stream>>>(n, yd, g->d2o, yo);
+    AG_HIP(hipGetLastError());
+    AG_HIP(hipMemcpyAsync(yq, yo, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+    return ag_wait_stream(g->dev);
+}
+
+template <typename VT, typename A>
+static int gather_typed(ag_dgraph *g, const void *values, void *out)
+{
+    const uint32_t n = g->n;
+    hipStream_t stream = g->dev->stream;
+    int st = ag_ensure_vec(g, (size_t)n * sizeof(uint64_t) * 2);
+    if (st != AG_OK)
+        return st;
+    VT *vo = (VT *)g->vec_a;
+    A *ao = (A *)g->vec_a;
+    VT *vd = (VT *)g->vec_b;
+    A *ad = (A *)((char *)g->vec_b + (size_t)n * sizeof(uint64_t));
+    const uint32_t blk = (n + AG_WG - 1) / AG_WG;
+    AG_HIP(hipMemcpyAsync(vo, values, (size_t)n * sizeof(VT), hipMemcpyHostToDevice, stream));
+    permute_in_kernel<VT><<<blk, AG_WG, 0, stream>>>(n, vo, g->d2o, vd);
+    AG_HIP(hipGetLastError());
+    st = run_pull(g, Gather<VT, A>{vd, ad}, 1, NULL);
+    if (st != AG_OK)
+        return st;
+    permute_out_kernel<A><<<blk, AG_WG, 0, stream>>>(n, ad, g->d2o, ao);
+    AG_HIP(hipGetLastError());
+    AG_HIP(hipMemcpyAsync(out, ao, (size_t)n * sizeof(A), hipMemcpyDeviceToHost, stream));
+    return ag_wait_stream(g->dev);
+}
+
+extern "C" int ag_pull_gather(ag_dgraph *g, uint32_t type, const void *values, void *out)
+{
+    if (!g || (g->n && (!values || !out)))
+        return ag_set_error(AG_ERR_INVALID, "null argument");
+    if (g->n == 0)
+        return AG_OK;
+    AG_HIP(hipSetDevice(g->dev->ordinal));
+    switch (type) {
+    case AG_GATHER_F32: return gather_typed<float, float>(g, values, out);
+    case AG_GATHER_U64: return gather_typed<uint64_t, uint64_t>(g, values, out);
+    case AG_GATHER_U32: return gather_typed<uint32_t, uint64_t>(g, values, out);
+    case AG_GATHER_U16: return gather_typed<uint16_t, uint64_t>(g, values, out);
+    case AG_GATHER_U8: return gather_typed<uint8_t, uint64_t>(g, values, out);
+    default: return ag_set_error(AG_ERR_INVALID, "unknown gather type %u", type);
+    }
+}

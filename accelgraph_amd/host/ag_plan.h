@@ -1,0 +1,67 @@
+/*
+ * ag_plan.h -- host-side pull plan: the device layout of one graph.
+ *
+ * Rows (destination vertices) are grouped by in-degree into the AG_BIN_* classes and
+ * renumbered so every class is a contiguous range of "device rows":
+ *   [THREAD rows][GROUP16 rows][WAVE rows][BLOCK rows][HUB rows]
+ * Sources inside each inverse row are renumbered with the same map and keep the reference's
+ * in-row order, so a row's stream-order sum is unchanged by the renumbering. The thread bin
+ * is cut into tiles of bounded rows/in-edges (staged through LDS by one workgroup); hub rows
+ * are cut into fixed segments (one workgroup each, combined by the last one to finish).
+ */
+#ifndef AG_PLAN_H
+#define AG_PLAN_H
+
+#include <stdint.h>
+#include "accelgraph.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AG_TILE_ROWS_MAX 1024u
+#define AG_TILE_NNZ_MAX 4096u
+#define AG_WG 256u
+
+/* launch order of the bins inside one fused launch: heaviest work first */
+#define AG_NSLOTS AG_NBINS
+extern const uint32_t ag_slot_bin[AG_NSLOTS];
+
+typedef struct ag_plan {
+    uint32_t n, m;
+    ag_plan_params params;
+    uint32_t *d2o;          /* [n]   device row -> original vertex              */
+    uint32_t *o2d;          /* [n]   original vertex -> device row              */
+    uint32_t *off;          /* [n+1] device-order inverse-CSR offsets           */
+    uint32_t *src;          /* [m]   device ids of in-edge sources              */
+    float *w;               /* [m]   in-edge weights, or NULL                   */
+    uint32_t *outdeg;       /* [n]   forward out-degree, device order           */
+    uint32_t bin_row[AG_NBINS + 1];
+    uint64_t bin_edges[AG_NBINS];
+    uint32_t n_tiles;
+    uint32_t *tile_row;     /* [n_tiles+1] device row where each tile starts    */
+    uint32_t n_hubs;
+    uint32_t n_segs;
+    uint32_t *seg_row;      /* [n_segs] device row of each hub segment          */
+    uint32_t *seg_lo;       /* [n_segs] first in-edge of the segment            */
+    uint32_t *seg_hi;       /* [n_segs] one past the last in-edge               */
+    uint32_t *hub_seg0;     /* [n_hubs+1] first segment of each hub row         */
+    uint32_t bin_blocks[AG_NBINS];
+    uint32_t slot_blk0[AG_NSLOTS + 1]; /* first block of each launch slot       */
+    uint32_t total_blocks;
+} ag_plan;
+
+/* Validates the inputs (every offset/degree in range, every source < n) and builds the
+ * plan. Returns AG_OK, AG_ERR_INVALID or AG_ERR_OOM; *err receives a static message. */
+int ag_plan_build(uint32_t n, uint32_t m, const uint32_t *out_degree,
+                  const uint32_t *inv_edges_idx, const uint32_t *inv_degree,
+                  const uint32_t *inv_src, const float *inv_weight,
+                  const ag_plan_params *params, ag_plan *plan, const char **err);
+void ag_plan_free(ag_plan *plan);
+int ag_plan_params_check(const ag_plan_params *p, const char **err);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

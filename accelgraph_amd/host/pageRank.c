@@ -1,0 +1,199 @@
+/*
+ * pageRank.c -- drop-in PageRank pull entry points on struct GraphCSR.
+ *
+ * Each entry point keeps the reference's contract (02_capi_graph/src/algorithms/capi/
+ * pageRank.c): the stats record, the printed table, the reference stop rule (no vertex with
+ * |delta| >= epsilon), `iterations` as the index of the stopping iteration, ranks divided by
+ * N at the end and error_total as the last iteration's L1 delta. The gather, the damping
+ * update, the reductions and the next contribution run on the GPU (ag_pr_* in accelgraph.h);
+ * this file only drives them and prints.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "pageRank.h"
+#include "accelgraph.h"
+#include "accel_context.h"
+
+/* pageRank.c:50-77 */
+struct PageRankStats *newPageRankStatsGraphCSR(struct GraphCSR *graph)
+{
+    struct PageRankStats *stats = (struct PageRankStats *)calloc(1, sizeof(struct PageRankStats));
+    if (!stats)
+        accelgraphFail("newPageRankStatsGraphCSR", AG_ERR_OOM);
+    stats->damp = Damp;
+    stats->base_pr = (1.0f - stats->damp);
+    stats->iterations = 0;
+    stats->num_vertices = graph->num_vertices;
+    stats->time_total = 0.0;
+    stats->error_total = 0.0;
+    size_t n = graph->num_vertices ? graph->num_vertices : 1;
+    stats->realRanks = (uint32_t *)malloc(n * sizeof(uint32_t));
+    stats->pageRanks = (float *)malloc(n * sizeof(float));
+    if (!stats->realRanks || !stats->pageRanks)
+        accelgraphFail("newPageRankStatsGraphCSR", AG_ERR_OOM);
+    for (uint32_t v = 0; v < stats->num_vertices; v++) {
+        stats->pageRanks[v] = stats->base_pr;
+        stats->realRanks[v] = v;
+    }
+    return stats;
+}
+
+/* pageRank.c:170-180 */
+void freePageRankStats(struct PageRankStats *stats)
+{
+    if (stats) {
+        free(stats->realRanks);
+        free(stats->pageRanks);
+        free(stats);
+    }
+}
+
+static const char *variant_title(uint32_t kind)
+{
+    switch (kind) {
+    case AG_KIND_FIXED64: return "Starting Page Rank Pull FP_64 (tolerance/epsilon)";
+    case AG_KIND_FIXED32: return "Starting Page Rank Pull FP_32 (tolerance/epsilon)";
+    case AG_KIND_FIXED16: return "Starting Page Rank Pull FP_16 (tolerance/epsilon)";
+    case AG_KIND_FIXED8: return "Starting Page Rank Pull FP_8 (tolerance/epsilon)";
+    default: return "Starting Page Rank Pull (tolerance/epsilon)";
+    }
+}
+
+/* Shared body of the FP32 and fixed-point pulls (pageRank.c:942-1109, 1276-1990). */
+static struct PageRankStats *pagerank_pull_kind(struct Arguments *arguments,
+                                                struct GraphCSR *graph, uint32_t kind)
+{
+    struct PageRankStats *stats = newPageRankStatsGraphCSR(graph);
+
+    printf(" -----------------------------------------------------\n");
+    printf("| %-51s | \n", "               ---->>> GPU <<<----");
+    printf(" -----------------------------------------------------\n");
+    printf("| %-51s | \n", variant_title(kind));
+    printf(" -----------------------------------------------------\n");
+    printf("| %-51.13lf | \n", arguments->epsilon);
+    printf(" -----------------------------------------------------\n");
+    printf("| %-10s | %-8s | %-15s | %-9s | \n", "Iteration", "Active", "Error", "Time (S)");
+    printf(" -----------------------------------------------------\n");
+
+    ag_dgraph *dg = accelgraphUploadGraph(graph, 0);
+    ag_pr_options o;
+    ag_pr_options_default(&o);
+    o.kind = kind;
+    o.max_iterations = arguments->iterations;
+    o.epsilon = arguments->epsilon;
+    o.stop_rule = AG_STOP_ACTIVE;
+    o.init = AG_INIT_REFERENCE;
+    o.record_history = 1;
+
+    ag_pr_session *s = NULL;
+    int st = ag_pr_begin(dg, &o, &s);
+    if (st != AG_OK)
+        accelgraphFail("ag_pr_begin", st);
+    ag_pr_result r;
+    memset(&r, 0, sizeof(r));
+    uint32_t enq = 0;
+    const uint32_t chunk = 8;
+    while (enq < o.max_iterations) {
+        uint32_t k = o.max_iterations - enq < chunk ? o.max_iterations - enq : chunk;
+        st = ag_pr_step(s, k);
+        if (st == AG_OK)
+            st = ag_pr_poll(s, &r);
+        if (st != AG_OK)
+            accelgraphFail("ag_pr_step", st);
+        enq += k;
+        if (r.executed > r.iterations)
+            break; /* the stop rule fired (iterations = index of the stopping iteration) */
+    }
+    uint32_t rows = 0, cap = r.executed;
+    uint32_t *act = (uint32_t *)calloc(cap ? cap : 1, sizeof(uint32_t));
+    double *err = (double *)calloc(cap ? cap : 1, sizeof(double));
+    float *ms = (float *)calloc(cap ? cap : 1, sizeof(float));
+    if (!act || !err || !ms)
+        accelgraphFail("history buffers", AG_ERR_OOM);
+    st = ag_pr_history(s, cap, act, err, ms, &rows);
+    if (st != AG_OK)
+        accelgraphFail("ag_pr_history", st);
+    for (uint32_t i = 0; i < rows; i++)
+        printf("| %-10u | %-8u | %-15.13lf | %-9f | \n", i, act[i], err[i], ms[i] * 1e-3);
+    free(act);
+    free(err);
+    free(ms);
+
+    st = ag_pr_end(s, stats->pageRanks, &r);
+    if (st != AG_OK)
+        accelgraphFail("ag_pr_end", st);
+
+    stats->iterations = r.iterations;
+    stats->activeVertices = r.active_vertices;
+    stats->error_total = r.error_total;
+    stats->time_total = r.total_ms * 1e-3;
+
+    printf(" -----------------------------------------------------\n");
+    printf("| %-10s | %-8s | %-15s | %-9s | \n", "Iterations", "PR Sum", "Error", "Time (S)");
+    printf(" -----------------------------------------------------\n");
+    printf("| %-10u | %-8lf | %-15.13lf | %-9f | \n", stats->iterations, r.rank_sum,
+           stats->error_total, stats->time_total);
+    printf(" -----------------------------------------------------\n");
+    fflush(stdout);
+    return stats;
+}
+
+/* pageRank.c:942-1109 */
+struct PageRankStats *pageRankPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    return pagerank_pull_kind(arguments, graph, AG_KIND_F32);
+}
+
+/* pageRank.c:1276-1453 */
+struct PageRankStats *pageRankPullFixedPoint64BitGraphCSR(struct Arguments *arguments,
+                                                          struct GraphCSR *graph)
+{
+    return pagerank_pull_kind(arguments, graph, AG_KIND_FIXED64);
+}
+
+/* pageRank.c:1455-1633 */
+struct PageRankStats *pageRankPullFixedPoint32BitGraphCSR(struct Arguments *arguments,
+                                                          struct GraphCSR *graph)
+{
+    return pagerank_pull_kind(arguments, graph, AG_KIND_FIXED32);
+}
+
+/* pageRank.c:1635-1811 */
+struct PageRankStats *pageRankPullFixedPoint16BitGraphCSR(struct Arguments *arguments,
+                                                          struct GraphCSR *graph)
+{
+    return pagerank_pull_kind(arguments, graph, AG_KIND_FIXED16);
+}
+
+/* pageRank.c:1813-1990 */
+struct PageRankStats *pageRankPullFixedPoint8BitGraphCSR(struct Arguments *arguments,
+                                                         struct GraphCSR *graph)
+{
+    return pagerank_pull_kind(arguments, graph, AG_KIND_FIXED8);
+}
+
+/* pageRank.c:874-939. Only the pull directions are provided; the push, quantised and
+ * data-driven variants (pushpull 1,3-8,12,13) are outside this library. */
+struct PageRankStats *pageRankGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    switch (arguments->pushpull) {
+    case 0:
+        return pageRankPullGraphCSR(arguments, graph);
+    case 2:
+        return pageRankPullFixedPoint64BitGraphCSR(arguments, graph);
+    case 9:
+        return pageRankPullFixedPoint32BitGraphCSR(arguments, graph);
+    case 10:
+        return pageRankPullFixedPoint16BitGraphCSR(arguments, graph);
+    case 11:
+        return pageRankPullFixedPoint8BitGraphCSR(arguments, graph);
+    case 1: case 3: case 4: case 5: case 6: case 7: case 8: case 12: case 13:
+        fprintf(stderr, "[accelgraph] pageRankGraphCSR: pushpull=%u is not a pull variant "
+                        "provided by this library\n", arguments->pushpull);
+        exit(EXIT_FAILURE);
+    default:
+        return pageRankPullGraphCSR(arguments, graph);
+    }
+}

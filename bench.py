@@ -1,0 +1,163 @@
+"""PageRank-pull benchmark on one MI355X.
+
+Workload (default): PageRank pull, FP32 ranks, on a synthetic directed RMAT graph of scale 25
+(33,554,432 vertices, edge factor 16, a=0.57 b=0.19 c=0.19 d=0.05, seed 27491095, self-loops
+and duplicate edges removed). One step = one PageRank iteration over the whole graph (gather
+over the inverse CSR, damping update, L1/active reduction, next contribution), with the graph
+already resident in HBM.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 25] [--graph rmat|er]
+
+Prints one JSON line: value = GTEPS (edges / second / 1e9), the HBM roofline of the
+iteration kernel (algorithmic bytes 8M + 20N + 4 per launch over the kernel's average
+duration, measured with HIP events on the library's stream), and the CPU baseline (the
+oracle's OpenMP restatement of the reference pull, timed on this host's cores on a bounded
+number of iterations of the same graph).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "PageRank-pull GTEPS + HBM GB/s; time-to-L1≤1e-6 plain vs warm-start/batched"
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(n: int, m: int, kind: str) -> int:
+    # inv offsets 4(N+1), source ids 4M, one contribution per edge, out-degree 4N,
+    # rank read 4N, rank write 4N, contribution write 4N
+    if kind in ("bf16", "f16"):
+        return 6 * m + 18 * n + 4
+    return 8 * m + 20 * n + 4
+
+
+def cpu_baseline(g, budget_s: float) -> dict:
+    import oracle
+    threads = len(os.sched_getaffinity(0))
+    args = (g.out_degree, g.inv_idx, g.in_degree, g.inv_src)
+    t0 = time.perf_counter()
+    oracle.pagerank_pull(*args, max_iterations=1, stop_rule=2, threads=threads)
+    t1 = time.perf_counter() - t0
+    k = int(max(1, min(20, budget_s / max(t1, 1e-3))))
+    t0 = time.perf_counter()
+    oracle.pagerank_pull(*args, max_iterations=k, stop_rule=2, threads=threads)
+    dt = (time.perf_counter() - t0) / k
+    return {"value": round(g.m / dt / 1e9, 4), "unit": "GTEPS", "cores": threads,
+            "kind": "port",
+            "sample": f"{k} iterations of the OpenMP restatement (schedule(dynamic,1024)) of "
+                      f"the reference pull on the same graph, {dt * 1e3:.1f} ms/iteration",
+            "ms_per_iteration": round(dt * 1e3, 3)}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--scale", type=int, default=25)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--graph", choices=["rmat", "er"], default="rmat")
+    ap.add_argument("--storage", choices=["f32", "bf16", "f16"], default="f32")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ttl", action="store_true", help="skip the time-to-L1 run")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the graph shards "
+              "by destination rows with an all-gather of contributions per iteration; see "
+              "DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    import accelgraph_amd as ag
+
+    t0 = time.perf_counter()
+    if args.graph == "rmat":
+        g = ag.Graph.rmat(args.scale, args.edge_factor)
+        gdesc = (f"RMAT scale {args.scale}, edge factor {args.edge_factor}, a=0.57 b=0.19 "
+                 f"c=0.19 d=0.05, seed {ag.DEFAULT_SEED}, simplified")
+    else:
+        n = 1 << args.scale
+        g = ag.Graph.erdos_renyi(n, args.edge_factor * n)
+        gdesc = (f"Erdos-Renyi n=2^{args.scale}, {args.edge_factor}*n draws, seed "
+                 f"{ag.DEFAULT_SEED}, simplified")
+    n, m = g.n, g.m
+    log(f"graph built: n={n} m={m} in {time.perf_counter() - t0:.1f} s")
+
+    dev = ag.Device(0)
+    t0 = time.perf_counter()
+    dg = dev.upload(g)
+    info = dg.info()
+    log(f"uploaded + planned in {time.perf_counter() - t0:.1f} s; bins rows="
+        f"{list(info.bin_rows)} edges={list(info.bin_edges)} tiles={info.num_tiles} "
+        f"segments={info.num_segments}")
+
+    storage = {"f32": ag.STORE_F32, "bf16": ag.STORE_BF16, "f16": ag.STORE_F16}[args.storage]
+    K, W = args.steps, args.warmup
+    # fixed-iteration session: every launch is a full iteration (no early stop)
+    sess = dg.session(max_iterations=W + K, stop_rule=ag.STOP_NONE, init=ag.INIT_UNIFORM,
+                      storage=storage, refine_l1=0.0)
+    if W:
+        sess.step(W)
+    dev.sync()
+    t0 = time.perf_counter()
+    sess.step(K)
+    dev.sync()
+    wall = time.perf_counter() - t0
+    kern_ms, launches = sess.last_step_kernel_ms()
+    sess.end()
+    ms_per_step = wall * 1e3 / K
+    gteps = m / (ms_per_step * 1e-3) / 1e9
+    nbytes = algorithmic_bytes(n, m, args.storage)
+    achieved = nbytes / (kern_ms * 1e-3) / 1e9
+    log(f"{K} steps: {ms_per_step:.3f} ms/step wall, kernel avg {kern_ms:.3f} ms over "
+        f"{launches} launches, {gteps:.2f} GTEPS, {achieved:.0f} GB/s algorithmic")
+
+    out = {
+        "metric": METRIC, "value": round(gteps, 3), "unit": "GTEPS", "n_gpus": 1,
+        "steps": K, "warmup": W, "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if args.storage == "f32" else f"{args.storage}-storage/f32-accumulate",
+        "data": "synthetic",
+        "config": {"workload": f"pagerank_pull_{args.storage}_{args.graph}{args.scale}",
+                   "graph": gdesc, "num_vertices": n, "num_edges": m,
+                   "step": "one PageRank-pull iteration (gather + damping update + L1/active "
+                           "reduction + next contribution)"},
+        "hbm_gb_per_s": round(nbytes / (ms_per_step * 1e-3) / 1e9, 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "algorithmic_bytes": nbytes,
+                     "kernel_ms": round(kern_ms, 4)},
+    }
+
+    if not args.no_ttl:
+        run = dg.pagerank(max_iterations=300, stop_rule=ag.STOP_L1, l1_tolerance=1e-6,
+                          init=ag.INIT_UNIFORM, history=True)
+        out["time_to_l1_1e-6"] = {"plain_f32_ms": round(float(run.hist_ms.sum()), 3),
+                                  "iterations": run.executed,
+                                  "final_l1": run.error_total}
+        log(f"time to L1<=1e-6: {run.executed} iterations, {run.hist_ms.sum():.2f} ms")
+
+    dg.close()
+    dev.close()
+    if not args.no_cpu:
+        t0 = time.perf_counter()
+        out["cpu_baseline"] = cpu_baseline(g, args.cpu_seconds)
+        log(f"cpu baseline {out['cpu_baseline']['value']} GTEPS on "
+            f"{out['cpu_baseline']['cores']} cores ({time.perf_counter() - t0:.1f} s)")
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,209 @@
+/*
+ * accelgraph.h -- C-ABI of the MI355X (gfx950) PageRank/SpMV pull engine.
+ *
+ * This is the device boundary that replaces the reference's CAPI runtime
+ * (02_capi_graph/include/capi_utils/capienv.h:146-167: mapGraphCSRToWED, setupAFUGraphCSR,
+ * startAFU, startCU, waitAFU, releaseAFU). Where the reference maps host arrays into a
+ * work-element descriptor that an FPGA reads coherently, this layer uploads the arrays the
+ * pull path reads (forward out-degree, inverse-CSR offsets/degrees/sources and optional
+ * weights: capienv.c:837-869) into HBM once, builds a degree-binned pull plan, and runs
+ * hand-written HIP kernels on one stream.
+ *
+ * All functions return an ag_status code (0 = AG_OK); ag_last_error() gives a message for
+ * the calling thread. No torch or C++ types cross this boundary. One ag_device is driven
+ * from one host thread at a time.
+ */
+#ifndef ACCELGRAPH_H
+#define ACCELGRAPH_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AG_ABI_VERSION 1
+
+typedef enum ag_status {
+    AG_OK = 0,
+    AG_ERR_INVALID = 1,      /* bad argument or size, rejected before any device work */
+    AG_ERR_NO_DEVICE = 2,    /* no usable GPU */
+    AG_ERR_HIP = 3,          /* a HIP runtime call failed */
+    AG_ERR_OOM = 4,          /* device or host allocation failed */
+    AG_ERR_TIMEOUT = 5,      /* bounded wait on the device expired */
+    AG_ERR_UNSUPPORTED = 6   /* option not implemented for this entry point */
+} ag_status;
+
+typedef struct ag_device ag_device;     /* device context: stream, events, watchdog */
+typedef struct ag_dgraph ag_dgraph;     /* graph resident in HBM plus its pull plan */
+typedef struct ag_pr_session ag_pr_session;
+
+/* ---------------------------------------------------------------- device */
+
+int ag_device_count(int *count);
+int ag_device_open(int ordinal, ag_device **out);
+/* Fails with AG_ERR_INVALID (and closes nothing) while graphs uploaded on it are alive. */
+int ag_device_close(ag_device *dev);
+int ag_device_sync(ag_device *dev);           /* bounded by the watchdog deadline */
+const char *ag_status_string(int status);
+const char *ag_last_error(void);
+int ag_abi_version(void);
+
+/* Watchdog for waits on the device (the role of the reference's accelerator
+ * watchdog, 03_capi_integration/accelerator_verification/host/accelerator_verification.c:
+ * 86-135,289-320). run_timeout_ms bounds every wait; poll_interval_us is the sleep between
+ * polls. Defaults come from ACCELERATOR_RUN_TIMEOUT_MS / ACCELERATOR_POLL_INTERVAL_US
+ * (0 = unbounded). */
+int ag_device_set_watchdog(ag_device *dev, uint64_t run_timeout_ms, uint32_t poll_interval_us);
+
+/* ---------------------------------------------------------------- graph + plan */
+
+/* Row classes of the pull plan, by in-degree. */
+enum {
+    AG_BIN_THREAD = 0,   /* a lane per row, rows staged as LDS tiles; stream-order sums */
+    AG_BIN_GROUP16 = 1,  /* 16 lanes per row */
+    AG_BIN_WAVE = 2,     /* a 64-lane wavefront per row */
+    AG_BIN_BLOCK = 3,    /* a 256-lane workgroup per row */
+    AG_BIN_HUB = 4,      /* a row split into fixed segments, one workgroup each */
+    AG_NBINS = 5
+};
+
+typedef struct ag_plan_params {
+    uint32_t thread_max_deg;   /* in-degree <= this: AG_BIN_THREAD               */
+    uint32_t group16_max_deg;  /* <= this: AG_BIN_GROUP16                       */
+    uint32_t wave_max_deg;     /* <= this: AG_BIN_WAVE                          */
+    uint32_t block_max_deg;    /* <= this: AG_BIN_BLOCK, above: AG_BIN_HUB      */
+    uint32_t tile_rows;        /* max rows per thread-bin tile (<= 1024)        */
+    uint32_t tile_nnz;         /* max in-edges per thread-bin tile (<= 4096)    */
+    uint32_t segment_nnz;      /* in-edges per hub segment                      */
+    uint32_t order;            /* row order inside a bin: 0 = vertex id, 1 = out-degree desc */
+} ag_plan_params;
+
+void ag_plan_params_default(ag_plan_params *p);
+
+typedef struct ag_graph_info {
+    uint32_t num_vertices;
+    uint32_t num_edges;
+    uint32_t bin_rows[AG_NBINS];
+    uint64_t bin_edges[AG_NBINS];
+    uint32_t bin_blocks[AG_NBINS];
+    uint32_t num_tiles;
+    uint32_t num_segments;
+    uint32_t has_weights;
+    uint64_t device_bytes;
+    ag_plan_params params;
+} ag_graph_info;
+
+/* Upload one directed graph. Arrays are host arrays in the reference's layout:
+ *   out_degree[n]      vertices->out_degree
+ *   inv_edges_idx[n]   inverse_vertices->edges_idx
+ *   inv_degree[n]      inverse_vertices->out_degree (in-degree)
+ *   inv_src[m]         inverse_sorted_edges_array->edges_array_dest
+ *   inv_weight[m]      inverse_sorted_edges_array->edges_array_weight, or NULL
+ * params may be NULL (defaults). Validates every index on the host before upload. */
+int ag_graph_upload(ag_device *dev, uint32_t n, uint32_t m, const uint32_t *out_degree,
+                    const uint32_t *inv_edges_idx, const uint32_t *inv_degree,
+                    const uint32_t *inv_src, const float *inv_weight,
+                    const ag_plan_params *params, ag_dgraph **out);
+int ag_graph_free(ag_dgraph *g);
+/* Replace the fixed-point SpMV weights with caller-supplied Q16.16 codes (the reference
+ * builds them on the host, 02_capi_graph/src/algorithms/capi/SPMV.c:920-926). wq[] is in the
+ * caller's inverse-edge order, addressed through the same inv_edges_idx/inv_degree arrays
+ * that were uploaded. */
+int ag_graph_set_fixed_weights(ag_dgraph *g, const uint32_t *inv_edges_idx,
+                               const uint32_t *inv_degree, const uint32_t *wq);
+int ag_graph_get_info(const ag_dgraph *g, ag_graph_info *info);
+
+/* ---------------------------------------------------------------- PageRank pull */
+
+enum { AG_STOP_ACTIVE = 0,   /* reference rule: stop when no vertex has |delta| >= epsilon */
+       AG_STOP_L1 = 1,       /* stop at the first iteration with error_total <= l1_tolerance */
+       AG_STOP_NONE = 2 };   /* run max_iterations */
+
+enum { AG_INIT_REFERENCE = 0,  /* every scaled rank 1 - Damp (pageRank.c:57-58,71) */
+       AG_INIT_UNIFORM = 1,    /* every scaled rank 1.0, i.e. 1/N after normalisation */
+       AG_INIT_USER = 2 };     /* init_ranks[], scaled units, original vertex order */
+
+enum { AG_KIND_F32 = 0, AG_KIND_FIXED64 = 64, AG_KIND_FIXED32 = 32, AG_KIND_FIXED16 = 16,
+       AG_KIND_FIXED8 = 8 };
+
+enum { AG_STORE_F32 = 0, AG_STORE_BF16 = 1, AG_STORE_F16 = 2 };
+
+typedef struct ag_pr_options {
+    uint32_t kind;             /* AG_KIND_*                                                */
+    uint32_t max_iterations;
+    double epsilon;            /* per-vertex active threshold, scaled units               */
+    uint32_t stop_rule;        /* AG_STOP_*                                               */
+    double l1_tolerance;
+    uint32_t init;             /* AG_INIT_*                                               */
+    const float *init_ranks;   /* host [n] for AG_INIT_USER                               */
+    uint32_t storage;          /* AG_KIND_F32 only: contribution storage AG_STORE_*       */
+    double refine_l1;          /* 16-bit storage: switch to f32 once error_total <= this  */
+    uint32_t chunk;            /* iterations enqueued between convergence polls (0=auto)  */
+    uint32_t record_history;   /* keep per-iteration active/error/time                    */
+} ag_pr_options;
+
+void ag_pr_options_default(ag_pr_options *o);
+
+typedef struct ag_pr_result {
+    uint32_t iterations;        /* reference meaning: index of the stopping iteration, or
+                                   max_iterations when the loop ran out (pageRank.c:1021) */
+    uint32_t active_vertices;   /* active count of the last executed iteration            */
+    double error_total;         /* L1 delta of the last executed iteration, normalised    */
+    double rank_sum;            /* FP64 sum of the normalised ranks                        */
+    double loop_ms;             /* device time of the iteration loop (HIP events)          */
+    double total_ms;            /* host wall time of the whole call                        */
+    uint32_t executed;          /* iterations actually executed (= iterations + 1 on stop) */
+    int32_t refine_iteration;   /* first iteration that wrote f32 contributions, or -1     */
+} ag_pr_result;
+
+/* One-call run (reference semantics when opts is the default): ranks_out[n] receives the
+ * normalised ranks (scaled / N) in original vertex order. */
+int ag_pagerank_pull(ag_dgraph *g, const ag_pr_options *opts, float *ranks_out,
+                     ag_pr_result *res);
+
+/* Stepping interface used by drivers and benchmarks. */
+int ag_pr_begin(ag_dgraph *g, const ag_pr_options *opts, ag_pr_session **out);
+int ag_pr_step(ag_pr_session *s, uint32_t iterations);     /* enqueue, asynchronous */
+int ag_pr_poll(ag_pr_session *s, ag_pr_result *res);       /* wait and read state    */
+int ag_pr_end(ag_pr_session *s, float *ranks_out, ag_pr_result *res); /* ranks may be NULL */
+/* Per-iteration history of the last session run with record_history (up to cap rows). */
+int ag_pr_history(ag_pr_session *s, uint32_t cap, uint32_t *active, double *error,
+                  float *iter_ms, uint32_t *rows);
+/* Scaled (un-normalised) ranks in original order, without ending the session. */
+int ag_pr_read_scaled(ag_pr_session *s, float *scaled_out);
+/* Average duration (ms) of the iteration kernel over the last ag_pr_step call, measured
+ * with HIP events on the session's stream. */
+int ag_pr_last_step_kernel_ms(ag_pr_session *s, double *avg_ms, uint32_t *launches);
+
+/* ---------------------------------------------------------------- SpMV pull */
+
+/* y[v] = sum over in-edges (u->v) of fp32(w*x[u]), accumulated in fp32; w = 0.0001f when
+ * the graph has no weights. Repeats the product `repetitions` times (each overwrites y). */
+int ag_spmv_pull(ag_dgraph *g, const float *x, float *y, uint32_t repetitions,
+                 double *avg_kernel_ms);
+/* Fixed point: y_q[v] = sum of (u32)(((u64)x_q[u] * w_q) >> 16) modulo 2^64, with
+ * w_q = FloatToFixed32(w) per edge, or the codes given to ag_graph_set_fixed_weights. */
+int ag_spmv_pull_fixed(ag_dgraph *g, const uint32_t *x_q, uint64_t *y_q,
+                       uint32_t repetitions, double *avg_kernel_ms);
+
+/* ---------------------------------------------------------------- raw gather */
+
+/* The bare pull gather (the FPGA sum kernel's contract): out[v] = sum over in-edges
+ * (u->v) of values[u], values and out in original vertex order.
+ *   AG_GATHER_F32  float -> float (fp32 adds)
+ *   AG_GATHER_U64  u64 -> u64 modulo 2^64, AG_GATHER_U32/U16/U8 zero-extended -> u64 */
+enum { AG_GATHER_F32 = 0, AG_GATHER_U64 = 1, AG_GATHER_U32 = 2, AG_GATHER_U16 = 3,
+       AG_GATHER_U8 = 4 };
+int ag_pull_gather(ag_dgraph *g, uint32_t type, const void *values, void *out);
+
+/* Device-side fixed-point conversions (the same functions the kernels use), for tests. */
+int ag_fixed_encode(ag_device *dev, uint32_t bits, const float *in, uint64_t *out, uint32_t n);
+int ag_fixed_decode(ag_device *dev, uint32_t bits, const uint64_t *in, double *out, uint32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

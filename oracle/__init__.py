@@ -1,0 +1,189 @@
+"""CPU oracle: ctypes wrapper of oracle/liboracle.so (a C restatement of the reference's
+PageRank-pull / SpMV-pull path; see ag_oracle.c for the file:line it follows).
+
+TEST INFRASTRUCTURE. Imported only by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, and only as the checker / CPU baseline. The product (accelgraph_amd) never
+imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+u32, u64, f32, f64 = C.c_uint32, C.c_uint64, C.c_float, C.c_double
+P = C.POINTER
+
+
+class _PrOut(C.Structure):
+    _fields_ = [("iterations", u32), ("executed", u32), ("active", u32), ("error_total", f64),
+                ("rank_sum", f64)]
+
+
+_LIB = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.or_fixed_encode.restype = u64
+        L.or_fixed_encode.argtypes = [u32, f32]
+        L.or_fixed_decode.restype = f64
+        L.or_fixed_decode.argtypes = [u32, u64]
+        L.or_gather_f32.argtypes = [u32, P(u32), P(u32), P(u32), P(f32), P(f32)]
+        L.or_gather_u64.argtypes = [u32, P(u32), P(u32), P(u32), P(u64), P(u64)]
+        L.or_spmv_f32.argtypes = [u32, P(u32), P(u32), P(u32), P(f32), P(f32), P(f32), C.c_int]
+        L.or_spmv_fixed.argtypes = [u32, P(u32), P(u32), P(u32), P(u32), u32, P(u32), P(u64),
+                                    C.c_int]
+        L.or_pagerank_pull.restype = C.c_int
+        L.or_pagerank_pull.argtypes = [u32, P(u32), P(u32), P(u32), P(u32), u32, u32, f64, u32,
+                                       f64, u32, P(f32), C.c_int, P(f32), P(f32), P(u32), P(f64),
+                                       P(_PrOut)]
+        L.or_pagerank_f64.restype = C.c_int
+        L.or_pagerank_f64.argtypes = [u32, P(u32), P(u32), P(u32), P(u32), f64, u32, C.c_int,
+                                      P(f64)]
+        L.or_build_csr.restype = C.c_int
+        L.or_build_csr.argtypes = [u32, u64, P(u32), P(u32), P(u32), P(u32), P(u32), P(u32),
+                                   P(u32), P(u32), P(u64)]
+        L.or_max_threads.restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(P(t))
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def fixed_encode(bits: int, x) -> np.ndarray:
+    L = lib()
+    return np.array([L.or_fixed_encode(bits, float(v)) for v in np.asarray(x, np.float32)],
+                    dtype=np.uint64)
+
+
+def fixed_decode(bits: int, q) -> np.ndarray:
+    L = lib()
+    return np.array([L.or_fixed_decode(bits, int(v)) for v in np.asarray(q, np.uint64)],
+                    dtype=np.float64)
+
+
+def gather_f32(idx, deg, src, vals) -> np.ndarray:
+    idx, deg, src = _u32(idx), _u32(deg), _u32(src)
+    vals = np.ascontiguousarray(vals, np.float32)
+    out = np.zeros(idx.size, np.float32)
+    lib().or_gather_f32(idx.size, _p(idx, u32), _p(deg, u32), _p(src, u32), _p(vals, f32),
+                        _p(out, f32))
+    return out
+
+
+def gather_u64(idx, deg, src, vals) -> np.ndarray:
+    idx, deg, src = _u32(idx), _u32(deg), _u32(src)
+    vals = np.ascontiguousarray(vals, np.uint64)
+    out = np.zeros(idx.size, np.uint64)
+    lib().or_gather_u64(idx.size, _p(idx, u32), _p(deg, u32), _p(src, u32), _p(vals, u64),
+                        _p(out, u64))
+    return out
+
+
+def spmv_f32(idx, deg, src, w, x, threads: int = 0) -> np.ndarray:
+    idx, deg, src = _u32(idx), _u32(deg), _u32(src)
+    w = None if w is None else np.ascontiguousarray(w, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.zeros(idx.size, np.float32)
+    lib().or_spmv_f32(idx.size, _p(idx, u32), _p(deg, u32), _p(src, u32), _p(w, f32),
+                      _p(x, f32), _p(y, f32), threads)
+    return y
+
+
+def spmv_fixed(idx, deg, src, wq, wconst, xq, threads: int = 0) -> np.ndarray:
+    idx, deg, src = _u32(idx), _u32(deg), _u32(src)
+    wq = None if wq is None else _u32(wq)
+    xq = _u32(xq)
+    y = np.zeros(idx.size, np.uint64)
+    lib().or_spmv_fixed(idx.size, _p(idx, u32), _p(deg, u32), _p(src, u32), _p(wq, u32),
+                        wconst, _p(xq, u32), _p(y, u64), threads)
+    return y
+
+
+@dataclass
+class PrOut:
+    ranks: np.ndarray
+    scaled: np.ndarray
+    iterations: int
+    executed: int
+    active: int
+    error_total: float
+    rank_sum: float
+    hist_active: np.ndarray
+    hist_err: np.ndarray
+
+
+def pagerank_pull(outdeg, idx, deg, src, kind: int = 0, max_iterations: int = 20,
+                  epsilon: float = 1e-4, stop_rule: int = 0, l1_tolerance: float = 1e-6,
+                  init: int = 0, init_ranks=None, threads: int = 0) -> PrOut:
+    outdeg, idx, deg, src = _u32(outdeg), _u32(idx), _u32(deg), _u32(src)
+    n = idx.size
+    ir = None if init_ranks is None else np.ascontiguousarray(init_ranks, np.float32)
+    ranks = np.zeros(n, np.float32)
+    scaled = np.zeros(n, np.float32)
+    ha = np.zeros(max(max_iterations, 1), np.uint32)
+    he = np.zeros(max(max_iterations, 1), np.float64)
+    o = _PrOut()
+    st = lib().or_pagerank_pull(n, _p(outdeg, u32), _p(idx, u32), _p(deg, u32), _p(src, u32),
+                                kind, max_iterations, epsilon, stop_rule, l1_tolerance, init,
+                                _p(ir, f32), threads, _p(ranks, f32), _p(scaled, f32),
+                                _p(ha, u32), _p(he, f64), C.byref(o))
+    if st != 0:
+        raise RuntimeError(f"or_pagerank_pull failed ({st})")
+    return PrOut(ranks, scaled, o.iterations, o.executed, o.active, o.error_total, o.rank_sum,
+                 ha[:o.executed], he[:o.executed])
+
+
+def pagerank_f64(outdeg, idx, deg, src, tol: float = 1e-14, max_iterations: int = 2000,
+                 threads: int = 0) -> np.ndarray:
+    outdeg, idx, deg, src = _u32(outdeg), _u32(idx), _u32(deg), _u32(src)
+    out = np.zeros(idx.size, np.float64)
+    lib().or_pagerank_f64(idx.size, _p(outdeg, u32), _p(idx, u32), _p(deg, u32), _p(src, u32),
+                          tol, max_iterations, threads, _p(out, f64))
+    return out
+
+
+def build_csr(n: int, src, dst):
+    src, dst = _u32(src), _u32(dst)
+    m = src.size
+    outdeg = np.zeros(n, np.uint32)
+    eidx = np.zeros(n, np.uint32)
+    fdest = np.zeros(max(m, 1), np.uint32)
+    indeg = np.zeros(n, np.uint32)
+    iidx = np.zeros(n, np.uint32)
+    isrc = np.zeros(max(m, 1), np.uint32)
+    mo = C.c_uint64(0)
+    st = lib().or_build_csr(n, m, _p(src, u32), _p(dst, u32), _p(outdeg, u32), _p(eidx, u32),
+                            _p(fdest, u32), _p(indeg, u32), _p(iidx, u32), _p(isrc, u32),
+                            C.byref(mo))
+    if st != 0:
+        raise RuntimeError(f"or_build_csr failed ({st})")
+    k = mo.value
+    return dict(out_degree=outdeg, edges_idx=eidx, fwd_dest=fdest[:k], in_degree=indeg,
+                inv_idx=iidx, inv_src=isrc[:k], m=k)
+
+
+def max_threads() -> int:
+    return lib().or_max_threads()

@@ -1,0 +1,304 @@
+"""GPU parity: the HIP path through the C-ABI against the CPU oracle.
+
+Bars: integer / fixed-point results bit-exact; FP32 ranks within 1e-5 max-abs per vertex of
+the oracle's FP32 path (the reference's own FP32 arithmetic), plus the error against the FP64
+oracle, which is asserted to be of the same size as the oracle's own FP32 error.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import accelgraph_amd as ag
+import oracle
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+KAT = json.load(open(os.path.join(HERE, "golden", "rtl_gather_kat.json")))
+FP32_TOL = 1e-5
+
+# plan thresholds small enough that a scale-11..14 graph populates every bin, including
+# hub rows split over several segments
+SMALL = dict(thread_max_deg=2, group16_max_deg=6, wave_max_deg=20, block_max_deg=60,
+             tile_rows=7, tile_nnz=9, segment_nnz=256, order=0)
+PARAMS = {"default": None, "small": SMALL, "small-order1": dict(SMALL, order=1),
+          "tiles": dict(thread_max_deg=16, tile_rows=1024, tile_nnz=4096)}
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def upload(device, g, name="default", weights=False):
+    p = PARAMS[name]
+    return device.upload(g, weights=weights, params=ag.plan_params(**p) if p else None)
+
+
+def ref_pr(g, **kw):
+    return oracle.pagerank_pull(g.out_degree, g.inv_idx, g.in_degree, g.inv_src, **kw)
+
+
+def f64_ref(g):
+    return oracle.pagerank_f64(g.out_degree, g.inv_idx, g.in_degree, g.inv_src, tol=1e-15,
+                               max_iterations=3000, threads=8)
+
+
+# ------------------------------------------------------------------ conversions
+
+@pytest.mark.parametrize("bits", [64, 32, 16, 8])
+def test_fixed_conversions_bit_exact(device, bits):
+    say(f"fixed conversions bits={bits}")
+    rng = np.random.default_rng(bits)
+    xs = np.concatenate([
+        np.array([0.0, -1.0, np.nan, np.inf, 1.0, 0.5, 2.0 ** -17, 2.0 ** -18, 0.15, 0.85,
+                  65535.996, 65536.0, 1e30, 1e-30, 255.9, 256.0, 15.97, 16.0, 2.0 ** 31,
+                  2.0 ** 32], np.float32),
+        rng.random(4000).astype(np.float32) * 8,
+        (rng.random(4000) * 2.0 ** rng.integers(-40, 40, 4000)).astype(np.float32)])
+    np.testing.assert_array_equal(device.fixed_encode(bits, xs), oracle.fixed_encode(bits, xs))
+    qs = np.concatenate([np.array([0, 1, 2, 3, 65535, 65536, 2 ** 24 + 1, 2 ** 53 + 1,
+                                   2 ** 63 + 12345, 2 ** 64 - 1], np.uint64),
+                         rng.integers(0, 2 ** 63, 4000, dtype=np.uint64),
+                         rng.integers(0, 2 ** 30, 4000, dtype=np.uint64)])
+    np.testing.assert_array_equal(device.fixed_decode(bits, qs), oracle.fixed_decode(bits, qs))
+
+
+# ------------------------------------------------------------------ RTL known answers
+
+def _expanded(case, xkey):
+    """edge j of the fixture reads its own value: vertex n + j carries it"""
+    n, vals = case["n"], case[xkey]
+    m = len(vals)
+    idx = np.array(case["idx"], np.uint32)
+    deg = np.array(case["deg"], np.uint32)
+    N = n + m
+    full_idx = np.concatenate([idx, np.full(m, m, np.uint32)]) if m else idx
+    full_deg = np.concatenate([deg, np.zeros(m, np.uint32)])
+    src = (n + np.arange(m)).astype(np.uint32)
+    return N, full_idx, full_deg, src
+
+
+@pytest.mark.parametrize("pname", ["default", "small"])
+@pytest.mark.parametrize("case", [c for c in KAT["cases"] if c["op"] == "pagerank_gather"
+                                  and c["n"] > 0],
+                         ids=lambda c: f"{c['kind']}-{c['fixture']}-{c['cls']}")
+def test_gather_rtl_kat(device, case, pname):
+    say(f"gather kat {case['kind']} {case['fixture']} {case['cls']} {pname}")
+    N, idx, deg, src = _expanded(case, "values")
+    dg = device.upload(n=N, out_degree=np.zeros(N, np.uint32), inv_idx=idx, in_degree=deg,
+                       inv_src=src, params=ag.plan_params(**PARAMS[pname]) if PARAMS[pname] else None)
+    n = case["n"]
+    vals = [int(v) for v in case["values"]]
+    if case["kind"] == "float":
+        full = np.zeros(N, np.uint32)
+        full[n:] = vals
+        got = dg.gather(full.view(np.float32), ag.GATHER_F32).view(np.uint32)[:n]
+        exp = np.array([int(v) for v in case["expected"]], np.uint32)
+        if pname == "default":
+            np.testing.assert_array_equal(got, exp)  # thread bin: stream order, bit-exact
+        else:
+            ef = exp.view(np.float32)
+            gf = got.view(np.float32)
+            np.testing.assert_allclose(gf, ef, rtol=1e-6)
+    else:
+        gtype = ag.GATHER_U64 if case["kind"] == "fixed64" else ag.GATHER_U32
+        full = np.zeros(N, np.uint64 if gtype == ag.GATHER_U64 else np.uint32)
+        full[n:] = vals
+        got = dg.gather(full, gtype)[:n]
+        np.testing.assert_array_equal(got, np.array([int(v) for v in case["expected"]], np.uint64))
+    dg.close()
+
+
+@pytest.mark.parametrize("case", [c for c in KAT["cases"] if c["op"] == "spmv_gather"
+                                  and c["n"] > 0],
+                         ids=lambda c: f"{c['kind']}-{c['fixture']}-{c['cls']}")
+def test_spmv_rtl_kat(device, case):
+    say(f"spmv kat {case['kind']} {case['fixture']} {case['cls']}")
+    N, idx, deg, src = _expanded(case, "x")
+    n, m = case["n"], len(case["x"])
+    xs = [int(v) for v in case["x"]]
+    ws = [int(v) for v in case["w"]]
+    exp = np.array([int(v) for v in case["expected"]])
+    if case["kind"] == "float":
+        w = np.array(ws, np.uint32).view(np.float32)
+        dg = device.upload(n=N, out_degree=np.zeros(N, np.uint32), inv_idx=idx, in_degree=deg,
+                           inv_src=src, inv_weight=w)
+        x = np.zeros(N, np.uint32)
+        x[n:] = xs
+        y, _ = dg.spmv(x.view(np.float32))
+        np.testing.assert_array_equal(y.view(np.uint32)[:n], exp.astype(np.uint32))
+    else:
+        dg = device.upload(n=N, out_degree=np.zeros(N, np.uint32), inv_idx=idx, in_degree=deg,
+                           inv_src=src)
+        import ctypes as C
+        u32p = C.POINTER(C.c_uint32)
+        wq = np.array(ws, np.uint32)
+        st = ag.lib().ag_graph_set_fixed_weights(dg.handle, idx.ctypes.data_as(u32p),
+                                                 deg.ctypes.data_as(u32p),
+                                                 wq.ctypes.data_as(u32p))
+        assert st == 0, ag.lib().ag_last_error()
+        x = np.zeros(N, np.uint32)
+        x[n:] = xs
+        y, _ = dg.spmv_fixed(x)
+        np.testing.assert_array_equal(y[:n], exp.astype(np.uint64))
+    dg.close()
+
+
+# ------------------------------------------------------------------ gathers, every bin
+
+@pytest.mark.parametrize("pname", list(PARAMS))
+def test_gather_all_bins(device, pname):
+    say(f"gather all bins {pname}")
+    g = ag.Graph.rmat(13)
+    dg = upload(device, g, pname)
+    info = dg.info()
+    if pname.startswith("small"):
+        assert all(info.bin_rows[b] > 0 for b in range(5)) and info.num_segments > info.bin_rows[4]
+    rng = np.random.default_rng(7)
+    for gtype, vt in [(ag.GATHER_U64, np.uint64), (ag.GATHER_U32, np.uint32),
+                      (ag.GATHER_U16, np.uint16), (ag.GATHER_U8, np.uint8)]:
+        hi = np.iinfo(vt).max
+        v = rng.integers(0, hi, g.n, dtype=vt, endpoint=True)
+        exp = oracle.gather_u64(g.inv_idx, g.in_degree, g.inv_src, v.astype(np.uint64))
+        np.testing.assert_array_equal(dg.gather(v, gtype), exp)
+    v = rng.random(g.n).astype(np.float32)
+    exp = oracle.gather_f32(g.inv_idx, g.in_degree, g.inv_src, v)
+    got = dg.gather(v, ag.GATHER_F32)
+    np.testing.assert_allclose(got, exp, rtol=2e-6, atol=1e-6)
+    # thread-bin rows are summed in stream order: bit-exact
+    tb = g.in_degree <= info.params.thread_max_deg
+    np.testing.assert_array_equal(got[tb], exp[tb])
+    # deterministic from run to run
+    np.testing.assert_array_equal(dg.gather(v, ag.GATHER_F32), got)
+    dg.close()
+
+
+# ------------------------------------------------------------------ PageRank FP32
+
+@pytest.mark.parametrize("pname", list(PARAMS))
+def test_pagerank_f32_reference_rule(device, rmat16, pname):
+    say(f"pagerank f32 reference rule {pname}")
+    g = rmat16
+    dg = upload(device, g, pname)
+    run = dg.pagerank(max_iterations=100, epsilon=1e-4)
+    ref = ref_pr(g, max_iterations=100, epsilon=1e-4)
+    # Rows outside the thread bin are summed in a fixed tree order rather than the reference's
+    # sequential order. Near the end of the run the deltas of the largest ranks are a few ulps
+    # (ulp(244.0f) = 1.5e-5 against epsilon = 1e-4), so the last active vertex can flicker:
+    # the active-count rule may stop a couple of iterations apart while every rank stays
+    # within the FP32 tolerance.
+    assert abs(run.iterations - ref.iterations) <= 3
+    k = min(run.executed, ref.executed)
+    assert np.all(np.abs(run.hist_active[:k].astype(np.int64) -
+                         ref.hist_active[:k].astype(np.int64)) <= 10 + g.n // 1000)
+    err = np.abs(run.ranks - ref.ranks).max()
+    assert err <= FP32_TOL
+    np.testing.assert_allclose(run.ranks, ref.ranks, rtol=5e-5)
+    # late deltas sit at the FP32 rounding floor (single-ulp rank changes)
+    np.testing.assert_allclose(run.hist_err[:k], ref.hist_err[:k], rtol=1e-3, atol=1e-7)
+    assert abs(run.rank_sum - ref.rank_sum) < 1e-6
+    f64 = f64_ref(g)
+    gpu_err, cpu_err = np.abs(run.ranks - f64).max(), np.abs(ref.ranks - f64).max()
+    say(f"  iterations={run.iterations} maxabs_vs_cpu={err:.3e} vs_f64 gpu={gpu_err:.3e} "
+        f"cpu={cpu_err:.3e}")
+    assert gpu_err <= 2 * cpu_err + 1e-9
+    dg.close()
+
+
+def test_pagerank_f32_config1_l1_rule(device, rmat16):
+    """RMAT-16, damping 0.85, uniform 1/N start, stop at L1 <= 1e-4 or 100 iterations."""
+    say("pagerank config-1 l1 rule")
+    g = rmat16
+    dg = device.upload(g)
+    run = dg.pagerank(max_iterations=100, stop_rule=ag.STOP_L1, l1_tolerance=1e-4,
+                      init=ag.INIT_UNIFORM)
+    ref = ref_pr(g, max_iterations=100, stop_rule=1, l1_tolerance=1e-4, init=1)
+    assert run.iterations == ref.iterations and run.error_total <= 1e-4
+    assert np.abs(run.ranks - ref.ranks).max() <= FP32_TOL
+    dg.close()
+
+
+def test_pagerank_f32_user_init_and_determinism(device, rmat12):
+    say("pagerank user init")
+    g = rmat12
+    dg = upload(device, g, "small")
+    rng = np.random.default_rng(5)
+    init = (rng.random(g.n) * 2).astype(np.float32)
+    run = dg.pagerank(max_iterations=30, stop_rule=ag.STOP_NONE, init_ranks=init)
+    ref = ref_pr(g, max_iterations=30, stop_rule=2, init=2, init_ranks=init)
+    assert run.iterations == 30 == ref.iterations
+    assert np.abs(run.ranks - ref.ranks).max() <= FP32_TOL
+    run2 = dg.pagerank(max_iterations=30, stop_rule=ag.STOP_NONE, init_ranks=init)
+    np.testing.assert_array_equal(run.ranks, run2.ranks)
+    dg.close()
+
+
+# ------------------------------------------------------------------ fixed point
+
+@pytest.mark.parametrize("bits", [64, 32, 16, 8])
+@pytest.mark.parametrize("pname", ["default", "small"])
+def test_pagerank_fixed_bit_exact(device, rmat16, bits, pname):
+    say(f"pagerank fixed{bits} {pname}")
+    g = rmat16
+    dg = upload(device, g, pname)
+    run = dg.pagerank(kind=bits, max_iterations=60, epsilon=1e-4, scaled=True)
+    ref = ref_pr(g, kind=bits, max_iterations=60, epsilon=1e-4)
+    assert run.iterations == ref.iterations and run.executed == ref.executed
+    np.testing.assert_array_equal(run.hist_active, ref.hist_active)
+    np.testing.assert_array_equal(run.scaled.view(np.uint32), ref.scaled.view(np.uint32))
+    np.testing.assert_array_equal(run.ranks.view(np.uint32), ref.ranks.view(np.uint32))
+    np.testing.assert_allclose(run.hist_err, ref.hist_err, rtol=1e-9, atol=1e-15)
+    dg.close()
+
+
+# ------------------------------------------------------------------ SpMV
+
+def test_spmv_f32_and_fixed(device):
+    say("spmv rmat-14 random weights")
+    g = ag.Graph.rmat(14)
+    rng = np.random.default_rng(11)
+    fw = ((rng.integers(0, 2 ** 24, g.m) + 1) * 2.0 ** -24).astype(np.float32)  # (0, 1]
+    g.set_weights(fw)
+    x = (rng.integers(0, 2 ** 24, g.n) * 2.0 ** -24).astype(np.float32)  # [0, 1)
+    for pname in ("default", "small"):
+        dg = upload(device, g, pname, weights=True)
+        y, _ = dg.spmv(x, reps=3)
+        exp = oracle.spmv_f32(g.inv_idx, g.in_degree, g.inv_src, g.inv_weight, x)
+        np.testing.assert_allclose(y, exp, rtol=2e-6, atol=1e-6)
+        tb = g.in_degree <= dg.info().params.thread_max_deg
+        np.testing.assert_array_equal(y[tb], exp[tb])
+        xq = oracle.fixed_encode(32, x).astype(np.uint32)
+        wq = oracle.fixed_encode(32, g.inv_weight).astype(np.uint32)
+        yq, _ = dg.spmv_fixed(xq, reps=2)
+        np.testing.assert_array_equal(yq, oracle.spmv_fixed(g.inv_idx, g.in_degree, g.inv_src,
+                                                            wq, 0, xq))
+        dg.close()
+    # unweighted graph: constant weight 0.0001f
+    dg = device.upload(g)
+    y, _ = dg.spmv(x)
+    exp = oracle.spmv_f32(g.inv_idx, g.in_degree, g.inv_src, None, x)
+    np.testing.assert_allclose(y, exp, rtol=2e-6, atol=1e-9)
+    dg.close()
+
+
+# ------------------------------------------------------------------ edge cases
+
+def test_edge_cases(device):
+    say("edge cases")
+    for src, dst, n in [([], [], 1), ([0], [1], 2), ([0, 1, 2], [1, 2, 0], 3),
+                        ([1] * 5, [0] * 5, 7)]:
+        g = ag.Graph.from_edges(src, dst, n=n)
+        dg = device.upload(g)
+        run = dg.pagerank(max_iterations=100)
+        ref = ref_pr(g, max_iterations=100)
+        assert run.iterations == ref.iterations
+        np.testing.assert_array_equal(run.ranks, ref.ranks)
+        dg.close()
+    g = ag.Graph.from_edges([], [], n=0)
+    dg = device.upload(g)
+    run = dg.pagerank(max_iterations=5)
+    assert run.iterations == 0 and run.ranks.size == 0
+    dg.close()

@@ -1,0 +1,231 @@
+"""Pins the CPU oracle (oracle/ag_oracle.c) before anything is compared against it.
+
+Checks, each chosen so that a plausible slip (dropped term, wrong sign/index, transposed
+operand) fails one of them:
+  * the reference's RTL kernel known-answer fixtures (tests/golden/rtl_gather_kat.json, made
+    with exact arithmetic by tests/golden/make_rtl_kat.py) for the gather arithmetic;
+  * closed forms of PageRank on the chain / star / cycle / isolated-vertex graphs;
+  * brute-force dense FP64 power iteration (numpy matrices) on tiny random graphs;
+  * exact rational definitions of the fixed-point encoders.
+"""
+import json
+import math
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle
+from golden.make_rtl_kat import f32_round
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+KAT = json.load(open(os.path.join(HERE, "golden", "rtl_gather_kat.json")))
+
+
+def _csr(n, edges):
+    """inverse CSR in edge-list order (the RTL fixtures' own convention)"""
+    rows = [[] for _ in range(n)]
+    for s, d in edges:
+        rows[d].append(s)
+    idx, deg, src = [], [], []
+    for v in range(n):
+        idx.append(len(src))
+        deg.append(len(rows[v]))
+        src.extend(rows[v])
+    outdeg = np.zeros(n, np.uint32)
+    for s, _ in edges:
+        outdeg[s] += 1
+    return outdeg, np.array(idx, np.uint32), np.array(deg, np.uint32), np.array(src, np.uint32)
+
+
+# ------------------------------------------------------------------ RTL known answers
+
+@pytest.mark.parametrize("case", [c for c in KAT["cases"] if c["op"] == "pagerank_gather"],
+                         ids=lambda c: f"{c['kind']}-{c['fixture']}-{c['cls']}")
+def test_oracle_pagerank_gather_kat(case):
+    n = case["n"]
+    if n == 0:
+        return
+    m = len(case["values"])
+    idx = np.array(case["idx"], np.uint32)
+    deg = np.array(case["deg"], np.uint32)
+    src = np.arange(m, dtype=np.uint32)  # edge j reads its own value
+    if case["kind"] == "float":
+        vals = np.array([int(v) for v in case["values"]], np.uint32).view(np.float32)
+        got = oracle.gather_f32(idx, deg, src, vals).view(np.uint32)
+        exp = np.array([int(v) for v in case["expected"]], np.uint32)
+    else:
+        vals = np.array([int(v) for v in case["values"]], np.uint64)
+        got = oracle.gather_u64(idx, deg, src, vals)
+        exp = np.array([int(v) for v in case["expected"]], np.uint64)
+    np.testing.assert_array_equal(got, exp)
+
+
+@pytest.mark.parametrize("case", [c for c in KAT["cases"] if c["op"] == "spmv_gather"],
+                         ids=lambda c: f"{c['kind']}-{c['fixture']}-{c['cls']}")
+def test_oracle_spmv_kat(case):
+    n = case["n"]
+    if n == 0:
+        return
+    m = len(case["x"])
+    idx = np.array(case["idx"], np.uint32)
+    deg = np.array(case["deg"], np.uint32)
+    src = np.arange(m, dtype=np.uint32)
+    if case["kind"] == "float":
+        x = np.array([int(v) for v in case["x"]], np.uint32).view(np.float32)
+        w = np.array([int(v) for v in case["w"]], np.uint32).view(np.float32)
+        got = oracle.spmv_f32(idx, deg, src, w, x).view(np.uint32)
+        exp = np.array([int(v) for v in case["expected"]], np.uint32)
+    else:
+        x = np.array([int(v) for v in case["x"]], np.uint32)
+        w = np.array([int(v) for v in case["w"]], np.uint32)
+        got = oracle.spmv_fixed(idx, deg, src, w, 0, x)
+        exp = np.array([int(v) for v in case["expected"]], np.uint64)
+    np.testing.assert_array_equal(got, exp)
+
+
+def test_kat_covers_reference_run_lists():
+    pr = {(c["kind"], c["fixture"], c["cls"]) for c in KAT["cases"] if c["op"] == "pagerank_gather"}
+    # pagerank_kernel_tb.sv:617-634 drives these
+    for need in [("float", "k4", "max"), ("fixed64", "k4", "mixed"), ("quant32", "k4", "max"),
+                 ("float", "directed-chain", "mixed"), ("fixed64", "dangling-sink", "zero")]:
+        assert need in pr
+    # the FP32 overflow case saturates to +inf exactly as binary32 adds do
+    k4max = [c for c in KAT["cases"] if c["op"] == "pagerank_gather" and c["kind"] == "float"
+             and c["fixture"] == "k4" and c["cls"] == "max"][0]
+    assert all(int(e) == 0x7F800000 for e in k4max["expected"])
+
+
+# ------------------------------------------------------------------ closed forms
+
+def _run(n, edges, **kw):
+    od, idx, deg, src = _csr(n, edges)
+    return oracle.pagerank_pull(od, idx, deg, src, **kw)
+
+
+def test_chain_closed_form():
+    r = _run(4, [(0, 1), (1, 2), (2, 3)], max_iterations=100)
+    b = np.float32(1.0) - np.float32(0.85)
+    r1 = b + np.float32(0.85) * b
+    r2 = b + np.float32(0.85) * r1
+    r3 = b + np.float32(0.85) * r2
+    assert abs(float(r1) - 0.2775) < 1e-6 and abs(float(r3) - 0.47799375) < 1e-6
+    np.testing.assert_allclose(r.scaled, [b, r1, r2, r3], rtol=0, atol=0)
+    assert r.iterations == 3 and r.executed == 4 and r.active == 0
+
+
+def test_star_closed_form():
+    r = _run(5, [(0, 1), (0, 2), (0, 3), (0, 4)], max_iterations=100)
+    assert r.iterations == 1
+    np.testing.assert_allclose(r.scaled[1:], 0.15 + 0.85 * 0.15 / 4, rtol=1e-6)
+    np.testing.assert_allclose(r.scaled[0], 0.15, rtol=1e-6)
+    np.testing.assert_allclose(r.ranks, r.scaled / 5, rtol=1e-7)
+
+
+def test_cycle_closed_form():
+    # from r0 = 0.15, iteration it gives |delta| = 0.15 * 0.85^(it+1); with eps = 1e-3 the
+    # first quiet iteration is it+1 = 31, and r = 1 - 0.85^(it+2)
+    r = _run(3, [(0, 1), (1, 2), (2, 0)], max_iterations=200, epsilon=1e-3)
+    it = r.iterations
+    assert 0.15 * 0.85 ** (it + 1) < 1e-3 <= 0.15 * 0.85 ** it
+    assert it == 30
+    np.testing.assert_allclose(r.scaled, 1 - 0.85 ** (it + 2), rtol=1e-6)
+    # converged to the fixed point 1 (scaled), i.e. 1/N normalised
+    r = _run(3, [(0, 1), (1, 2), (2, 0)], max_iterations=400, epsilon=1e-9)
+    np.testing.assert_allclose(r.ranks, 1 / 3, rtol=1e-6)
+
+
+def test_isolated_and_sources_get_base():
+    r = _run(4, [(0, 1)], max_iterations=1, stop_rule=2)
+    np.testing.assert_allclose(r.scaled[[0, 2, 3]], np.float32(1) - np.float32(0.85), rtol=0)
+
+
+def test_dangling_mass_is_dropped():
+    # 0 -> 1, 1 is a sink: the rank sum stays below 1 (pageRank.c:1029-1032)
+    r = _run(2, [(0, 1)], max_iterations=100)
+    assert r.rank_sum < 1.0
+    np.testing.assert_allclose(r.scaled, [0.15, 0.15 + 0.85 * 0.15], rtol=1e-6)
+
+
+def _dense_f64(n, edges, iters, init):
+    A = np.zeros((n, n))
+    od = np.zeros(n)
+    for s, d in edges:
+        A[d, s] += 1.0
+        od[s] += 1
+    P = A * np.where(od > 0, 1.0 / np.maximum(od, 1), 0.0)[None, :]
+    r = np.full(n, init)
+    for _ in range(iters):
+        r = 0.15 + 0.85 * P @ r
+    return r
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_bruteforce_dense_power_iteration(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(2, 40))
+    m = int(rng.integers(0, 4 * n))
+    edges = sorted({(int(rng.integers(n)), int(rng.integers(n))) for _ in range(m)})
+    edges = [(s, d) for s, d in edges if s != d]
+    r = _run(n, edges, max_iterations=25, stop_rule=2)
+    dense = _dense_f64(n, edges, 25, 0.15)
+    np.testing.assert_allclose(r.scaled, dense, rtol=2e-6)
+    od, idx, deg, src = _csr(n, edges)
+    f64 = oracle.pagerank_f64(od, idx, deg, src, tol=1e-15, max_iterations=5000)
+    np.testing.assert_allclose(f64, _dense_f64(n, edges, 400, 1.0) / n, rtol=1e-9)
+
+
+def test_openmp_matches_sequential():
+    rng = np.random.default_rng(3)
+    n = 3000
+    edges = {(int(rng.integers(n)), int(rng.integers(n))) for _ in range(20000)}
+    edges = sorted((s, d) for s, d in edges if s != d)
+    od, idx, deg, src = _csr(n, edges)
+    a = oracle.pagerank_pull(od, idx, deg, src, max_iterations=50)
+    b = oracle.pagerank_pull(od, idx, deg, src, max_iterations=50, threads=4)
+    np.testing.assert_array_equal(a.ranks, b.ranks)
+    assert a.iterations == b.iterations
+    assert abs(a.error_total - b.error_total) <= 1e-12
+
+
+# ------------------------------------------------------------------ fixed point
+
+def _exact_code(x, frac, width):
+    if not (x > 0) or math.isnan(x):
+        return 0
+    if math.isinf(x):
+        return (1 << width) - 1
+    q = math.floor(Fraction(float(x)) * (1 << frac) + Fraction(1, 2))
+    return min(q, (1 << width) - 1)
+
+
+FIX_CASES = [0.0, -1.0, float("nan"), float("inf"), 1.0, 0.5, 2.0 ** -17, 2.0 ** -17 * 0.999,
+             2.0 ** -18, 0.15, 0.85, 1 / 3, 1e-7, 65535.99609375, 65536.0, 1e30, 1e-30, 255.9,
+             256.0, 15.97, 16.0, 3.0517578125e-05, 2.0 ** 31, 2.0 ** 32, 1.5 * 2 ** 31]
+
+
+@pytest.mark.parametrize("bits,frac", [(64, 32), (32, 16), (16, 8), (8, 4)])
+def test_fixed_encode_exact(bits, frac):
+    xs = np.array(FIX_CASES, np.float32)
+    rng = np.random.default_rng(bits)
+    xs = np.concatenate([xs, rng.random(500).astype(np.float32) * 4,
+                         (rng.random(200) * 2.0 ** rng.integers(-30, 40, 200)).astype(np.float32)])
+    got = oracle.fixed_encode(bits, xs)
+    exp = np.array([_exact_code(float(x), frac, bits) for x in xs], np.uint64)
+    np.testing.assert_array_equal(got, exp)
+
+
+@pytest.mark.parametrize("bits,frac", [(64, 32), (32, 16), (16, 8), (8, 4)])
+def test_fixed_decode_exact(bits, frac):
+    qs = np.array([0, 1, 2, 3, 65536, 65537, 2 ** 24 + 1, 2 ** 40 + 12345, 2 ** 63 + 1,
+                   2 ** 64 - 1], np.uint64)
+    got = oracle.fixed_decode(bits, qs)
+    for q, g in zip(qs, got):
+        exact = Fraction(int(q), 1 << frac)
+        if bits == 64:
+            assert g == float(exact)  # one round-to-nearest-even to double
+        else:
+            # round the integer to binary32 (nearest even), then scale exactly
+            f32 = np.uint32(f32_round(Fraction(int(q)))).view(np.float32)
+            assert g == float(f32) / (1 << frac)
