@@ -219,6 +219,13 @@ class Device:
     def sync(self) -> None:
         _check(lib().ag_device_sync(self._h), "ag_device_sync")
 
+    def copy_bandwidth(self, nbytes: int = 1 << 30, reps: int = 10) -> float:
+        """Measured HBM copy bandwidth in GB/s (read + write bytes)."""
+        g = C.c_double(0.0)
+        _check(lib().ag_device_copy_bandwidth(self._h, nbytes, reps, C.byref(g)),
+               "ag_device_copy_bandwidth")
+        return g.value
+
     def set_watchdog(self, run_timeout_ms: int, poll_interval_us: int = 50) -> None:
         _check(lib().ag_device_set_watchdog(self._h, run_timeout_ms, poll_interval_us),
                "ag_device_set_watchdog")

@@ -33,7 +33,7 @@ GATHER_F32, GATHER_U64, GATHER_U32, GATHER_U16, GATHER_U8 = 0, 1, 2, 3, 4
 class PlanParams(C.Structure):
     _fields_ = [("thread_max_deg", u32), ("group16_max_deg", u32), ("wave_max_deg", u32),
                 ("block_max_deg", u32), ("tile_rows", u32), ("tile_nnz", u32),
-                ("segment_nnz", u32), ("order", u32)]
+                ("segment_nnz", u32), ("order", u32), ("cold_below_bin", u32)]
 
 
 class GraphInfo(C.Structure):
@@ -115,6 +115,7 @@ SIGNATURES = {
     "ag_last_error": (C.c_char_p, []),
     "ag_abi_version": (C.c_int, []),
     "ag_device_set_watchdog": (C.c_int, [VP, u64, u32]),
+    "ag_device_copy_bandwidth": (C.c_int, [VP, u64, u32, P(f64)]),
     "ag_plan_params_default": (None, [P(PlanParams)]),
     "ag_graph_upload": (C.c_int, [VP, u32, u32, P(u32), P(u32), P(u32), P(u32), P(f32),
                                   P(PlanParams), P(VP)]),

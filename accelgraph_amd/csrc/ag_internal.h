@@ -15,6 +15,7 @@ struct ag_device {
     uint64_t run_timeout_ms;
     uint32_t poll_us;
     uint32_t live_graphs;   // graphs uploaded on this device and not yet freed
+    uint32_t num_cus;       // compute units (persistent-grid sizing)
 };
 
 // Device-resident graph: the plan arrays in HBM plus launch geometry.
@@ -41,7 +42,6 @@ struct ag_dgraph {
     uint32_t *tile_row;  // [n_tiles+1]
     uint32_t *seg_row, *seg_lo, *seg_hi;  // [n_segs]
     uint32_t *hub_seg0;  // [n_hubs+1]
-    uint32_t *hub_ticket;  // [n_hubs] zeroed; each hub's last segment resets its counter
     uint64_t *hub_part;    // [n_segs] 8-byte partial sums (float or u64 bits)
     // scratch for permuted vectors (x, y) of SpMV / gather, sized lazily
     void *vec_a;
@@ -58,12 +58,16 @@ struct PullGeom {
     const uint32_t *seg_lo;
     const uint32_t *seg_hi;
     const uint32_t *hub_seg0;
-    uint32_t *hub_ticket;
     uint64_t *hub_part;
     uint32_t n;
+    uint32_t n_hubs;
     uint32_t bin_row[AG_NBINS + 1];
     uint32_t slot_blk0[AG_NSLOTS + 1];
 };
+
+// Persistent-grid size of a pull launch: resident workgroups per CU x CUs, capped by the
+// number of work items.
+uint32_t ag_pull_grid(const ag_dgraph *g, uint32_t lds_bytes);
 
 // Device-side iteration state of a PageRank session.
 struct PrState {

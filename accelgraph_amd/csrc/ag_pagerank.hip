@@ -34,6 +34,14 @@ struct bf16s {
     uint16_t bits;
 };
 
+// Non-temporal loads of one stored contribution.
+__device__ __forceinline__ float ld_nt(const float *p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ bf16s ld_nt(const bf16s *p)
+{
+    return bf16s{__builtin_nontemporal_load((const uint16_t *)p)};
+}
+__device__ __forceinline__ _Float16 ld_nt(const _Float16 *p) { return __builtin_nontemporal_load(p); }
+
 __device__ __forceinline__ float ld_f32(float v) { return v; }
 __device__ __forceinline__ float ld_f32(bf16s v) { return __uint_as_float((uint32_t)v.bits << 16); }
 __device__ __forceinline__ float ld_f32(_Float16 v) { return (float)v; }
@@ -57,77 +65,30 @@ __device__ __forceinline__ _Float16 st_from_f32<_Float16>(float x) { return (_Fl
 
 struct PrCtl {
     PrState *st;
-    double *part_err;      // [gridDim.x]
-    uint32_t *part_act;    // [gridDim.x]
+    double *part_err;      // [nparts] per-workgroup partials of the main launch
+    uint32_t *part_act;    // [nparts]
     double *hist_err;      // [max_iterations]
     uint32_t *hist_act;    // [max_iterations]
+    uint32_t nparts;
     uint32_t stop_rule;
     uint32_t mixed;        // 16-bit storage with f32 refinement
     double l1_tol;
     double refine_l1;
 };
 
-__device__ __forceinline__ void pr_epilogue(const PrCtl &C, uint32_t iter, Red red)
+// Record iteration `iter` and decide whether the run stops (one thread of the finish kernel).
+__device__ __forceinline__ void pr_control(const PrCtl &C, uint32_t iter, double E, uint32_t A)
 {
-    __shared__ double s_e[AG_WG / AG_WAVE];
-    __shared__ uint32_t s_a[AG_WG / AG_WAVE];
-    __shared__ uint32_t s_last;
-    const uint32_t tid = threadIdx.x;
-    double e = group_reduce<double, AG_WAVE>(red.err);
-    uint32_t a = group_reduce<uint32_t, AG_WAVE>(red.active);
-    if ((tid & (AG_WAVE - 1)) == 0) {
-        s_e[tid / AG_WAVE] = e;
-        s_a[tid / AG_WAVE] = a;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        double be = s_e[0];
-        uint32_t ba = s_a[0];
-        for (uint32_t k = 1; k < AG_WG / AG_WAVE; k++) {
-            be += s_e[k];
-            ba += s_a[k];
-        }
-        __hip_atomic_store(&C.part_err[blockIdx.x], be, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&C.part_act[blockIdx.x], ba, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        uint32_t t = __hip_atomic_fetch_add(&C.st->tickets, 1u, __ATOMIC_ACQ_REL,
-                                            __HIP_MEMORY_SCOPE_AGENT);
-        s_last = (t == gridDim.x - 1) ? 1u : 0u;
-    }
-    __syncthreads();
-    if (!s_last)
-        return;
-    double te = 0.0;
-    uint32_t ta = 0;
-    for (uint32_t i = tid; i < gridDim.x; i += AG_WG) {
-        te += __hip_atomic_load(&C.part_err[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        ta += __hip_atomic_load(&C.part_act[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    te = group_reduce<double, AG_WAVE>(te);
-    ta = group_reduce<uint32_t, AG_WAVE>(ta);
-    __syncthreads();
-    if ((tid & (AG_WAVE - 1)) == 0) {
-        s_e[tid / AG_WAVE] = te;
-        s_a[tid / AG_WAVE] = ta;
-    }
-    __syncthreads();
-    if (tid != 0)
-        return;
-    double E = s_e[0];
-    uint32_t A = s_a[0];
-    for (uint32_t k = 1; k < AG_WG / AG_WAVE; k++) {
-        E += s_e[k];
-        A += s_a[k];
-    }
     PrState *st = C.st;
     C.hist_err[iter] = E;
     C.hist_act[iter] = A;
     bool can_stop = true;
     if (C.mixed) {
-        int32_t ri = st->refine_iter;
+        const int32_t ri = st->refine_iter;
         if (ri < 0) {
             // switch to f32 contributions once the L1 delta is small enough, or as soon as
             // the 16-bit iteration stops improving
-            bool stalled = iter > 0 && E >= st->last_err;
+            const bool stalled = iter > 0 && E >= st->last_err;
             if (E <= C.refine_l1 || stalled)
                 st->refine_iter = (int32_t)iter + 1;
             can_stop = false;
@@ -147,7 +108,33 @@ __device__ __forceinline__ void pr_epilogue(const PrCtl &C, uint32_t iter, Red r
     st->last_err = E;
     st->last_active = A;
     st->iter = iter + 1;
-    __hip_atomic_store(&st->tickets, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Per-workgroup partials of the main launch (plain stores; read after the launch boundary).
+__device__ __forceinline__ void pr_store_partial(const PrCtl &C, Red red)
+{
+    __shared__ double s_e[AG_WG / AG_WAVE];
+    __shared__ uint32_t s_a[AG_WG / AG_WAVE];
+    Red t = block_reduce_red<AG_WG>(red, s_e, s_a);
+    if (threadIdx.x == 0) {
+        C.part_err[blockIdx.x] = t.err;
+        C.part_act[blockIdx.x] = t.active;
+    }
+}
+
+// Finish kernel tail: hub rows already folded into `red`; add the main launch's partials in
+// index order and close the iteration.
+__device__ __forceinline__ void pr_close(const PrCtl &C, uint32_t iter, Red red)
+{
+    __shared__ double s_e[AG_FIN_WG / AG_WAVE];
+    __shared__ uint32_t s_a[AG_FIN_WG / AG_WAVE];
+    for (uint32_t i = threadIdx.x; i < C.nparts; i += AG_FIN_WG) {
+        red.err += C.part_err[i];
+        red.active += C.part_act[i];
+    }
+    Red t = block_reduce_red<AG_FIN_WG>(red, s_e, s_a);
+    if (threadIdx.x == 0)
+        pr_control(C, iter, t.err, t.active);
 }
 
 // ---------------------------------------------------------------- FP32 policy
@@ -161,17 +148,21 @@ struct PrF32Pol {
     const uint32_t *outdeg;
     float base, damp;
     double dn, eps;
+    uint32_t hot_lo;   // sources below this device id are cold: non-temporal gathers
 
-    __device__ __forceinline__ float edge(uint32_t, uint32_t s) const { return ld_f32(cin[s]); }
+    __device__ __forceinline__ float edge(uint32_t, uint32_t s) const
+    {
+        return s < hot_lo ? ld_f32(ld_nt(&cin[s])) : ld_f32(cin[s]);
+    }
     __device__ __forceinline__ void finish(uint32_t r, float sum, Red &red) const
     {
         const float nw = __fadd_rn(base, __fmul_rn(damp, sum));
-        const float old = rank[r];
-        rank[r] = nw;
+        const float old = __builtin_nontemporal_load(&rank[r]);
+        __builtin_nontemporal_store(nw, &rank[r]);
         const double e = (double)fabsf(__fsub_rn(nw, old));
         red.err += __ddiv_rn(e, dn);
         red.active += (e >= eps) ? 1u : 0u;
-        const uint32_t od = outdeg[r];
+        const uint32_t od = __builtin_nontemporal_load(&outdeg[r]);
         cout[r] = st_from_f32<WT>(od ? __fdiv_rn(nw, __uint2float_rn(od)) : 0.0f);
     }
 };
@@ -183,11 +174,37 @@ struct PrF32Args {
     const uint32_t *outdeg;
     float base, damp;
     double dn, eps;
+    uint32_t hot_lo;
 };
 
+// Runs `body(policy)` with the policy of this iteration's storage mode.
+template <typename HT, class F>
+__device__ __forceinline__ void pr_f32_dispatch(const PrState *st, uint32_t iter,
+                                                const PrF32Args &A, bool mixed, F &&body)
+{
+    const uint32_t rd = iter & 1u, wr = rd ^ 1u;
+    if (!mixed) {
+        body(PrF32Pol<float, float>{(const float *)A.c32[rd], (float *)A.c32[wr], A.rank,
+                                    A.outdeg, A.base, A.damp, A.dn, A.eps, A.hot_lo});
+        return;
+    }
+    const int32_t ri = *(volatile const int32_t *)&st->refine_iter;
+    const bool wf32 = ri >= 0 && (int32_t)iter >= ri;
+    const bool rf32 = ri >= 0 && (int32_t)iter >= ri + 1;
+    if (rf32)
+        body(PrF32Pol<float, float>{(const float *)A.c32[rd], (float *)A.c32[wr], A.rank,
+                                    A.outdeg, A.base, A.damp, A.dn, A.eps, A.hot_lo});
+    else if (wf32)
+        body(PrF32Pol<HT, float>{(const HT *)A.c16[rd], (float *)A.c32[wr], A.rank, A.outdeg,
+                                 A.base, A.damp, A.dn, A.eps, A.hot_lo});
+    else
+        body(PrF32Pol<HT, HT>{(const HT *)A.c16[rd], (HT *)A.c16[wr], A.rank, A.outdeg, A.base,
+                              A.damp, A.dn, A.eps, A.hot_lo});
+}
+
 template <typename HT>
-__device__ __forceinline__ void pr_f32_iteration(const PullGeom &G, const PrCtl &C,
-                                                 const PrF32Args &A, bool mixed)
+__device__ __forceinline__ void pr_f32_main(const PullGeom &G, const PrCtl &C,
+                                            const PrF32Args &A, bool mixed)
 {
     __shared__ float s_vals[AG_TILE_NNZ_MAX];
     __shared__ float s_wave[AG_WG / AG_WAVE];
@@ -195,44 +212,48 @@ __device__ __forceinline__ void pr_f32_iteration(const PullGeom &G, const PrCtl 
     if (*(volatile const uint32_t *)&st->done)
         return;
     const uint32_t iter = *(volatile const uint32_t *)&st->iter;
-    const uint32_t rd = iter & 1u, wr = rd ^ 1u;
     Red red{0.0, 0u};
-    if (!mixed) {
-        PrF32Pol<float, float> p{(const float *)A.c32[rd], (float *)A.c32[wr], A.rank, A.outdeg,
-                                 A.base, A.damp, A.dn, A.eps};
-        pull_body(G, p, red, s_vals, s_wave);
-    } else {
-        const int32_t ri = *(volatile const int32_t *)&st->refine_iter;
-        const bool wf32 = ri >= 0 && (int32_t)iter >= ri;
-        const bool rf32 = ri >= 0 && (int32_t)iter >= ri + 1;
-        if (rf32) {
-            PrF32Pol<float, float> p{(const float *)A.c32[rd], (float *)A.c32[wr], A.rank,
-                                     A.outdeg, A.base, A.damp, A.dn, A.eps};
-            pull_body(G, p, red, s_vals, s_wave);
-        } else if (wf32) {
-            PrF32Pol<HT, float> p{(const HT *)A.c16[rd], (float *)A.c32[wr], A.rank, A.outdeg,
-                                  A.base, A.damp, A.dn, A.eps};
-            pull_body(G, p, red, s_vals, s_wave);
-        } else {
-            PrF32Pol<HT, HT> p{(const HT *)A.c16[rd], (HT *)A.c16[wr], A.rank, A.outdeg, A.base,
-                               A.damp, A.dn, A.eps};
-            pull_body(G, p, red, s_vals, s_wave);
-        }
-    }
-    pr_epilogue(C, iter, red);
+    pr_f32_dispatch<HT>(st, iter, A, mixed,
+                        [&](const auto &p) { pull_items(G, p, red, s_vals, s_wave); });
+    pr_store_partial(C, red);
+}
+
+template <typename HT>
+__device__ __forceinline__ void pr_f32_fin(const PullGeom &G, const PrCtl &C,
+                                           const PrF32Args &A, bool mixed)
+{
+    const PrState *st = C.st;
+    if (*(volatile const uint32_t *)&st->done)
+        return;
+    const uint32_t iter = *(volatile const uint32_t *)&st->iter;
+    Red red{0.0, 0u};
+    pr_f32_dispatch<HT>(st, iter, A, mixed, [&](const auto &p) { pull_hub_finish(G, p, red); });
+    pr_close(C, iter, red);
 }
 
 __global__ __launch_bounds__(AG_WG) void pr_f32_kernel(PullGeom G, PrCtl C, PrF32Args A)
 {
-    pr_f32_iteration<float>(G, C, A, false);
+    pr_f32_main<float>(G, C, A, false);
 }
 __global__ __launch_bounds__(AG_WG) void pr_bf16_kernel(PullGeom G, PrCtl C, PrF32Args A)
 {
-    pr_f32_iteration<bf16s>(G, C, A, true);
+    pr_f32_main<bf16s>(G, C, A, true);
 }
 __global__ __launch_bounds__(AG_WG) void pr_f16_kernel(PullGeom G, PrCtl C, PrF32Args A)
 {
-    pr_f32_iteration<_Float16>(G, C, A, true);
+    pr_f32_main<_Float16>(G, C, A, true);
+}
+__global__ __launch_bounds__(AG_FIN_WG) void pr_f32_finish(PullGeom G, PrCtl C, PrF32Args A)
+{
+    pr_f32_fin<float>(G, C, A, false);
+}
+__global__ __launch_bounds__(AG_FIN_WG) void pr_bf16_finish(PullGeom G, PrCtl C, PrF32Args A)
+{
+    pr_f32_fin<bf16s>(G, C, A, true);
+}
+__global__ __launch_bounds__(AG_FIN_WG) void pr_f16_finish(PullGeom G, PrCtl C, PrF32Args A)
+{
+    pr_f32_fin<_Float16>(G, C, A, true);
 }
 
 // ---------------------------------------------------------------- fixed-point policy
@@ -291,32 +312,49 @@ struct PrFixPol {
     __device__ __forceinline__ void finish(uint32_t r, uint64_t q, Red &red) const
     {
         const float nw = fix_update<BITS>(base, damp, q);
-        const float old = rank[r];
-        rank[r] = nw;
+        const float old = __builtin_nontemporal_load(&rank[r]);
+        __builtin_nontemporal_store(nw, &rank[r]);
         const double e = (double)fabsf(__fsub_rn(nw, old));
         red.err += __ddiv_rn(e, dn);
         red.active += (e >= eps) ? 1u : 0u;
-        const uint32_t od = outdeg[r];
+        const uint32_t od = __builtin_nontemporal_load(&outdeg[r]);
         cout[r] = od ? fix_encode<BITS>(__fdiv_rn(nw, __uint2float_rn(od))) : (CT)0;
     }
 };
+
+template <int BITS>
+__device__ __forceinline__ PrFixPol<BITS> fix_policy(uint32_t iter, const PrF32Args &A)
+{
+    using CT = typename FixT<BITS>::T;
+    const uint32_t rd = iter & 1u, wr = rd ^ 1u;
+    return PrFixPol<BITS>{(const CT *)A.c32[rd], (CT *)A.c32[wr], A.rank, A.outdeg, A.base,
+                          A.damp, A.dn, A.eps};
+}
 
 template <int BITS>
 __global__ __launch_bounds__(AG_WG) void pr_fixed_kernel(PullGeom G, PrCtl C, PrF32Args A)
 {
     __shared__ uint64_t s_vals[AG_TILE_NNZ_MAX];
     __shared__ uint64_t s_wave[AG_WG / AG_WAVE];
-    using CT = typename FixT<BITS>::T;
     const PrState *st = C.st;
     if (*(volatile const uint32_t *)&st->done)
         return;
     const uint32_t iter = *(volatile const uint32_t *)&st->iter;
-    const uint32_t rd = iter & 1u, wr = rd ^ 1u;
     Red red{0.0, 0u};
-    PrFixPol<BITS> p{(const CT *)A.c32[rd], (CT *)A.c32[wr], A.rank, A.outdeg, A.base, A.damp,
-                     A.dn, A.eps};
-    pull_body(G, p, red, s_vals, s_wave);
-    pr_epilogue(C, iter, red);
+    pull_items(G, fix_policy<BITS>(iter, A), red, s_vals, s_wave);
+    pr_store_partial(C, red);
+}
+
+template <int BITS>
+__global__ __launch_bounds__(AG_FIN_WG) void pr_fixed_finish(PullGeom G, PrCtl C, PrF32Args A)
+{
+    const PrState *st = C.st;
+    if (*(volatile const uint32_t *)&st->done)
+        return;
+    const uint32_t iter = *(volatile const uint32_t *)&st->iter;
+    Red red{0.0, 0u};
+    pull_hub_finish(G, fix_policy<BITS>(iter, A), red);
+    pr_close(C, iter, red);
 }
 
 // ---------------------------------------------------------------- init / finalise
@@ -361,7 +399,7 @@ __global__ __launch_bounds__(AG_WG) void pr_finalize_kernel(uint32_t n, const fl
             scaled_orig[d2o[r]] = x;
         v = (double)y;
     }
-    v = block_reduce<double>(v, s_wave);
+    v = block_reduce<double, AG_WG>(v, s_wave);
     if (threadIdx.x == 0)
         part[blockIdx.x] = v;
 }
@@ -372,7 +410,7 @@ __global__ void sum_partials_kernel(const double *part, uint32_t count, double *
     double v = 0.0;
     for (uint32_t i = threadIdx.x; i < count; i += AG_WG)
         v += part[i];
-    v = block_reduce<double>(v, s_wave);
+    v = block_reduce<double, AG_WG>(v, s_wave);
     if (threadIdx.x == 0)
         *out = v;
 }
@@ -523,7 +561,7 @@ extern "C" int ag_pr_begin(ag_dgraph *g, const ag_pr_options *opts, ag_pr_sessio
     hipStream_t stream = g->dev->stream;
     const uint32_t n = g->n;
     const uint32_t maxit = o.max_iterations ? o.max_iterations : 1;
-    const uint32_t nblk = g->total_blocks ? g->total_blocks : 1;
+    const uint32_t nblk = ag_pull_grid(g, AG_TILE_NNZ_MAX * 4u) ? ag_pull_grid(g, AG_TILE_NNZ_MAX * 4u) : 1;
     const uint32_t fblk = (n + AG_WG - 1) / AG_WG ? (n + AG_WG - 1) / AG_WG : 1;
 
     AG_TRY(salloc(s, (void **)&s->st, sizeof(PrState)));
@@ -633,15 +671,40 @@ static int launch_iteration(ag_pr_session *s)
     A.damp = pr_damp();
     A.dn = (double)g->n;
     A.eps = s->o.epsilon;
-    dim3 grid(g->total_blocks), block(AG_WG);
+    A.hot_lo = g->params.cold_below_bin ? g->bin_row[g->params.cold_below_bin] : 0u;
+    const uint32_t lds = (s->kind_code >= 8) ? AG_TILE_NNZ_MAX * 8u : AG_TILE_NNZ_MAX * 4u;
+    const uint32_t grid = ag_pull_grid(g, lds);
+    C.nparts = grid;
+    dim3 block(AG_WG), fin(AG_FIN_WG);
     switch (s->kind_code) {
-    case 0: pr_f32_kernel<<<grid, block, 0, stream>>>(G, C, A); break;
-    case 1: pr_bf16_kernel<<<grid, block, 0, stream>>>(G, C, A); break;
-    case 2: pr_f16_kernel<<<grid, block, 0, stream>>>(G, C, A); break;
-    case 64: pr_fixed_kernel<64><<<grid, block, 0, stream>>>(G, C, A); break;
-    case 32: pr_fixed_kernel<32><<<grid, block, 0, stream>>>(G, C, A); break;
-    case 16: pr_fixed_kernel<16><<<grid, block, 0, stream>>>(G, C, A); break;
-    default: pr_fixed_kernel<8><<<grid, block, 0, stream>>>(G, C, A); break;
+    case 0:
+        pr_f32_kernel<<<grid, block, 0, stream>>>(G, C, A);
+        pr_f32_finish<<<1, fin, 0, stream>>>(G, C, A);
+        break;
+    case 1:
+        pr_bf16_kernel<<<grid, block, 0, stream>>>(G, C, A);
+        pr_bf16_finish<<<1, fin, 0, stream>>>(G, C, A);
+        break;
+    case 2:
+        pr_f16_kernel<<<grid, block, 0, stream>>>(G, C, A);
+        pr_f16_finish<<<1, fin, 0, stream>>>(G, C, A);
+        break;
+    case 64:
+        pr_fixed_kernel<64><<<grid, block, 0, stream>>>(G, C, A);
+        pr_fixed_finish<64><<<1, fin, 0, stream>>>(G, C, A);
+        break;
+    case 32:
+        pr_fixed_kernel<32><<<grid, block, 0, stream>>>(G, C, A);
+        pr_fixed_finish<32><<<1, fin, 0, stream>>>(G, C, A);
+        break;
+    case 16:
+        pr_fixed_kernel<16><<<grid, block, 0, stream>>>(G, C, A);
+        pr_fixed_finish<16><<<1, fin, 0, stream>>>(G, C, A);
+        break;
+    default:
+        pr_fixed_kernel<8><<<grid, block, 0, stream>>>(G, C, A);
+        pr_fixed_finish<8><<<1, fin, 0, stream>>>(G, C, A);
+        break;
     }
     AG_HIP(hipGetLastError());
     return AG_OK;

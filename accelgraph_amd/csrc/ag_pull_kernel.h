@@ -8,22 +8,27 @@
 // per-edge reads of auxiliary1[src] (cu_edge_data_read_command_control.sv:179-185), and on
 // the CPU the loop shape of 02_capi_graph/src/algorithms/capi/pageRank.c:2890-2907.
 //
-// One launch covers every bin; blockIdx selects the bin (heaviest first):
-//   HUB     one 256-lane workgroup per fixed segment of a hub row; the last segment of a row
-//           to finish (ticket counter) sums the segment partials in segment order
-//   BLOCK   one workgroup per row
-//   WAVE    one 64-lane wavefront per row, four rows per workgroup
-//   GROUP16 16 lanes per row, sixteen rows per workgroup
-//   THREAD  one workgroup per tile: the tile's in-edge values are gathered cooperatively
-//           (coalesced source-id reads) into LDS, then each lane sums whole rows from LDS in
-//           stream order -- the same order and rounding as the reference's sequential sum
-// Sums in the other bins use a fixed lane-strided order plus a fixed shuffle tree, so every
-// result is deterministic from run to run.
+// Work items, heaviest first (the plan's launch slots):
+//   HUB     a fixed segment of a hub row: one 256-lane workgroup writes the segment's partial
+//           sum; the finish step adds a row's partials in segment order
+//   BLOCK   one row per workgroup
+//   WAVE    one row per 64-lane wavefront, four rows per item
+//   GROUP16 one row per 16 lanes, sixteen rows per item
+//   THREAD  a tile: its in-edge values are gathered cooperatively (coalesced source-id reads)
+//           into LDS, then each lane sums whole rows from LDS in stream order -- the same order
+//           and rounding as the reference's sequential sum
+// The main kernel runs a persistent grid (compute units x resident workgroups); workgroup b
+// takes items b, b + grid, b + 2*grid, ... so the heavy items spread over the whole chip first.
+// No atomics: hub partials and per-workgroup reductions go to plain arrays that a one-workgroup
+// finish kernel reads after the launch boundary, in a fixed order, so every result is
+// deterministic from run to run. Lane partials use four independent accumulators (memory-level
+// parallelism) combined in a fixed tree.
 #pragma once
 
 #include "ag_internal.h"
 
 #define AG_WAVE 64
+#define AG_FIN_WG 1024
 
 struct Red {
     double err;
@@ -45,24 +50,33 @@ __device__ __forceinline__ Acc group_reduce(Acc v)
     return v;
 }
 
-// Sum of one Acc per lane over the workgroup; valid in thread 0.
-template <typename Acc>
-__device__ __forceinline__ Acc block_reduce(Acc v, Acc *s_wave)
+// Sum of one value per lane over an NT-lane workgroup; valid in thread 0. s_wave holds NT/64.
+template <typename T, uint32_t NT>
+__device__ __forceinline__ T block_reduce(T v, T *s_wave)
 {
     const uint32_t tid = threadIdx.x;
-    v = group_reduce<Acc, AG_WAVE>(v);
+    v = group_reduce<T, AG_WAVE>(v);
     if ((tid & (AG_WAVE - 1)) == 0)
         s_wave[tid / AG_WAVE] = v;
     __syncthreads();
-    Acc t = v;
+    T t = v;
     if (tid == 0) {
         t = s_wave[0];
 #pragma unroll
-        for (uint32_t k = 1; k < AG_WG / AG_WAVE; k++)
+        for (uint32_t k = 1; k < NT / AG_WAVE; k++)
             t = t + s_wave[k];
     }
     __syncthreads();
     return t;
+}
+
+template <uint32_t NT>
+__device__ __forceinline__ Red block_reduce_red(Red r, double *s_e, uint32_t *s_a)
+{
+    Red out;
+    out.err = block_reduce<double, NT>(r.err, s_e);
+    out.active = block_reduce<uint32_t, NT>(r.active, s_a);
+    return out;
 }
 
 template <typename Acc>
@@ -79,34 +93,55 @@ __device__ __forceinline__ float acc_from_bits<float>(uint64_t b) { return __uin
 template <>
 __device__ __forceinline__ uint64_t acc_from_bits<uint64_t>(uint64_t b) { return b; }
 
-template <class P>
-__device__ __forceinline__ void pull_hub(const PullGeom &G, const P &pol, uint32_t s, Red &red,
-                                         typename P::Acc *s_wave)
+// Streaming loads of the index arrays: read once per iteration, so they are marked
+// non-temporal to leave L2/MALL capacity to the contribution vector the gathers reuse.
+__device__ __forceinline__ uint32_t ld_stream(const uint32_t *p)
+{
+    return __builtin_nontemporal_load(p);
+}
+
+#define AG_UNROLL 4
+
+// Lane-strided partial sum of in-edges lo, lo+S, ... < hi. Each round issues the source-id
+// loads of AG_UNROLL elements, then their gathers, all predicated, so a lane has up to
+// AG_UNROLL independent gathers in flight and a short row costs one round trip, not one per
+// element. Accumulators combine in a fixed tree.
+template <class P, uint32_t S>
+__device__ __forceinline__ typename P::Acc strided_sum(const PullGeom &G, const P &pol,
+                                                        uint32_t lo, uint32_t hi)
 {
     using Acc = typename P::Acc;
-    const uint32_t tid = threadIdx.x;
-    const uint32_t row = G.seg_row[s], lo = G.seg_lo[s], hi = G.seg_hi[s];
-    Acc acc = Acc(0);
-    for (uint32_t j = lo + tid; j < hi; j += AG_WG)
-        acc = acc + pol.edge(j, G.src[j]);
-    acc = block_reduce<Acc>(acc, s_wave);
-    if (tid == 0) {
-        __hip_atomic_store(&G.hub_part[s], acc_bits<Acc>(acc), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t h = row - G.bin_row[AG_BIN_HUB];
-        const uint32_t s0 = G.hub_seg0[h], s1 = G.hub_seg0[h + 1];
-        uint32_t t = __hip_atomic_fetch_add(&G.hub_ticket[h], 1u, __ATOMIC_ACQ_REL,
-                                            __HIP_MEMORY_SCOPE_AGENT);
-        if (t == s1 - s0 - 1) {
-            Acc total = acc_from_bits<Acc>(
-                __hip_atomic_load(&G.hub_part[s0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            for (uint32_t k = s0 + 1; k < s1; k++)
-                total = total + acc_from_bits<Acc>(__hip_atomic_load(
-                                    &G.hub_part[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            __hip_atomic_store(&G.hub_ticket[h], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            pol.finish(row, total, red);
+    Acc a[AG_UNROLL];
+#pragma unroll
+    for (int u = 0; u < AG_UNROLL; u++)
+        a[u] = Acc(0);
+    for (uint32_t j = lo; j < hi; j += AG_UNROLL * S) {
+        uint32_t sidx[AG_UNROLL];
+#pragma unroll
+        for (int u = 0; u < AG_UNROLL; u++) {
+            const uint32_t jj = j + u * S;
+            sidx[u] = jj < hi ? ld_stream(&G.src[jj]) : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < AG_UNROLL; u++) {
+            const uint32_t jj = j + u * S;
+            if (jj < hi)
+                a[u] = a[u] + pol.edge(jj, sidx[u]);
         }
     }
+    return (a[0] + a[1]) + (a[2] + a[3]);
+}
+
+template <class P>
+__device__ __forceinline__ void pull_hub_segment(const PullGeom &G, const P &pol, uint32_t s,
+                                                 typename P::Acc *s_wave)
+{
+    using Acc = typename P::Acc;
+    const uint32_t lo = G.seg_lo[s], hi = G.seg_hi[s];
+    Acc acc = strided_sum<P, AG_WG>(G, pol, lo + threadIdx.x, hi);
+    acc = block_reduce<Acc, AG_WG>(acc, s_wave);
+    if (threadIdx.x == 0)
+        G.hub_part[s] = acc_bits<Acc>(acc);
 }
 
 template <class P>
@@ -114,31 +149,25 @@ __device__ __forceinline__ void pull_block_row(const PullGeom &G, const P &pol, 
                                                Red &red, typename P::Acc *s_wave)
 {
     using Acc = typename P::Acc;
-    const uint32_t tid = threadIdx.x;
     const uint32_t lo = G.off[row], hi = G.off[row + 1];
-    Acc acc = Acc(0);
-    for (uint32_t j = lo + tid; j < hi; j += AG_WG)
-        acc = acc + pol.edge(j, G.src[j]);
-    acc = block_reduce<Acc>(acc, s_wave);
-    if (tid == 0)
+    Acc acc = strided_sum<P, AG_WG>(G, pol, lo + threadIdx.x, hi);
+    acc = block_reduce<Acc, AG_WG>(acc, s_wave);
+    if (threadIdx.x == 0)
         pol.finish(row, acc, red);
 }
 
-template <class P, int GS>
+template <class P, uint32_t GS>
 __device__ __forceinline__ void pull_group_rows(const PullGeom &G, const P &pol, uint32_t bin,
-                                                uint32_t blk, Red &red)
+                                                uint32_t item, Red &red)
 {
     using Acc = typename P::Acc;
     const uint32_t tid = threadIdx.x;
-    const uint32_t row = G.bin_row[bin] + blk * (AG_WG / GS) + tid / GS;
+    const uint32_t row = G.bin_row[bin] + item * (AG_WG / GS) + tid / GS;
     const uint32_t lane = tid % GS;
     const bool valid = row < G.bin_row[bin + 1];
     Acc acc = Acc(0);
-    if (valid) {
-        const uint32_t lo = G.off[row], hi = G.off[row + 1];
-        for (uint32_t j = lo + lane; j < hi; j += GS)
-            acc = acc + pol.edge(j, G.src[j]);
-    }
+    if (valid)
+        acc = strided_sum<P, GS>(G, pol, G.off[row] + lane, G.off[row + 1]);
     acc = group_reduce<Acc, GS>(acc);
     if (valid && lane == 0)
         pol.finish(row, acc, red);
@@ -153,9 +182,19 @@ __device__ __forceinline__ void pull_thread_tile(const PullGeom &G, const P &pol
     const uint32_t r0 = G.tile_row[tile], r1 = G.tile_row[tile + 1];
     const uint32_t e0 = G.off[r0], e1 = G.off[r1];
     const uint32_t nnz = e1 - e0;  // <= AG_TILE_NNZ_MAX, guaranteed by the plan
-    for (uint32_t k = tid; k < nnz; k += AG_WG) {
-        const uint32_t j = e0 + k;
-        s_vals[k] = pol.edge(j, G.src[j]);
+    for (uint32_t k = tid; k < nnz; k += AG_UNROLL * AG_WG) {
+        uint32_t sidx[AG_UNROLL];
+#pragma unroll
+        for (int u = 0; u < AG_UNROLL; u++) {
+            const uint32_t kk = k + u * AG_WG;
+            sidx[u] = kk < nnz ? ld_stream(&G.src[e0 + kk]) : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < AG_UNROLL; u++) {
+            const uint32_t kk = k + u * AG_WG;
+            if (kk < nnz)
+                s_vals[kk] = pol.edge(e0 + kk, sidx[u]);
+        }
     }
     __syncthreads();
     for (uint32_t r = r0 + tid; r < r1; r += AG_WG) {
@@ -163,37 +202,61 @@ __device__ __forceinline__ void pull_thread_tile(const PullGeom &G, const P &pol
         Acc acc = Acc(0);
         if (e > b) {
             acc = s_vals[b];  // the first element starts the sum, as the FPGA accumulator does
-            for (uint32_t k = b + 1; k < e; k++)
-                acc = acc + s_vals[k];
+            for (uint32_t q = b + 1; q < e; q++)
+                acc = acc + s_vals[q];
         }
         pol.finish(r, acc, red);
     }
+    __syncthreads();  // s_vals is reused by this workgroup's next tile
 }
 
-// The bin body of one workgroup.
+// All work items of one workgroup of the persistent grid.
 template <class P>
-__device__ __forceinline__ void pull_body(const PullGeom &G, const P &pol, Red &red,
-                                          typename P::Acc *s_vals, typename P::Acc *s_wave)
+__device__ __forceinline__ void pull_items(const PullGeom &G, const P &pol, Red &red,
+                                           typename P::Acc *s_vals, typename P::Acc *s_wave)
 {
-    const uint32_t b = blockIdx.x;
-    if (b < G.slot_blk0[1])
-        pull_hub(G, pol, b - G.slot_blk0[0], red, s_wave);
-    else if (b < G.slot_blk0[2])
-        pull_block_row(G, pol, G.bin_row[AG_BIN_BLOCK] + (b - G.slot_blk0[1]), red, s_wave);
-    else if (b < G.slot_blk0[3])
-        pull_group_rows<P, AG_WAVE>(G, pol, AG_BIN_WAVE, b - G.slot_blk0[2], red);
-    else if (b < G.slot_blk0[4])
-        pull_group_rows<P, 16>(G, pol, AG_BIN_GROUP16, b - G.slot_blk0[3], red);
-    else
-        pull_thread_tile(G, pol, b - G.slot_blk0[4], red, s_vals);
+    const uint32_t total = G.slot_blk0[AG_NSLOTS];
+    for (uint32_t it = blockIdx.x; it < total; it += gridDim.x) {
+        if (it < G.slot_blk0[1])
+            pull_hub_segment(G, pol, it - G.slot_blk0[0], s_wave);
+        else if (it < G.slot_blk0[2])
+            pull_block_row(G, pol, G.bin_row[AG_BIN_BLOCK] + (it - G.slot_blk0[1]), red, s_wave);
+        else if (it < G.slot_blk0[3])
+            pull_group_rows<P, AG_WAVE>(G, pol, AG_BIN_WAVE, it - G.slot_blk0[2], red);
+        else if (it < G.slot_blk0[4])
+            pull_group_rows<P, 16>(G, pol, AG_BIN_GROUP16, it - G.slot_blk0[3], red);
+        else
+            pull_thread_tile(G, pol, it - G.slot_blk0[4], red, s_vals);
+    }
 }
 
-// Plain pull launch (SpMV, raw gather): no per-iteration control.
+// Finish step of the hub rows: add each row's segment partials in segment order.
+template <class P>
+__device__ __forceinline__ void pull_hub_finish(const PullGeom &G, const P &pol, Red &red)
+{
+    using Acc = typename P::Acc;
+    for (uint32_t h = threadIdx.x; h < G.n_hubs; h += blockDim.x) {
+        const uint32_t s0 = G.hub_seg0[h], s1 = G.hub_seg0[h + 1];
+        Acc total = acc_from_bits<Acc>(G.hub_part[s0]);
+        for (uint32_t k = s0 + 1; k < s1; k++)
+            total = total + acc_from_bits<Acc>(G.hub_part[k]);
+        pol.finish(G.bin_row[AG_BIN_HUB] + h, total, red);
+    }
+}
+
+// Plain pull (SpMV, raw gather): main persistent launch plus the hub finish.
 template <class P>
 __global__ __launch_bounds__(AG_WG) void pull_kernel(PullGeom G, P pol)
 {
     __shared__ typename P::Acc s_vals[AG_TILE_NNZ_MAX];
     __shared__ typename P::Acc s_wave[AG_WG / AG_WAVE];
     Red red{0.0, 0u};
-    pull_body(G, pol, red, s_vals, s_wave);
+    pull_items(G, pol, red, s_vals, s_wave);
+}
+
+template <class P>
+__global__ __launch_bounds__(AG_FIN_WG) void pull_finish_kernel(PullGeom G, P pol)
+{
+    Red red{0.0, 0u};
+    pull_hub_finish(G, pol, red);
 }

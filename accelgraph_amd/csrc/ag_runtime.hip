@@ -96,6 +96,13 @@ extern "C" int ag_device_open(int ordinal, ag_device **out)
     d->ordinal = ordinal;
     d->run_timeout_ms = env_u64("ACCELERATOR_RUN_TIMEOUT_MS", 0);
     d->poll_us = (uint32_t)env_u64("ACCELERATOR_POLL_INTERVAL_US", 50);
+    int cus = 0;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ordinal);
+    if (e != hipSuccess || cus <= 0) {
+        free(d);
+        return ag_hip_fail(e, "hipDeviceGetAttribute(CU count)", __FILE__, __LINE__);
+    }
+    d->num_cus = (uint32_t)cus;
     e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         free(d);
@@ -198,8 +205,7 @@ extern "C" int ag_graph_free(ag_dgraph *g)
     hipSetDevice(g->dev->ordinal);
     hipStreamSynchronize(g->dev->stream);
     void *ptrs[] = {g->d2o, g->off, g->src, g->w, g->wq, g->outdeg, g->tile_row, g->seg_row,
-                    g->seg_lo, g->seg_hi, g->hub_seg0, g->hub_ticket, g->hub_part, g->vec_a,
-                    g->vec_b};
+                    g->seg_lo, g->seg_hi, g->hub_seg0, g->hub_part, g->vec_a, g->vec_b};
     for (void *p : ptrs)
         if (p)
             hipFree(p);
@@ -220,14 +226,25 @@ PullGeom ag_make_geom(const ag_dgraph *g)
     G.seg_lo = g->seg_lo;
     G.seg_hi = g->seg_hi;
     G.hub_seg0 = g->hub_seg0;
-    G.hub_ticket = g->hub_ticket;
     G.hub_part = g->hub_part;
     G.n = g->n;
+    G.n_hubs = g->n_hubs;
     for (int b = 0; b <= AG_NBINS; b++)
         G.bin_row[b] = g->bin_row[b];
     for (int s = 0; s <= AG_NSLOTS; s++)
         G.slot_blk0[s] = g->slot_blk0[s];
     return G;
+}
+
+uint32_t ag_pull_grid(const ag_dgraph *g, uint32_t lds_bytes)
+{
+    // 160 KB of LDS and 8 four-wave workgroups (32 waves) per CU
+    uint32_t by_lds = lds_bytes ? (160u * 1024u) / lds_bytes : 8u;
+    uint32_t per_cu = by_lds < 8u ? by_lds : 8u;
+    if (per_cu == 0)
+        per_cu = 1;
+    uint32_t grid = g->dev->num_cus * per_cu;
+    return grid < g->total_blocks ? grid : g->total_blocks;
 }
 
 // vec_a / vec_b: two scratch arrays of `bytes` each, used to permute SpMV/gather vectors
@@ -306,16 +323,11 @@ extern "C" int ag_graph_upload(ag_device *dev, uint32_t n, uint32_t m, const uin
     if (st == AG_OK) st = dev_upload(g, &g->seg_hi, plan.seg_hi, plan.n_segs);
     if (st == AG_OK) st = dev_upload(g, &g->hub_seg0, plan.hub_seg0, (size_t)plan.n_hubs + 1);
     if (st == AG_OK) {
-        size_t hb = (size_t)(plan.n_hubs ? plan.n_hubs : 1) * sizeof(uint32_t);
         size_t pb = (size_t)(plan.n_segs ? plan.n_segs : 1) * sizeof(uint64_t);
-        he = hipMalloc((void **)&g->hub_ticket, hb);
-        if (he == hipSuccess)
-            he = hipMemsetAsync(g->hub_ticket, 0, hb, dev->stream);
-        if (he == hipSuccess)
-            he = hipMalloc((void **)&g->hub_part, pb);
+        he = hipMalloc((void **)&g->hub_part, pb);
         if (he != hipSuccess)
             st = ag_hip_fail(he, "hub scratch allocation", __FILE__, __LINE__);
-        g->device_bytes += hb + pb;
+        g->device_bytes += pb;
     }
     // the host plan arrays must outlive the asynchronous copies
     if (st == AG_OK)
@@ -383,6 +395,64 @@ extern "C" int ag_graph_get_info(const ag_dgraph *g, ag_graph_info *info)
     info->device_bytes = g->device_bytes;
     info->params = g->params;
     return AG_OK;
+}
+
+// ------------------------------------------------------------------ copy bandwidth
+
+typedef uint32_t ag_u32x4 __attribute__((ext_vector_type(4)));
+
+__global__ void ag_copy_kernel(const ag_u32x4 *__restrict__ in, ag_u32x4 *__restrict__ out,
+                               uint64_t n16)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+        __builtin_nontemporal_store(__builtin_nontemporal_load(&in[i]), &out[i]);
+}
+
+extern "C" int ag_device_copy_bandwidth(ag_device *dev, uint64_t bytes, uint32_t reps, double *gbps)
+{
+    if (!dev || !gbps || bytes < 16 || reps == 0)
+        return ag_set_error(AG_ERR_INVALID, "bad argument");
+    AG_HIP(hipSetDevice(dev->ordinal));
+    const uint64_t n16 = bytes / 16;
+    ag_u32x4 *a = NULL, *b = NULL;
+    AG_HIP(hipMalloc((void **)&a, n16 * 16));
+    hipError_t e = hipMalloc((void **)&b, n16 * 16);
+    hipEvent_t e0 = NULL, e1 = NULL;
+    if (e == hipSuccess)
+        e = hipMemsetAsync(a, 1, n16 * 16, dev->stream);
+    if (e == hipSuccess)
+        e = hipEventCreate(&e0);
+    if (e == hipSuccess)
+        e = hipEventCreate(&e1);
+    const uint32_t grid = dev->num_cus * 8;
+    if (e == hipSuccess) {
+        ag_copy_kernel<<<grid, 256, 0, dev->stream>>>(a, b, n16);  // warm-up
+        e = hipEventRecord(e0, dev->stream);
+    }
+    for (uint32_t r = 0; r < reps && e == hipSuccess; r++) {
+        ag_copy_kernel<<<grid, 256, 0, dev->stream>>>(a, b, n16);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipEventRecord(e1, dev->stream);
+    int st = (e == hipSuccess) ? ag_wait_stream(dev) : ag_hip_fail(e, "copy bandwidth", __FILE__, __LINE__);
+    if (st == AG_OK) {
+        float ms = 0.0f;
+        e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess)
+            st = ag_hip_fail(e, "hipEventElapsedTime", __FILE__, __LINE__);
+        else
+            *gbps = (double)(2 * n16 * 16) * reps / ((double)ms * 1e-3) / 1e9;
+    }
+    if (e0)
+        hipEventDestroy(e0);
+    if (e1)
+        hipEventDestroy(e1);
+    hipFree(a);
+    if (b)
+        hipFree(b);
+    return st;
 }
 
 // ------------------------------------------------------------------ fixed-point test hooks

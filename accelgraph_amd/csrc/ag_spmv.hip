@@ -27,7 +27,7 @@ struct SpmvF32Pol {
     float *y;           // device order
     __device__ __forceinline__ float edge(uint32_t j, uint32_t s) const
     {
-        return __fmul_rn(x[s], WEIGHTED ? w[j] : 0.0001f);
+        return __fmul_rn(x[s], WEIGHTED ? __builtin_nontemporal_load(&w[j]) : 0.0001f);
     }
     __device__ __forceinline__ void finish(uint32_t r, float sum, Red &) const { y[r] = sum; }
 };
@@ -41,7 +41,8 @@ struct SpmvFixPol {
     uint64_t *y;
     __device__ __forceinline__ uint64_t edge(uint32_t j, uint32_t s) const
     {
-        const uint64_t wide = (uint64_t)xq[s] * (uint64_t)(WEIGHTED ? wq[j] : wconst);
+        const uint64_t wide =
+            (uint64_t)xq[s] * (uint64_t)(WEIGHTED ? __builtin_nontemporal_load(&wq[j]) : wconst);
         return (uint64_t)(uint32_t)(wide >> 16);
     }
     __device__ __forceinline__ void finish(uint32_t r, uint64_t sum, Red &) const { y[r] = sum; }
@@ -94,8 +95,12 @@ static int run_pull(ag_dgraph *g, const P &pol, uint32_t reps, double *avg_ms)
         AG_HIP(hipEventCreate(&e1));
         AG_HIP(hipEventRecord(e0, stream));
     }
-    for (uint32_t i = 0; i < reps && g->total_blocks; i++)
-        pull_kernel<P><<<g->total_blocks, AG_WG, 0, stream>>>(G, pol);
+    const uint32_t grid = ag_pull_grid(g, AG_TILE_NNZ_MAX * (uint32_t)sizeof(typename P::Acc));
+    for (uint32_t i = 0; i < reps && g->total_blocks; i++) {
+        pull_kernel<P><<<grid, AG_WG, 0, stream>>>(G, pol);
+        if (g->n_hubs)
+            pull_finish_kernel<P><<<1, AG_FIN_WG, 0, stream>>>(G, pol);
+    }
     hipError_t e = hipGetLastError();
     if (e == hipSuccess && avg_ms)
         e = hipEventRecord(e1, stream);

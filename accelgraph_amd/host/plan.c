@@ -24,6 +24,7 @@ void ag_plan_params_default(ag_plan_params *p)
     p->tile_nnz = 4096;
     p->segment_nnz = 16384;
     p->order = 0;
+    p->cold_below_bin = 0;
 }
 
 int ag_plan_params_check(const ag_plan_params *p, const char **err)
@@ -47,6 +48,10 @@ int ag_plan_params_check(const ag_plan_params *p, const char **err)
     }
     if (p->segment_nnz < AG_WG || p->segment_nnz > (1u << 24)) {
         *err = "segment_nnz must be in [256, 2^24]";
+        return AG_ERR_INVALID;
+    }
+    if (p->cold_below_bin > AG_NBINS) {
+        *err = "cold_below_bin must be in [0, 5]";
         return AG_ERR_INVALID;
     }
     if (p->order > 1) {
@@ -133,8 +138,8 @@ int ag_plan_build(uint32_t n, uint32_t m, const uint32_t *out_degree,
         *err = "inverse row extends past num_edges";
         return AG_ERR_INVALID;
     }
-    if (total > 0xFFFFFFFFull) {
-        *err = "total in-degree exceeds 2^32-1";
+    if (total > 0xFFFF0000ull) {
+        *err = "total in-degree exceeds 2^32 - 2^16";
         return AG_ERR_INVALID;
     }
     int bad_src = 0;

@@ -95,6 +95,8 @@ def main() -> int:
     log(f"graph built: n={n} m={m} in {time.perf_counter() - t0:.1f} s")
 
     dev = ag.Device(0)
+    copy_gbs = dev.copy_bandwidth(1 << 30, 10)
+    log(f"device copy bandwidth {copy_gbs:.0f} GB/s (1 GiB, read + write)")
     t0 = time.perf_counter()
     dg = dev.upload(g)
     info = dg.info()
@@ -123,21 +125,29 @@ def main() -> int:
     log(f"{K} steps: {ms_per_step:.3f} ms/step wall, kernel avg {kern_ms:.3f} ms over "
         f"{launches} launches, {gteps:.2f} GTEPS, {achieved:.0f} GB/s algorithmic")
 
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    wl = f"pagerank_pull_{args.storage}_{args.graph}{args.scale}"
+    if os.path.exists(tpath):
+        t = json.load(open(tpath)).get(wl)
+        if t:
+            traffic = t["traffic_bytes_per_launch"]
     out = {
         "metric": METRIC, "value": round(gteps, 3), "unit": "GTEPS", "n_gpus": 1,
         "steps": K, "warmup": W, "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32" if args.storage == "f32" else f"{args.storage}-storage/f32-accumulate",
         "data": "synthetic",
-        "config": {"workload": f"pagerank_pull_{args.storage}_{args.graph}{args.scale}",
+        "config": {"workload": wl,
                    "graph": gdesc, "num_vertices": n, "num_edges": m,
                    "step": "one PageRank-pull iteration (gather + damping update + L1/active "
                            "reduction + next contribution)"},
         "hbm_gb_per_s": round(nbytes / (ms_per_step * 1e-3) / 1e9, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "algorithmic_bytes": nbytes,
-                     "kernel_ms": round(kern_ms, 4)},
+                     "traffic": traffic, "algorithmic_bytes": nbytes,
+                     "kernel_ms": round(kern_ms, 4),
+                     "measured_copy_gb_per_s": round(copy_gbs, 1)},
     }
 
     if not args.no_ttl:

@@ -57,6 +57,12 @@ int ag_abi_version(void);
  * (0 = unbounded). */
 int ag_device_set_watchdog(ag_device *dev, uint64_t run_timeout_ms, uint32_t poll_interval_us);
 
+/* Device-to-device copy bandwidth: copies `bytes` (rounded down to 16-byte units) between two
+ * fresh HBM buffers `reps` times with the library's streaming copy kernel and reports the
+ * mean GB/s (read + write bytes / time). The measured ceiling the pull kernels are compared
+ * with, and a known-size kernel for calibrating hardware byte counters. */
+int ag_device_copy_bandwidth(ag_device *dev, uint64_t bytes, uint32_t reps, double *gbps);
+
 /* ---------------------------------------------------------------- graph + plan */
 
 /* Row classes of the pull plan, by in-degree. */
@@ -78,6 +84,8 @@ typedef struct ag_plan_params {
     uint32_t tile_nnz;         /* max in-edges per thread-bin tile (<= 4096)    */
     uint32_t segment_nnz;      /* in-edges per hub segment                      */
     uint32_t order;            /* row order inside a bin: 0 = vertex id, 1 = out-degree desc */
+    uint32_t cold_below_bin;   /* 0: off; k: gathers of sources in rows of bins < k use
+                                  non-temporal loads (they stop evicting hot L2 lines)    */
 } ag_plan_params;
 
 void ag_plan_params_default(ag_plan_params *p);
