@@ -20,7 +20,7 @@ from ._abi import (BIN_NAMES, GATHER_F32, GATHER_U8, GATHER_U16, GATHER_U32, GAT
                    STORE_BF16, STORE_F16, STORE_F32, GraphInfo, PlanParams, PrOptions,
                    PrResult)
 
-__all__ = ["lib", "AGError", "Graph", "Device", "DeviceGraph", "PrRun", "plan_params",
+__all__ = ["lib", "AGError", "Graph", "Device", "DeviceGraph", "PrRun", "Batch", "plan_params",
            "RMAT_A", "RMAT_B", "RMAT_C", "DEFAULT_SEED"]
 
 RMAT_A, RMAT_B, RMAT_C = 0.57, 0.19, 0.19
@@ -396,6 +396,59 @@ class DeviceGraph:
         _check(lib().ag_pull_gather(self._h, gtype, v.ctypes.data_as(C.c_void_p),
                                     out.ctypes.data_as(C.c_void_p)), "ag_pull_gather")
         return out
+
+
+class Batch:
+    """Many independent graphs solved by one persistent (hipGraph-captured) launch."""
+
+    def __init__(self, dev: "Device", graphs):
+        self.dev = dev
+        G = len(graphs)
+        vo = np.zeros(G + 1, np.uint32)
+        eo = np.zeros(G + 1, np.uint32)
+        for i, g in enumerate(graphs):
+            vo[i + 1] = vo[i] + g.n
+            eo[i + 1] = eo[i] + g.m
+        cat = lambda f: (np.concatenate([np.asarray(f(g), np.uint32) for g in graphs])
+                         if vo[-1] else np.zeros(0, np.uint32))
+        od, ii, dg = cat(lambda g: g.out_degree), cat(lambda g: g.inv_idx), cat(lambda g: g.in_degree)
+        src = (np.concatenate([np.asarray(g.inv_src, np.uint32) for g in graphs])
+               if eo[-1] else np.zeros(0, np.uint32))
+        self.vert_off, self.edge_off = vo, eo
+        h = C.c_void_p()
+        _check(lib().ag_batch_upload(dev.handle, G, _ptr(vo, C.c_uint32), _ptr(eo, C.c_uint32),
+                                     _ptr(od, C.c_uint32), _ptr(ii, C.c_uint32),
+                                     _ptr(dg, C.c_uint32), _ptr(src, C.c_uint32), C.byref(h)),
+               "ag_batch_upload")
+        self._h = h
+        dev._graphs.add(self)
+
+    def close(self) -> None:
+        if self._h:
+            lib().ag_batch_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def pagerank(self, **kw):
+        o = _abi.BatchOptions()
+        lib().ag_batch_options_default(C.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        G = self.vert_off.size - 1
+        ranks = np.zeros(int(self.vert_off[-1]), np.float32)
+        its = np.zeros(G, np.uint32)
+        err = np.zeros(G, np.float64)
+        ms = C.c_double(0.0)
+        _check(lib().ag_batch_pagerank(self._h, C.byref(o), _ptr(ranks, C.c_float),
+                                       _ptr(its, C.c_uint32), _ptr(err, C.c_double),
+                                       C.byref(ms)), "ag_batch_pagerank")
+        per = [ranks[self.vert_off[i]:self.vert_off[i + 1]] for i in range(G)]
+        return per, its, err, ms.value
 
 
 class PrSession:

@@ -56,6 +56,11 @@ class PrResult(C.Structure):
                 ("refine_iteration", i32)]
 
 
+class BatchOptions(C.Structure):
+    _fields_ = [("max_iterations", u32), ("stop_rule", u32), ("epsilon", f64),
+                ("l1_tolerance", f64), ("init", u32), ("use_hip_graph", u32)]
+
+
 class Vertex(C.Structure):
     _fields_ = [("out_degree", P(u32)), ("in_degree", P(u32)), ("edges_idx", P(u32))]
 
@@ -134,6 +139,11 @@ SIGNATURES = {
     "ag_spmv_pull": (C.c_int, [VP, P(f32), P(f32), u32, P(f64)]),
     "ag_spmv_pull_fixed": (C.c_int, [VP, P(u32), P(u64), u32, P(f64)]),
     "ag_pull_gather": (C.c_int, [VP, u32, VP, VP]),
+    "ag_batch_upload": (C.c_int, [VP, u32, P(u32), P(u32), P(u32), P(u32), P(u32), P(u32),
+                                  P(VP)]),
+    "ag_batch_free": (C.c_int, [VP]),
+    "ag_batch_options_default": (None, [P(BatchOptions)]),
+    "ag_batch_pagerank": (C.c_int, [VP, P(BatchOptions), P(f32), P(u32), P(f64), P(f64)]),
     "ag_fixed_encode": (C.c_int, [VP, u32, P(f32), P(u64), u32]),
     "ag_fixed_decode": (C.c_int, [VP, u32, P(u64), P(f64), u32]),
     # graphCSR.h

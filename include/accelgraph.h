@@ -196,6 +196,41 @@ int ag_spmv_pull(ag_dgraph *g, const float *x, float *y, uint32_t repetitions,
 int ag_spmv_pull_fixed(ag_dgraph *g, const uint32_t *x_q, uint64_t *y_q,
                        uint32_t repetitions, double *avg_kernel_ms);
 
+/* ---------------------------------------------------------------- many small graphs */
+
+/* A batch of independent graphs resident in HBM, solved together by one persistent launch
+ * (one workgroup per graph, largest graphs first) that a hipGraph captures and replays.
+ * Inputs are the per-graph reference arrays concatenated: graph g owns vertices
+ * [vert_off[g], vert_off[g+1]) and in-edges [edge_off[g], edge_off[g+1]); inv_edges_idx is
+ * relative to the graph's first edge and inv_src holds graph-local vertex ids. */
+typedef struct ag_batch ag_batch;
+
+int ag_batch_upload(ag_device *dev, uint32_t num_graphs, const uint32_t *vert_off,
+                    const uint32_t *edge_off, const uint32_t *out_degree,
+                    const uint32_t *inv_edges_idx, const uint32_t *inv_degree,
+                    const uint32_t *inv_src, ag_batch **out);
+int ag_batch_free(ag_batch *b);
+
+typedef struct ag_batch_options {
+    uint32_t max_iterations;
+    uint32_t stop_rule;        /* AG_STOP_ACTIVE or AG_STOP_L1 (per graph)              */
+    double epsilon;
+    double l1_tolerance;
+    uint32_t init;             /* AG_INIT_REFERENCE or AG_INIT_UNIFORM                   */
+    uint32_t use_hip_graph;    /* capture the launch once and replay it                  */
+} ag_batch_options;
+
+/* Rows with more in-edges than this are summed by a whole wavefront; the others by one lane
+ * each, in the reference's stream order. */
+#define AG_BATCH_HEAVY_DEG 32
+
+void ag_batch_options_default(ag_batch_options *o);
+
+/* ranks_out[vert_off[G]] normalised per graph (scaled / n_g); iterations[g] in the reference's
+ * meaning; error[g] = last L1 delta; device_ms = time of the launch (HIP events). */
+int ag_batch_pagerank(ag_batch *b, const ag_batch_options *opts, float *ranks_out,
+                      uint32_t *iterations, double *error, double *device_ms);
+
 /* ---------------------------------------------------------------- raw gather */
 
 /* The bare pull gather (the FPGA sum kernel's contract): out[v] = sum over in-edges

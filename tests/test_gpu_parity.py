@@ -302,3 +302,104 @@ def test_edge_cases(device):
     run = dg.pagerank(max_iterations=5)
     assert run.iterations == 0 and run.ranks.size == 0
     dg.close()
+
+
+# ------------------------------------------------------------------ 16-bit storage
+
+@pytest.mark.parametrize("storage", [ag.STORE_BF16, ag.STORE_F16])
+def test_pagerank_16bit_storage_refined(device, rmat16, storage):
+    """16-bit contributions (FP32 accumulate) until the L1 delta drops below refine_l1, then
+    FP32 contributions to L1 <= 1e-6: the result must be as accurate as the plain FP32 run
+    against the FP64 oracle."""
+    say(f"pagerank 16-bit storage {storage}")
+    g = rmat16
+    dg = device.upload(g)
+    kw = dict(max_iterations=300, stop_rule=ag.STOP_L1, l1_tolerance=1e-6,
+              init=ag.INIT_UNIFORM)
+    plain = dg.pagerank(**kw)
+    mixed = dg.pagerank(storage=storage, refine_l1=1e-3, **kw)
+    assert mixed.error_total <= 1e-6 and mixed.refine_iteration > 0
+    assert mixed.iterations > mixed.refine_iteration
+    f64 = f64_ref(g)
+    e_plain = np.abs(plain.ranks - f64).max()
+    e_mixed = np.abs(mixed.ranks - f64).max()
+    l1_mixed = np.abs(mixed.ranks - f64).sum()
+    say(f"  plain it={plain.iterations} err={e_plain:.3e}; mixed it={mixed.iterations} "
+        f"refine@{mixed.refine_iteration} err={e_mixed:.3e} l1={l1_mixed:.3e}")
+    assert e_mixed <= max(2 * e_plain, 1e-9)
+    assert np.abs(mixed.ranks - plain.ranks).max() <= FP32_TOL
+    # the 16-bit phase really ran: its early deltas differ from the FP32 run's
+    assert not np.array_equal(mixed.hist_err[:3], plain.hist_err[:3])
+    dg.close()
+
+
+# ------------------------------------------------------------------ many small graphs
+
+def _small_graphs(count, seed=1):
+    rng = np.random.default_rng(seed)
+    gs = []
+    for i in range(count):
+        scale = int(rng.integers(4, 12))
+        if i % 3 == 2:
+            n = 1 << scale
+            gs.append(ag.Graph.erdos_renyi(n, 8 * n, seed=100 + i))
+        else:
+            gs.append(ag.Graph.rmat(scale, seed=100 + i))
+    gs.append(ag.Graph.from_edges([], [], n=1))   # a lone vertex
+    gs.append(ag.Graph.from_edges([0, 1, 2], [1, 2, 0], n=3))
+    return gs
+
+
+@pytest.mark.parametrize("rule", ["l1", "active"])
+def test_batch_matches_oracle_and_single(device, rule):
+    say(f"batch {rule}")
+    gs = _small_graphs(40)
+    b = ag.Batch(device, gs)
+    if rule == "l1":
+        kw = dict(max_iterations=300, stop_rule=ag.STOP_L1, l1_tolerance=1e-6,
+                  init=ag.INIT_UNIFORM)
+        okw = dict(max_iterations=300, stop_rule=1, l1_tolerance=1e-6, init=1)
+    else:
+        kw = dict(max_iterations=100, stop_rule=ag.STOP_ACTIVE, epsilon=1e-4,
+                  init=ag.INIT_REFERENCE)
+        okw = dict(max_iterations=100, stop_rule=0, epsilon=1e-4, init=0)
+    ranks, its, err, ms = b.pagerank(**kw)
+    say(f"  {len(gs)} graphs in {ms:.2f} ms")
+    for i, g in enumerate(gs):
+        ref = ref_pr(g, **okw)
+        assert abs(int(its[i]) - ref.iterations) <= 3, (i, its[i], ref.iterations)
+        assert np.abs(ranks[i] - ref.ranks).max() <= FP32_TOL
+        if g.n and g.in_degree.max() <= 32:   # every row stream-ordered
+            if its[i] == ref.iterations:
+                np.testing.assert_array_equal(ranks[i], ref.ranks)
+        if rule == "l1" and g.n > 1:
+            f64 = f64_ref(g)
+            assert np.abs(ranks[i] - f64).max() <= 2 * np.abs(ref.ranks - f64).max() + 1e-9
+            dg = device.upload(g)
+            single = dg.pagerank(**kw)
+            assert np.abs(single.ranks - ranks[i]).max() <= FP32_TOL
+            dg.close()
+    b.close()
+
+
+def test_batch_hip_graph_replays(device):
+    """The captured launch replayed several times, with eager launches in between, gives the
+    eager result every time."""
+    say("batch hipGraph replays")
+    gs = _small_graphs(24, seed=9)
+    b = ag.Batch(device, gs)
+    kw = dict(max_iterations=200, stop_rule=ag.STOP_L1, l1_tolerance=1e-6, init=ag.INIT_UNIFORM)
+    eager, eits, _, _ = b.pagerank(use_hip_graph=0, **kw)
+    for rep in range(4):
+        got, gits, _, _ = b.pagerank(use_hip_graph=1, **kw)
+        for i in range(len(gs)):
+            np.testing.assert_array_equal(got[i], eager[i])
+        np.testing.assert_array_equal(gits, eits)
+        again, _, _, _ = b.pagerank(use_hip_graph=0, **kw)
+        for i in range(len(gs)):
+            np.testing.assert_array_equal(again[i], eager[i])
+    # options change between replays without re-capture
+    got, gits, _, _ = b.pagerank(use_hip_graph=1, max_iterations=3, stop_rule=ag.STOP_NONE,
+                                 init=ag.INIT_UNIFORM)
+    assert np.all(gits == 3)
+    b.close()
