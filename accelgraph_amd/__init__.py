@@ -17,8 +17,8 @@ from . import _abi
 from ._abi import (BIN_NAMES, GATHER_F32, GATHER_U8, GATHER_U16, GATHER_U32, GATHER_U64,
                    INIT_REFERENCE, INIT_UNIFORM, INIT_USER, KIND_F32, KIND_FIXED8,
                    KIND_FIXED16, KIND_FIXED32, KIND_FIXED64, STOP_ACTIVE, STOP_L1, STOP_NONE,
-                   STORE_BF16, STORE_F16, STORE_F32, GraphInfo, PlanParams, PrOptions,
-                   PrResult)
+                   STORE_BF16, STORE_F16, STORE_F32, PPR_UNIFORM, GraphInfo, PlanParams,
+                   PrOptions, PrResult)
 
 __all__ = ["lib", "AGError", "Graph", "Device", "DeviceGraph", "PrRun", "Batch", "plan_params",
            "RMAT_A", "RMAT_B", "RMAT_C", "DEFAULT_SEED"]
@@ -368,6 +368,23 @@ class DeviceGraph:
         if scaled:
             run.scaled = sc
         return run
+
+    def ppr(self, seeds, **kw):
+        """Personalised PageRank, one column per seed (PPR_UNIFORM = plain restart).
+        Returns (ranks [n, k], column L1 deltas, PprResult)."""
+        sd = np.ascontiguousarray(seeds, dtype=np.uint32)
+        k = sd.size
+        o = _abi.PprOptions()
+        lib().ag_ppr_options_default(C.byref(o))
+        for key, v in kw.items():
+            setattr(o, key, v)
+        out = np.zeros(self.n * k, np.float32)
+        ce = np.zeros(k, np.float64)
+        r = _abi.PprResult()
+        _check(lib().ag_ppr_pull(self._h, k, _ptr(sd, C.c_uint32), C.byref(o),
+                                 _ptr(out, C.c_float), _ptr(ce, C.c_double), C.byref(r)),
+               "ag_ppr_pull")
+        return out.reshape(self.n, k), ce, r
 
     def session(self, **kw) -> "PrSession":
         return PrSession(self, **kw)

@@ -28,6 +28,7 @@ INIT_REFERENCE, INIT_UNIFORM, INIT_USER = 0, 1, 2
 KIND_F32, KIND_FIXED64, KIND_FIXED32, KIND_FIXED16, KIND_FIXED8 = 0, 64, 32, 16, 8
 STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
 GATHER_F32, GATHER_U64, GATHER_U32, GATHER_U16, GATHER_U8 = 0, 1, 2, 3, 4
+PPR_UNIFORM = 0xFFFFFFFF
 
 
 class PlanParams(C.Structure):
@@ -59,6 +60,16 @@ class PrResult(C.Structure):
 class BatchOptions(C.Structure):
     _fields_ = [("max_iterations", u32), ("stop_rule", u32), ("epsilon", f64),
                 ("l1_tolerance", f64), ("init", u32), ("use_hip_graph", u32)]
+
+
+class PprOptions(C.Structure):
+    _fields_ = [("max_iterations", u32), ("stop_rule", u32), ("l1_tolerance", f64),
+                ("storage", u32), ("refine_l1", f64), ("chunk", u32)]
+
+
+class PprResult(C.Structure):
+    _fields_ = [("iterations", u32), ("executed", u32), ("max_error", f64), ("loop_ms", f64),
+                ("refine_iteration", i32)]
 
 
 class Vertex(C.Structure):
@@ -139,6 +150,8 @@ SIGNATURES = {
     "ag_spmv_pull": (C.c_int, [VP, P(f32), P(f32), u32, P(f64)]),
     "ag_spmv_pull_fixed": (C.c_int, [VP, P(u32), P(u64), u32, P(f64)]),
     "ag_pull_gather": (C.c_int, [VP, u32, VP, VP]),
+    "ag_ppr_options_default": (None, [P(PprOptions)]),
+    "ag_ppr_pull": (C.c_int, [VP, u32, P(u32), P(PprOptions), P(f32), P(f64), P(PprResult)]),
     "ag_batch_upload": (C.c_int, [VP, u32, P(u32), P(u32), P(u32), P(u32), P(u32), P(u32),
                                   P(VP)]),
     "ag_batch_free": (C.c_int, [VP]),

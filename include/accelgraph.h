@@ -196,6 +196,41 @@ int ag_spmv_pull(ag_dgraph *g, const float *x, float *y, uint32_t repetitions,
 int ag_spmv_pull_fixed(ag_dgraph *g, const uint32_t *x_q, uint64_t *y_q,
                        uint32_t repetitions, double *avg_kernel_ms);
 
+/* ---------------------------------------------------------------- personalised PageRank */
+
+/* Up to 64 PageRank columns pulled together: column c restarts at seeds[c] (an original vertex
+ * id; AG_PPR_UNIFORM restarts uniformly, which is plain PageRank). Scaled units as the plain
+ * path: the restart mass is (1 - Damp) * N, all of it on the seed; every column starts at 1.0.
+ * Contributions are stored vertex-major, 64 per vertex (one 128-byte line in BF16/FP16), and
+ * accumulated in FP32; with 16-bit storage the last iterations switch to FP32 contributions
+ * (refine_l1, as for ag_pr_options). The L1 delta of each column is reduced in FP64. */
+#define AG_PPR_MAX_COLS 64
+#define AG_PPR_UNIFORM 0xFFFFFFFFu
+
+typedef struct ag_ppr_options {
+    uint32_t max_iterations;
+    uint32_t stop_rule;        /* AG_STOP_L1 (every column <= l1_tolerance) or AG_STOP_NONE */
+    double l1_tolerance;
+    uint32_t storage;          /* AG_STORE_*                                                */
+    double refine_l1;
+    uint32_t chunk;            /* iterations queued between convergence polls (0 = auto)   */
+} ag_ppr_options;
+
+typedef struct ag_ppr_result {
+    uint32_t iterations;       /* index of the stopping iteration, or max_iterations        */
+    uint32_t executed;
+    double max_error;          /* largest column L1 delta of the last executed iteration    */
+    double loop_ms;            /* device time of the iteration loop                         */
+    int32_t refine_iteration;
+} ag_ppr_result;
+
+void ag_ppr_options_default(ag_ppr_options *o);
+
+/* ranks_out[v * k + c]: normalised rank of vertex v (original order) in column c.
+ * col_error[c] (may be NULL): last L1 delta of column c. */
+int ag_ppr_pull(ag_dgraph *g, uint32_t k, const uint32_t *seeds, const ag_ppr_options *opts,
+                float *ranks_out, double *col_error, ag_ppr_result *res);
+
 /* ---------------------------------------------------------------- many small graphs */
 
 /* A batch of independent graphs resident in HBM, solved together by one persistent launch

@@ -56,6 +56,9 @@ def lib() -> C.CDLL:
         L.or_pagerank_f64.restype = C.c_int
         L.or_pagerank_f64.argtypes = [u32, P(u32), P(u32), P(u32), P(u32), f64, u32, C.c_int,
                                       P(f64)]
+        L.or_ppr_f64.restype = C.c_int
+        L.or_ppr_f64.argtypes = [u32, P(u32), P(u32), P(u32), P(u32), u32, f64, u32, C.c_int,
+                                 P(f64)]
         L.or_build_csr.restype = C.c_int
         L.or_build_csr.argtypes = [u32, u64, P(u32), P(u32), P(u32), P(u32), P(u32), P(u32),
                                    P(u32), P(u32), P(u64)]
@@ -162,6 +165,16 @@ def pagerank_f64(outdeg, idx, deg, src, tol: float = 1e-14, max_iterations: int 
     out = np.zeros(idx.size, np.float64)
     lib().or_pagerank_f64(idx.size, _p(outdeg, u32), _p(idx, u32), _p(deg, u32), _p(src, u32),
                           tol, max_iterations, threads, _p(out, f64))
+    return out
+
+
+def ppr_f64(outdeg, idx, deg, src, seed: int, tol: float = 1e-14,
+            max_iterations: int = 3000, threads: int = 0) -> np.ndarray:
+    """FP64 personalised PageRank column (seed = 0xFFFFFFFF: uniform restart)."""
+    outdeg, idx, deg, src = _u32(outdeg), _u32(idx), _u32(deg), _u32(src)
+    out = np.zeros(idx.size, np.float64)
+    lib().or_ppr_f64(idx.size, _p(outdeg, u32), _p(idx, u32), _p(deg, u32), _p(src, u32),
+                     seed, tol, max_iterations, threads, _p(out, f64))
     return out
 
 

@@ -403,3 +403,34 @@ def test_batch_hip_graph_replays(device):
                                  init=ag.INIT_UNIFORM)
     assert np.all(gits == 3)
     b.close()
+
+
+# ------------------------------------------------------------------ personalised PageRank
+
+@pytest.mark.parametrize("storage", [ag.STORE_F32, ag.STORE_BF16, ag.STORE_F16])
+@pytest.mark.parametrize("pname", ["default", "small"])
+def test_ppr_64_columns(device, rmat12, storage, pname):
+    say(f"ppr 64 columns storage={storage} {pname}")
+    g = rmat12
+    dg = upload(device, g, pname)
+    rng = np.random.default_rng(4)
+    seeds = np.concatenate([[ag.PPR_UNIFORM], rng.choice(g.n, 63, replace=False)]).astype(np.uint32)
+    ranks, colerr, res = dg.ppr(seeds, max_iterations=300, stop_rule=ag.STOP_L1,
+                                l1_tolerance=1e-6, storage=storage, refine_l1=1e-3)
+    assert res.max_error <= 1e-6 and np.all(colerr <= 1e-6)
+    if storage != ag.STORE_F32:
+        assert res.refine_iteration > 0
+    # column 0 (uniform restart) is plain PageRank
+    plain = dg.pagerank(max_iterations=300, stop_rule=ag.STOP_L1, l1_tolerance=1e-6,
+                        init=ag.INIT_UNIFORM)
+    assert np.abs(ranks[:, 0] - plain.ranks).max() <= FP32_TOL
+    worst = 0.0
+    for c in [0, 1, 2, 17, 40, 63]:
+        f64 = oracle.ppr_f64(g.out_degree, g.inv_idx, g.in_degree, g.inv_src, int(seeds[c]),
+                             threads=8)
+        e = np.abs(ranks[:, c] - f64).max()
+        worst = max(worst, e / max(f64.max(), 1e-30))
+        assert e <= FP32_TOL, (c, e)
+    say(f"  iterations={res.iterations} refine@{res.refine_iteration} "
+        f"max relative err vs FP64 {worst:.2e}")
+    dg.close()
