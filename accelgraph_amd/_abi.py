@@ -34,7 +34,8 @@ PPR_UNIFORM = 0xFFFFFFFF
 class PlanParams(C.Structure):
     _fields_ = [("thread_max_deg", u32), ("group16_max_deg", u32), ("wave_max_deg", u32),
                 ("block_max_deg", u32), ("tile_rows", u32), ("tile_nnz", u32),
-                ("segment_nnz", u32), ("order", u32), ("cold_below_bin", u32)]
+                ("segment_nnz", u32), ("order", u32), ("cold_below_bin", u32),
+                ("wg_per_cu", u32)]
 
 
 class GraphInfo(C.Structure):
@@ -138,6 +139,8 @@ SIGNATURES = {
     "ag_graph_free": (C.c_int, [VP]),
     "ag_graph_set_fixed_weights": (C.c_int, [VP, P(u32), P(u32), P(u32)]),
     "ag_graph_get_info": (C.c_int, [VP, P(GraphInfo)]),
+    "ag_autotune_plan": (C.c_int, [VP, u32, u32, P(u32), P(u32), P(u32), P(u32),
+                                   P(PlanParams), u32, u32, u32, P(f64), P(u32)]),
     "ag_pr_options_default": (None, [P(PrOptions)]),
     "ag_pagerank_pull": (C.c_int, [VP, P(PrOptions), P(f32), P(PrResult)]),
     "ag_pr_begin": (C.c_int, [VP, P(PrOptions), P(VP)]),

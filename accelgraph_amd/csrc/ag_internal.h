@@ -78,6 +78,9 @@ struct PrState {
     int32_t refine_iter;    // first iteration that writes f32 contributions (-1: none yet)
     uint32_t last_active;
     double last_err;
+    double best_err;        // smallest L1 delta of the 16-bit phase so far
+    uint32_t stall;         // 16-bit iterations since best_err last improved by 1%
+    uint32_t pad;
 };
 
 int ag_set_error(int status, const char *fmt, ...);

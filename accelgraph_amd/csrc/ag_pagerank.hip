@@ -86,11 +86,7 @@ __device__ __forceinline__ void pr_control(const PrCtl &C, uint32_t iter, double
     if (C.mixed) {
         const int32_t ri = st->refine_iter;
         if (ri < 0) {
-            // switch to f32 contributions once the L1 delta is small enough, or as soon as
-            // the 16-bit iteration stops improving
-            const bool stalled = iter > 0 && E >= st->last_err;
-            if (E <= C.refine_l1 || stalled)
-                st->refine_iter = (int32_t)iter + 1;
+            refine_decide(st, iter, E, C.refine_l1);
             can_stop = false;
         } else {
             can_stop = (int32_t)iter >= ri + 1;

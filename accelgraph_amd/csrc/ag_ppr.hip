@@ -265,9 +265,7 @@ __global__ __launch_bounds__(AG_FIN_WG) void ppr_finish_kernel(PullGeom G, PprAr
     if (A.storage != AG_STORE_F32) {
         const int32_t ri = st->refine_iter;
         if (ri < 0) {
-            const bool stalled = iter > 0 && E >= st->last_err;
-            if (E <= A.refine_l1 || stalled)
-                st->refine_iter = (int32_t)iter + 1;
+            refine_decide(st, iter, E, A.refine_l1);
             can_stop = false;
         } else {
             can_stop = (int32_t)iter >= ri + 1;

@@ -260,3 +260,19 @@ __global__ __launch_bounds__(AG_FIN_WG) void pull_finish_kernel(PullGeom G, P po
     Red red{0.0, 0u};
     pull_hub_finish(G, pol, red);
 }
+
+// Storage switch of the 16-bit contribution phase (one thread): switch to f32 contributions
+// from iteration iter + 1 once the L1 delta E is at most refine_l1, or once E has not improved
+// on its best value by 1% for three iterations (the 16-bit rounding floor has been reached).
+__device__ __forceinline__ void refine_decide(PrState *st, uint32_t iter, double E,
+                                              double refine_l1)
+{
+    if (iter == 0 || E < 0.99 * st->best_err) {
+        st->best_err = E;
+        st->stall = 0;
+    } else {
+        st->stall++;
+    }
+    if (E <= refine_l1 || st->stall >= 3)
+        st->refine_iter = (int32_t)iter + 1;
+}

@@ -241,6 +241,8 @@ uint32_t ag_pull_grid(const ag_dgraph *g, uint32_t lds_bytes)
     // 160 KB of LDS and 8 four-wave workgroups (32 waves) per CU
     uint32_t by_lds = lds_bytes ? (160u * 1024u) / lds_bytes : 8u;
     uint32_t per_cu = by_lds < 8u ? by_lds : 8u;
+    if (g->params.wg_per_cu && g->params.wg_per_cu < per_cu)
+        per_cu = g->params.wg_per_cu;
     if (per_cu == 0)
         per_cu = 1;
     uint32_t grid = g->dev->num_cus * per_cu;

@@ -25,6 +25,7 @@ void ag_plan_params_default(ag_plan_params *p)
     p->segment_nnz = 16384;
     p->order = 0;
     p->cold_below_bin = 0;
+    p->wg_per_cu = 0;
 }
 
 int ag_plan_params_check(const ag_plan_params *p, const char **err)
@@ -48,6 +49,10 @@ int ag_plan_params_check(const ag_plan_params *p, const char **err)
     }
     if (p->segment_nnz < AG_WG || p->segment_nnz > (1u << 24)) {
         *err = "segment_nnz must be in [256, 2^24]";
+        return AG_ERR_INVALID;
+    }
+    if (p->wg_per_cu > 32) {
+        *err = "wg_per_cu must be in [0, 32]";
         return AG_ERR_INVALID;
     }
     if (p->cold_below_bin > AG_NBINS) {

@@ -59,6 +59,94 @@ def cpu_baseline(g, budget_s: float) -> dict:
             "ms_per_iteration": round(dt * 1e3, 3)}
 
 
+def time_to_l1(ag, dg, g, f64_iters: int) -> dict:
+    """Plain FP32 vs BF16-stored / FP32-refined time to L1 <= 1e-6 from the uniform start,
+    each checked against the FP64 oracle."""
+    import numpy as np
+    kw = dict(max_iterations=300, stop_rule=ag.STOP_L1, l1_tolerance=1e-6,
+              init=ag.INIT_UNIFORM, history=True)
+    plain = dg.pagerank(**kw)
+    mixed = dg.pagerank(storage=ag.STORE_BF16, refine_l1=1e-3, **kw)
+    out = {"plain_f32": {"ms": round(float(plain.hist_ms.sum()), 3),
+                         "iterations": plain.executed, "final_l1": plain.error_total},
+           "bf16_refined": {"ms": round(float(mixed.hist_ms.sum()), 3),
+                            "iterations": mixed.executed, "final_l1": mixed.error_total,
+                            "refine_iteration": mixed.refine_iteration}}
+    out["bf16_speedup"] = round(out["plain_f32"]["ms"] / max(out["bf16_refined"]["ms"], 1e-9), 3)
+    if f64_iters:
+        import oracle
+        f64 = oracle.pagerank_f64(g.out_degree, g.inv_idx, g.in_degree, g.inv_src, tol=1e-13,
+                                  max_iterations=f64_iters,
+                                  threads=len(os.sched_getaffinity(0)))
+        for key, run in (("plain_f32", plain), ("bf16_refined", mixed)):
+            d = np.abs(run.ranks.astype(np.float64) - f64)
+            out[key]["max_abs_err_vs_f64"] = float(d.max())
+            out[key]["l1_err_vs_f64"] = float(d.sum())
+    return out
+
+
+def batched_section(ag, dev, count: int, lo: int, hi: int) -> dict:
+    """count RMAT graphs with vertex counts 2^s log-uniform, i.e. scale s uniform over the
+    integers [lo, hi] (seeded), to L1 <= 1e-6: one persistent hipGraph launch versus the plain
+    single-graph GPU path run graph by graph."""
+    import numpy as np
+    rng = np.random.default_rng(ag.DEFAULT_SEED)
+    scales = rng.integers(lo, hi + 1, count)
+    t0 = time.perf_counter()
+    graphs = [ag.Graph.rmat(int(s), seed=ag.DEFAULT_SEED + i) for i, s in enumerate(scales)]
+    log(f"batch: built {count} graphs (scales {lo}-{hi}, {sum(g.m for g in graphs)} edges) in "
+        f"{time.perf_counter() - t0:.1f} s")
+    b = ag.Batch(dev, graphs)
+    kw = dict(max_iterations=300, stop_rule=ag.STOP_L1, l1_tolerance=1e-6, init=ag.INIT_UNIFORM)
+    b.pagerank(use_hip_graph=1, **kw)                       # capture + warm
+    ranks, its, err, ms = b.pagerank(use_hip_graph=1, **kw)
+    dgs = [dev.upload(g) for g in graphs]
+    for dg in dgs[:4]:
+        dg.pagerank(history=False, **kw)                    # warm
+    dev.sync()
+    t0 = time.perf_counter()
+    singles = [dg.pagerank(history=False, **kw) for dg in dgs]
+    plain_ms = (time.perf_counter() - t0) * 1e3
+    worst = max(float(np.abs(s.ranks - r).max()) for s, r in zip(singles, ranks))
+    for dg in dgs:
+        dg.close()
+    b.close()
+    return {"graphs": count, "scales": f"{lo}-{hi} (2^s log-uniform)", "edges": int(sum(g.m for g in graphs)),
+            "batched_ms": round(ms, 3), "plain_loop_ms": round(plain_ms, 3),
+            "speedup": round(plain_ms / max(ms, 1e-9), 2),
+            "max_iterations": int(its.max()), "max_final_l1": float(err.max()),
+            "max_abs_diff_vs_single_graph_path": worst}
+
+
+def ppr_section(ag, dev, scale: int, k: int) -> dict:
+    """k-column personalised PageRank (column 0 uniform = plain PageRank, the others seeded
+    uniformly at random by seed) to L1 <= 1e-6, BF16 storage with FP32 refinement, versus k
+    plain single-vector runs (estimated as k x one plain run's time)."""
+    import numpy as np
+    g = ag.Graph.rmat(scale)
+    dg = dev.upload(g)
+    rng = np.random.default_rng(ag.DEFAULT_SEED)
+    seeds = np.concatenate([[ag.PPR_UNIFORM], rng.choice(g.n, k - 1, replace=False)])
+    res = {}
+    for name, st in (("bf16", ag.STORE_BF16), ("f16", ag.STORE_F16), ("f32", ag.STORE_F32)):
+        ranks, colerr, r = dg.ppr(seeds, max_iterations=300, stop_rule=ag.STOP_L1,
+                                  l1_tolerance=1e-6, storage=st, refine_l1=1e-3)
+        res[name] = {"ms": round(r.loop_ms, 3), "iterations": r.executed,
+                     "max_col_l1": r.max_error, "refine_iteration": r.refine_iteration}
+        if name == "bf16":
+            col0 = ranks[:, 0].copy()
+    plain = dg.pagerank(max_iterations=300, stop_rule=ag.STOP_L1, l1_tolerance=1e-6,
+                        init=ag.INIT_UNIFORM, history=True)
+    plain_ms = float(plain.hist_ms.sum())
+    res["plain_single_ms"] = round(plain_ms, 3)
+    res["k_plain_runs_ms"] = round(k * plain_ms, 3)
+    res["speedup_vs_k_plain"] = round(k * plain_ms / max(res["bf16"]["ms"], 1e-9), 2)
+    res["col0_max_abs_diff_vs_plain"] = float(np.abs(col0 - plain.ranks).max())
+    res["graph"] = f"RMAT scale {scale} (n={g.n}, m={g.m}), {k} columns"
+    dg.close()
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,7 +158,13 @@ def main() -> int:
     ap.add_argument("--storage", choices=["f32", "bf16", "f16"], default="f32")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-ttl", action="store_true", help="skip the time-to-L1 run")
+    ap.add_argument("--no-ttl", action="store_true", help="skip the time-to-L1 runs")
+    ap.add_argument("--f64-iters", type=int, default=150,
+                    help="FP64 oracle iterations for the error report (0 = skip)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 512-graph batch")
+    ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--no-ppr", action="store_true", help="skip the 64-column PPR")
+    ap.add_argument("--ppr-scale", type=int, default=22)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -151,14 +245,18 @@ def main() -> int:
     }
 
     if not args.no_ttl:
-        run = dg.pagerank(max_iterations=300, stop_rule=ag.STOP_L1, l1_tolerance=1e-6,
-                          init=ag.INIT_UNIFORM, history=True)
-        out["time_to_l1_1e-6"] = {"plain_f32_ms": round(float(run.hist_ms.sum()), 3),
-                                  "iterations": run.executed,
-                                  "final_l1": run.error_total}
-        log(f"time to L1<=1e-6: {run.executed} iterations, {run.hist_ms.sum():.2f} ms")
-
+        t0 = time.perf_counter()
+        out["time_to_l1_1e-6"] = time_to_l1(ag, dg, g, args.f64_iters)
+        log(f"time to L1<=1e-6: {out['time_to_l1_1e-6']} ({time.perf_counter() - t0:.1f} s)")
     dg.close()
+    if not args.no_batch and args.batch_count > 0:
+        t0 = time.perf_counter()
+        out["batched"] = batched_section(ag, dev, args.batch_count, 10, 16)
+        log(f"batched: {out['batched']} ({time.perf_counter() - t0:.1f} s)")
+    if not args.no_ppr:
+        t0 = time.perf_counter()
+        out["ppr_64"] = ppr_section(ag, dev, args.ppr_scale, 64)
+        log(f"ppr: {out['ppr_64']} ({time.perf_counter() - t0:.1f} s)")
     dev.close()
     if not args.no_cpu:
         t0 = time.perf_counter()

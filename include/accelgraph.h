@@ -86,6 +86,7 @@ typedef struct ag_plan_params {
     uint32_t order;            /* row order inside a bin: 0 = vertex id, 1 = out-degree desc */
     uint32_t cold_below_bin;   /* 0: off; k: gathers of sources in rows of bins < k use
                                   non-temporal loads (they stop evicting hot L2 lines)    */
+    uint32_t wg_per_cu;        /* resident workgroups per CU of the persistent grid (0 = auto) */
 } ag_plan_params;
 
 void ag_plan_params_default(ag_plan_params *p);
@@ -122,6 +123,15 @@ int ag_graph_free(ag_dgraph *g);
 int ag_graph_set_fixed_weights(ag_dgraph *g, const uint32_t *inv_edges_idx,
                                const uint32_t *inv_degree, const uint32_t *wq);
 int ag_graph_get_info(const ag_dgraph *g, ag_graph_info *info);
+
+/* Autotuner: builds each candidate plan for this graph, times `iterations` FP32 PageRank-pull
+ * iterations (median of `repeats` runs, HIP events) and reports the fastest candidate. A plan
+ * changes only the schedule -- results are identical for the integer paths and within FP32
+ * rounding (tree order outside the thread bin) for the float paths. ms_out[count] may be NULL. */
+int ag_autotune_plan(ag_device *dev, uint32_t n, uint32_t m, const uint32_t *out_degree,
+                     const uint32_t *inv_edges_idx, const uint32_t *inv_degree,
+                     const uint32_t *inv_src, const ag_plan_params *candidates, uint32_t count,
+                     uint32_t iterations, uint32_t repeats, double *ms_out, uint32_t *best);
 
 /* ---------------------------------------------------------------- PageRank pull */
 

@@ -434,3 +434,29 @@ def test_ppr_64_columns(device, rmat12, storage, pname):
     say(f"  iterations={res.iterations} refine@{res.refine_iteration} "
         f"max relative err vs FP64 {worst:.2e}")
     dg.close()
+
+
+# ------------------------------------------------------------------ autotuner
+
+def test_autotuner_choice_keeps_results(device, tmp_path):
+    """The tuned plan runs the same computation: fixed-point bit-exact and FP32 within
+    tolerance of the oracle; the table answers the second call."""
+    from accelgraph_amd import autotune
+    say("autotuner")
+    g = ag.Graph.rmat(14)
+    table = str(tmp_path / "tuning.json")
+    params, rep = autotune.autotune(device, g, iterations=3, repeats=3, passes=1,
+                                    table_path=table)
+    assert not rep["cached"] and rep["ms_per_iteration"] <= rep["default_ms_per_iteration"]
+    params2, rep2 = autotune.autotune(device, g, table_path=table)
+    assert rep2["cached"] and autotune._params_dict(params2) == autotune._params_dict(params)
+    say(f"  tuned {autotune._params_dict(params)}: {rep['ms_per_iteration']:.4f} ms vs default "
+        f"{rep['default_ms_per_iteration']:.4f} ms over {rep['trials']} trials")
+    dg = device.upload(g, params=params)
+    fx = dg.pagerank(kind=32, max_iterations=60, epsilon=1e-4)
+    ref = ref_pr(g, kind=32, max_iterations=60, epsilon=1e-4)
+    np.testing.assert_array_equal(fx.ranks.view(np.uint32), ref.ranks.view(np.uint32))
+    f = dg.pagerank(max_iterations=100, epsilon=1e-4)
+    rf = ref_pr(g, max_iterations=100, epsilon=1e-4)
+    assert np.abs(f.ranks - rf.ranks).max() <= FP32_TOL
+    dg.close()

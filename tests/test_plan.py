@@ -98,7 +98,8 @@ def check_plan(g, p):
 
 
 SMALL = dict(thread_max_deg=2, group16_max_deg=6, wave_max_deg=20, block_max_deg=60,
-             tile_rows=7, tile_nnz=9, segment_nnz=256, order=0, cold_below_bin=0)
+             tile_rows=7, tile_nnz=9, segment_nnz=256, order=0, cold_below_bin=0,
+             wg_per_cu=0)
 
 
 @pytest.mark.parametrize("params", [None, SMALL, dict(SMALL, order=1)],
