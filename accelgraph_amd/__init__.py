@@ -20,7 +20,8 @@ from ._abi import (BIN_NAMES, GATHER_F32, GATHER_U8, GATHER_U16, GATHER_U32, GAT
                    STORE_BF16, STORE_F16, STORE_F32, PPR_UNIFORM, GraphInfo, PlanParams,
                    PrOptions, PrResult)
 
-__all__ = ["lib", "AGError", "Graph", "Device", "DeviceGraph", "PrRun", "Batch", "plan_params",
+__all__ = ["lib", "AGError", "Graph", "Device", "DeviceGraph", "PrRun", "Batch", "MLP",
+           "plan_params",
            "RMAT_A", "RMAT_B", "RMAT_C", "DEFAULT_SEED"]
 
 RMAT_A, RMAT_B, RMAT_C = 0.57, 0.19, 0.19
@@ -320,9 +321,17 @@ class DeviceGraph:
         _check(lib().ag_graph_get_info(self._h, C.byref(i)), "ag_graph_get_info")
         return i
 
-    def pagerank(self, history: bool = True, scaled: bool = False, **kw) -> PrRun:
+    def features(self) -> np.ndarray:
+        """The warm-start features of every vertex (original order), shape [n, 5]."""
+        X = np.zeros((self.n, 5), np.float32)
+        _check(lib().ag_mlp_features(self._h, X.ctypes.data_as(C.POINTER(C.c_float))),
+               "ag_mlp_features")
+        return X
+
+    def pagerank(self, history: bool = True, scaled: bool = False, warm=None, **kw) -> PrRun:
         """Run PageRank pull; kw are ag_pr_options fields (kind, max_iterations, epsilon,
-        stop_rule, l1_tolerance, init, init_ranks, storage, refine_l1, chunk)."""
+        stop_rule, l1_tolerance, init, init_ranks, storage, refine_l1, chunk). warm = an MLP
+        starts the run from its predicted ranks (uniform init, FP32), with the fallback rule."""
         init_ranks = kw.get("init_ranks")
         ir = None
         if init_ranks is not None:
@@ -336,6 +345,14 @@ class DeviceGraph:
         r = PrResult()
         maxit, chunk = o.max_iterations, (o.chunk or 8)
         enq = 0
+        wres = None
+        if warm is not None:
+            wres = _abi.WarmResult()
+            st = lib().ag_pr_warm_start(s, warm.handle, C.byref(wres))
+            if st != _abi.AG_OK:
+                lib().ag_pr_end(s, None, None)
+                _check(st, "ag_pr_warm_start")
+            enq = 3
         try:
             while enq < maxit:
                 k = min(chunk, maxit - enq)
@@ -367,6 +384,7 @@ class DeviceGraph:
                     hist_ms=hm[:rows.value])
         if scaled:
             run.scaled = sc
+        run.warm = wres
         return run
 
     def ppr(self, seeds, **kw):
@@ -466,6 +484,100 @@ class Batch:
                                        C.byref(ms)), "ag_batch_pagerank")
         per = [ranks[self.vert_off[i]:self.vert_off[i + 1]] for i in range(G)]
         return per, its, err, ms.value
+
+
+class MLP:
+    """The warm-start network on the device (see include/accelgraph.h, learned warm start)."""
+
+    def __init__(self, dev: "Device", seed: int = 0):
+        self.dev = dev
+        h = C.c_void_p()
+        _check(lib().ag_mlp_create(dev.handle, seed, C.byref(h)), "ag_mlp_create")
+        self._h = h
+        self.P = lib().ag_mlp_param_count()
+        self.norm = (np.zeros(5, np.float32), np.ones(5, np.float32), 0.0, 1.0)
+        dev._graphs.add(self)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self) -> None:
+        if self._h:
+            lib().ag_mlp_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.P, np.float32)
+        _check(lib().ag_mlp_get_params(self._h, _ptr(p, C.c_float)), "ag_mlp_get_params")
+        return p
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.P
+        _check(lib().ag_mlp_set_params(self._h, _ptr(p, C.c_float)), "ag_mlp_set_params")
+
+    def set_normalization(self, in_mean, in_std, y_mean, y_std) -> None:
+        m = np.ascontiguousarray(in_mean, np.float32)
+        s = np.ascontiguousarray(in_std, np.float32)
+        _check(lib().ag_mlp_set_normalization(self._h, _ptr(m, C.c_float), _ptr(s, C.c_float),
+                                              float(y_mean), float(y_std)),
+               "ag_mlp_set_normalization")
+        self.norm = (m.copy(), s.copy(), float(y_mean), float(y_std))
+
+    def fit_normalization(self, X, y) -> None:
+        X = np.asarray(X, np.float64)
+        y = np.asarray(y, np.float64)
+        self.set_normalization(X.mean(0), np.maximum(X.std(0), 1e-3), y.mean(),
+                               max(y.std(), 1e-3))
+
+    def loss_grad(self, X, y):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        g = np.zeros(self.P, np.float32)
+        loss = C.c_double(0.0)
+        _check(lib().ag_mlp_loss_grad(self._h, _ptr(X, C.c_float), _ptr(y, C.c_float),
+                                      y.size, C.byref(loss), _ptr(g, C.c_float)),
+               "ag_mlp_loss_grad")
+        return loss.value, g
+
+    def train(self, X, y, epochs: int = 5, batch: int = 8192, lr: float = 2e-3,
+              seed: int = 0) -> np.ndarray:
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        losses = np.zeros(epochs, np.float32)
+        _check(lib().ag_mlp_train(self._h, _ptr(X, C.c_float), _ptr(y, C.c_float), y.size,
+                                  epochs, batch, lr, seed, _ptr(losses, C.c_float)),
+               "ag_mlp_train")
+        return losses
+
+    def predict(self, X) -> np.ndarray:
+        X = np.ascontiguousarray(X, np.float32)
+        out = np.zeros(X.shape[0], np.float32)
+        _check(lib().ag_mlp_predict(self._h, _ptr(X, C.c_float), X.shape[0],
+                                    _ptr(out, C.c_float)), "ag_mlp_predict")
+        return out
+
+    def save(self, path: str) -> None:
+        import json
+        m, s, ym, ys = self.norm
+        with open(path, "w") as f:
+            json.dump({"params": [float(v) for v in self.get_params()],
+                       "in_mean": [float(v) for v in m], "in_std": [float(v) for v in s],
+                       "y_mean": ym, "y_std": ys}, f)
+
+    def load(self, path: str) -> None:
+        import json
+        with open(path) as f:
+            d = json.load(f)
+        self.set_params(np.array(d["params"], np.float32))
+        self.set_normalization(d["in_mean"], d["in_std"], d["y_mean"], d["y_std"])
 
 
 class PrSession:

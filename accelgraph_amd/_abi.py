@@ -73,6 +73,10 @@ class PprResult(C.Structure):
                 ("refine_iteration", i32)]
 
 
+class WarmResult(C.Structure):
+    _fields_ = [("fallback", u32), ("uniform_l1", f64), ("warm_l1", f64), ("ms", f64)]
+
+
 class Vertex(C.Structure):
     _fields_ = [("out_degree", P(u32)), ("in_degree", P(u32)), ("edges_idx", P(u32))]
 
@@ -155,6 +159,17 @@ SIGNATURES = {
     "ag_pull_gather": (C.c_int, [VP, u32, VP, VP]),
     "ag_ppr_options_default": (None, [P(PprOptions)]),
     "ag_ppr_pull": (C.c_int, [VP, u32, P(u32), P(PprOptions), P(f32), P(f64), P(PprResult)]),
+    "ag_mlp_param_count": (u32, []),
+    "ag_mlp_create": (C.c_int, [VP, u32, P(VP)]),
+    "ag_mlp_free": (C.c_int, [VP]),
+    "ag_mlp_set_params": (C.c_int, [VP, P(f32)]),
+    "ag_mlp_get_params": (C.c_int, [VP, P(f32)]),
+    "ag_mlp_set_normalization": (C.c_int, [VP, P(f32), P(f32), f32, f32]),
+    "ag_mlp_loss_grad": (C.c_int, [VP, P(f32), P(f32), u32, P(f64), P(f32)]),
+    "ag_mlp_train": (C.c_int, [VP, P(f32), P(f32), u32, u32, u32, f32, u32, P(f32)]),
+    "ag_mlp_predict": (C.c_int, [VP, P(f32), u32, P(f32)]),
+    "ag_mlp_features": (C.c_int, [VP, P(f32)]),
+    "ag_pr_warm_start": (C.c_int, [VP, VP, P(WarmResult)]),
     "ag_batch_upload": (C.c_int, [VP, u32, P(u32), P(u32), P(u32), P(u32), P(u32), P(u32),
                                   P(VP)]),
     "ag_batch_free": (C.c_int, [VP]),

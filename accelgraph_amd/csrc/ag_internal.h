@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <time.h>
 
 #include "accelgraph.h"
 #include "../host/ag_plan.h"
@@ -82,6 +83,35 @@ struct PrState {
     uint32_t stall;         // 16-bit iterations since best_err last improved by 1%
     uint32_t pad;
 };
+
+// A PageRank session (ag_pr_begin .. ag_pr_end).
+struct ag_pr_session {
+    ag_dgraph *g;
+    ag_pr_options o;
+    uint32_t kind_code;         // 0 f32, 1 bf16, 2 f16, 64/32/16/8
+    size_t celem;               // bytes per stored contribution
+    PrState *st;                // device
+    PrState *st_host;           // pinned
+    double *part_err;
+    uint32_t *part_act;
+    double *hist_err;
+    uint32_t *hist_act;
+    float *rank;
+    void *c[2];
+    void *c16[2];
+    double *fin_part;
+    double *fin_sum;
+    float *user;
+    uint32_t enqueued;
+    hipEvent_t *ev;             // per-launch events when record_history (max_iterations + 1)
+    hipEvent_t step_ev[2];
+    uint32_t step_launches;
+    uint32_t ev_count;
+    struct timespec t0;  // wall clock at ag_pr_begin
+};
+
+// Enqueue one PageRank iteration of a session (main + finish kernel).
+int ag_pr_launch_iteration(ag_pr_session *s);
 
 int ag_set_error(int status, const char *fmt, ...);
 int ag_hip_fail(hipError_t e, const char *what, const char *file, int line);

@@ -430,30 +430,7 @@ static inline float pr_base(void)
     return 1.0f - d;
 }
 
-struct ag_pr_session {
-    ag_dgraph *g;
-    ag_pr_options o;
-    uint32_t kind_code;         // 0 f32, 1 bf16, 2 f16, 64/32/16/8
-    size_t celem;               // bytes per stored contribution
-    PrState *st;                // device
-    PrState *st_host;           // pinned
-    double *part_err;
-    uint32_t *part_act;
-    double *hist_err;
-    uint32_t *hist_act;
-    float *rank;
-    void *c[2];
-    void *c16[2];
-    double *fin_part;
-    double *fin_sum;
-    float *user;
-    uint32_t enqueued;
-    hipEvent_t *ev;             // per-launch events when record_history (max_iterations + 1)
-    hipEvent_t step_ev[2];
-    uint32_t step_launches;
-    uint32_t ev_count;
-    struct timespec t0;
-};
+
 
 extern "C" void ag_pr_options_default(ag_pr_options *o)
 {
@@ -641,7 +618,7 @@ extern "C" int ag_pr_begin(ag_dgraph *g, const ag_pr_options *opts, ag_pr_sessio
     return AG_OK;
 }
 
-static int launch_iteration(ag_pr_session *s)
+int ag_pr_launch_iteration(ag_pr_session *s)
 {
     ag_dgraph *g = s->g;
     hipStream_t stream = g->dev->stream;
@@ -725,7 +702,7 @@ extern "C" int ag_pr_step(ag_pr_session *s, uint32_t iterations)
     for (uint32_t i = 0; i < iterations && s->enqueued < maxit; i++) {
         if (s->ev)
             AG_HIP(hipEventRecord(s->ev[s->enqueued], stream));
-        int st = launch_iteration(s);
+        int st = ag_pr_launch_iteration(s);
         if (st != AG_OK)
             return st;
         s->enqueued++;

@@ -241,6 +241,54 @@ void ag_ppr_options_default(ag_ppr_options *o);
 int ag_ppr_pull(ag_dgraph *g, uint32_t k, const uint32_t *seeds, const ag_ppr_options *opts,
                 float *ranks_out, double *col_error, ag_ppr_result *res);
 
+/* ---------------------------------------------------------------- learned warm start */
+
+/* A 3-layer MLP (5 inputs -> 64 -> 64 -> 1, ReLU) that predicts, for each vertex, the log
+ * ratio of its converged scaled rank to its scaled rank after two iterations, from five
+ * features: log(1 + in-degree), log(1 + out-degree), log(1 + mean in-degree of the
+ * in-neighbours), and the log scaled ranks after one and two FP32 PageRank iterations from the
+ * uniform start. Weights are kept in FP32 and fed to BF16 MFMA (FP32 accumulate); training is
+ * on the GPU with the library's own forward/backward kernels and Adam. Parameter layout (FP32):
+ * W1[16][64] (rows 5..15 unused), b1[64], W2[64][64], b2[64], w3[64], b3. */
+#define AG_MLP_FEATURES 5
+#define AG_MLP_HIDDEN 64
+
+typedef struct ag_mlp ag_mlp;
+
+uint32_t ag_mlp_param_count(void);
+int ag_mlp_create(ag_device *dev, uint32_t seed, ag_mlp **out);      /* He-uniform init */
+int ag_mlp_free(ag_mlp *m);
+int ag_mlp_set_params(ag_mlp *m, const float *params);
+int ag_mlp_get_params(ag_mlp *m, float *params);
+/* Inputs are standardised as (x - mean) / std and the target as (y - y_mean) / y_std. */
+int ag_mlp_set_normalization(ag_mlp *m, const float *in_mean, const float *in_std,
+                             float y_mean, float y_std);
+/* Mean-squared loss (standardised units) and its gradient on S samples, no update. */
+int ag_mlp_loss_grad(ag_mlp *m, const float *X, const float *y, uint32_t S, double *loss,
+                     float *grad);
+/* Adam training: epochs over S samples in minibatches of `batch`, shuffled by `seed`;
+ * loss_per_epoch[epochs] (may be NULL) receives each epoch's mean minibatch loss. */
+int ag_mlp_train(ag_mlp *m, const float *X, const float *y, uint32_t S, uint32_t epochs,
+                 uint32_t batch, float lr, uint32_t seed, float *loss_per_epoch);
+/* Predictions in target units for S samples. */
+int ag_mlp_predict(ag_mlp *m, const float *X, uint32_t S, float *y_out);
+/* The five features of every vertex of an uploaded graph, original order, X_out[n][5]. */
+int ag_mlp_features(ag_dgraph *g, float *X_out);
+
+typedef struct ag_warm_result {
+    uint32_t fallback;      /* 1 when the predicted start was rejected for the uniform one */
+    double uniform_l1;      /* L1 delta of the second iteration from the uniform start      */
+    double warm_l1;         /* L1 delta of one iteration from the predicted start          */
+    double ms;              /* device time of features + inference + check                 */
+} ag_warm_result;
+
+/* Warm start of a fresh FP32 session begun with AG_INIT_UNIFORM: runs two iterations
+ * (features), infers the start vector r2 * exp(prediction), rescales it so its total mass
+ * meets the fixed-point balance (restart mass (1 - Damp) N, dangling mass dropped), runs one
+ * iteration from it and keeps it only if its L1 delta beats the uniform start's second delta
+ * (else continues from the uniform run). Those iterations count in the session. */
+int ag_pr_warm_start(ag_pr_session *s, ag_mlp *m, ag_warm_result *res);
+
 /* ---------------------------------------------------------------- many small graphs */
 
 /* A batch of independent graphs resident in HBM, solved together by one persistent launch

@@ -460,3 +460,100 @@ def test_autotuner_choice_keeps_results(device, tmp_path):
     rf = ref_pr(g, max_iterations=100, epsilon=1e-4)
     assert np.abs(f.ranks - rf.ranks).max() <= FP32_TOL
     dg.close()
+
+
+# ------------------------------------------------------------------ learned warm start
+
+def _mlp_case(seed, S):
+    from oracle import mlp as omlp
+    rng = np.random.default_rng(seed)
+    X = np.column_stack([rng.normal(2, 1, S), rng.normal(2, 1, S), rng.normal(3, 1, S),
+                         rng.normal(0, 1, S), rng.normal(0, 1, S)]).astype(np.float32)
+    y = (0.7 * X[:, 0] - 0.2 * X[:, 3] + 0.5 * X[:, 4]).astype(np.float32)
+    p = (rng.normal(size=omlp.P_COUNT) * 0.15).astype(np.float32)
+    p[omlp.P_W1 + 5 * omlp.H:omlp.P_B1] = 0.0
+    norm = (X.mean(0), X.std(0), float(y.mean()), float(y.std()))
+    return X, y, p, norm
+
+
+@pytest.mark.parametrize("S", [16, 100, 1000])
+def test_mlp_gradient_matches_oracle(device, S):
+    """Forward loss and every gradient of the BF16-MFMA kernels against the FP64 oracle that
+    applies the same BF16 operand roundings."""
+    from oracle import mlp as omlp
+    say(f"mlp gradient S={S}")
+    X, y, p, norm = _mlp_case(S, S)
+    m = ag.MLP(device)
+    m.set_params(p)
+    m.set_normalization(*norm)
+    loss, g = m.loss_grad(X, y)
+    oloss, og, oyhat = omlp.forward_backward(p, X, y, *norm, bf16_ops=True)
+    assert abs(loss - oloss) <= 1e-4 * oloss + 1e-7
+    err = np.linalg.norm(g - og) / np.linalg.norm(og)
+    say(f"  loss {loss:.6f} vs {oloss:.6f}; relative gradient error {err:.2e}")
+    assert err <= 5e-3
+    big = np.abs(og) > 1e-3 * np.abs(og).max()
+    np.testing.assert_allclose(g[big], og[big], rtol=5e-2)
+    pred = m.predict(X)
+    np.testing.assert_allclose(pred, oyhat * norm[3] + norm[2], rtol=1e-3, atol=1e-3 * norm[3])
+    m.close()
+
+
+def test_mlp_training_reduces_loss(device):
+    say("mlp training")
+    X, y, p, norm = _mlp_case(7, 20000)
+    m = ag.MLP(device, seed=3)
+    m.fit_normalization(X, y)
+    l0, _ = m.loss_grad(X, y)
+    losses = m.train(X, y, epochs=6, batch=1024, lr=3e-3, seed=1)
+    l1, _ = m.loss_grad(X, y)
+    say(f"  loss {l0:.4f} -> {l1:.4f} (epochs {losses})")
+    assert l1 < 0.1 * l0 and losses[-1] < losses[0]
+    m.close()
+
+
+def test_features_match_definition(device, rmat12):
+    """The GPU features against their definition computed from the graph and the oracle's
+    first two FP32 iterations."""
+    say("warm-start features")
+    g = rmat12
+    dg = device.upload(g)
+    X = dg.features()
+    dg.close()
+    np.testing.assert_allclose(X[:, 0], np.log1p(g.in_degree.astype(np.float64)), rtol=1e-6)
+    np.testing.assert_allclose(X[:, 1], np.log1p(g.out_degree.astype(np.float64)), rtol=1e-6)
+    indeg = g.in_degree.astype(np.float64)
+    nb = np.zeros(g.n)
+    for v in range(g.n):
+        b, d = g.inv_idx[v], g.in_degree[v]
+        if d:
+            nb[v] = np.log1p(indeg[g.inv_src[b:b + d]].mean())
+    np.testing.assert_allclose(X[:, 2], nb, rtol=1e-5, atol=1e-6)
+    r1 = ref_pr(g, max_iterations=1, stop_rule=2, init=1).scaled
+    r2 = ref_pr(g, max_iterations=2, stop_rule=2, init=1).scaled
+    np.testing.assert_allclose(X[:, 3], np.log(r1), rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(X[:, 4], np.log(r2), rtol=1e-5, atol=1e-6)
+
+
+def test_warm_start_gain_and_fallback(device):
+    """A network trained (on the GPU) on small graphs' solutions starts a held-out graph closer
+    to its fixed point: fewer iterations to L1 <= 1e-6 at the same accuracy. An untrained
+    network is rejected by the fallback rule and the run still converges."""
+    from accelgraph_amd import warmstart
+    say("warm start gain")
+    m, losses, _ = warmstart.train(device, count=16, lo=9, hi=12, epochs=6, batch=4096,
+                                   lr=3e-3, log=say)
+    g = ag.Graph.rmat(14, seed=999)
+    f64 = f64_ref(g)
+    res = warmstart.evaluate(device, g, m, f64)
+    say(f"  {res}")
+    assert res["warm"]["fallback"] == 0
+    assert res["warm"]["iterations"] < res["uniform"]["iterations"]
+    assert res["warm"]["max_abs_err_vs_f64"] <= 2 * res["uniform"]["max_abs_err_vs_f64"] + 1e-9
+    bad = ag.MLP(device, seed=5)      # untrained: predicts nonsense ranks
+    bad.set_normalization(np.zeros(5), np.ones(5), 40.0, 1.0)
+    res2 = warmstart.evaluate(device, g, bad, f64)
+    assert res2["warm"]["fallback"] == 1
+    assert res2["warm"]["max_abs_err_vs_f64"] <= 2 * res2["uniform"]["max_abs_err_vs_f64"] + 1e-9
+    m.close()
+    bad.close()
