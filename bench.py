@@ -59,7 +59,7 @@ def cpu_baseline(g, budget_s: float) -> dict:
             "ms_per_iteration": round(dt * 1e3, 3)}
 
 
-def time_to_l1(ag, dg, g, f64_iters: int) -> dict:
+def time_to_l1(ag, dg, g, f64) -> dict:
     """Plain FP32 vs BF16-stored / FP32-refined time to L1 <= 1e-6 from the uniform start,
     each checked against the FP64 oracle."""
     import numpy as np
@@ -73,16 +73,26 @@ def time_to_l1(ag, dg, g, f64_iters: int) -> dict:
                             "iterations": mixed.executed, "final_l1": mixed.error_total,
                             "refine_iteration": mixed.refine_iteration}}
     out["bf16_speedup"] = round(out["plain_f32"]["ms"] / max(out["bf16_refined"]["ms"], 1e-9), 3)
-    if f64_iters:
-        import oracle
-        f64 = oracle.pagerank_f64(g.out_degree, g.inv_idx, g.in_degree, g.inv_src, tol=1e-13,
-                                  max_iterations=f64_iters,
-                                  threads=len(os.sched_getaffinity(0)))
+    if f64 is not None:
         for key, run in (("plain_f32", plain), ("bf16_refined", mixed)):
             d = np.abs(run.ranks.astype(np.float64) - f64)
             out[key]["max_abs_err_vs_f64"] = float(d.max())
             out[key]["l1_err_vs_f64"] = float(d.sum())
     return out
+
+
+def warm_section(ag, dev, g, f64) -> dict:
+    """Learned warm start on the bench graph (held out: the network was trained on graphs of
+    scale 14-18), time-to-L1 <= 1e-6 including features and inference, vs the uniform start."""
+    from accelgraph_amd import warmstart
+    wpath = os.path.join(ROOT, "accelgraph_amd", "warmstart_weights.json")
+    m = ag.MLP(dev)
+    m.load(wpath)
+    res = warmstart.evaluate(dev, g, m, f64)
+    with open(wpath) as f:
+        res["training"] = json.load(f).get("training")
+    m.close()
+    return res
 
 
 def batched_section(ag, dev, count: int, lo: int, hi: int) -> dict:
@@ -162,6 +172,7 @@ def main() -> int:
     ap.add_argument("--f64-iters", type=int, default=150,
                     help="FP64 oracle iterations for the error report (0 = skip)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-graph batch")
+    ap.add_argument("--no-warm", action="store_true", help="skip the learned warm start")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--no-ppr", action="store_true", help="skip the 64-column PPR")
     ap.add_argument("--ppr-scale", type=int, default=22)
@@ -246,9 +257,23 @@ def main() -> int:
 
     if not args.no_ttl:
         t0 = time.perf_counter()
-        out["time_to_l1_1e-6"] = time_to_l1(ag, dg, g, args.f64_iters)
+        f64 = None
+        if args.f64_iters:
+            import oracle
+            f64 = oracle.pagerank_f64(g.out_degree, g.inv_idx, g.in_degree, g.inv_src,
+                                      tol=1e-13, max_iterations=args.f64_iters,
+                                      threads=len(os.sched_getaffinity(0)))
+        log(f"FP64 oracle ({time.perf_counter() - t0:.1f} s)")
+        out["time_to_l1_1e-6"] = time_to_l1(ag, dg, g, f64)
         log(f"time to L1<=1e-6: {out['time_to_l1_1e-6']} ({time.perf_counter() - t0:.1f} s)")
-    dg.close()
+        dg.close()
+        if not args.no_warm:
+            t0 = time.perf_counter()
+            out["time_to_l1_1e-6"]["learned_warm_start"] = warm_section(ag, dev, g, f64)
+            log(f"warm start: {out['time_to_l1_1e-6']['learned_warm_start']} "
+                f"({time.perf_counter() - t0:.1f} s)")
+    else:
+        dg.close()
     if not args.no_batch and args.batch_count > 0:
         t0 = time.perf_counter()
         out["batched"] = batched_section(ag, dev, args.batch_count, 10, 16)
