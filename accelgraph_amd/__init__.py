@@ -17,8 +17,8 @@ from . import _abi
 from ._abi import (BIN_NAMES, GATHER_F32, GATHER_U8, GATHER_U16, GATHER_U32, GATHER_U64,
                    INIT_REFERENCE, INIT_UNIFORM, INIT_USER, KIND_F32, KIND_FIXED8,
                    KIND_FIXED16, KIND_FIXED32, KIND_FIXED64, STOP_ACTIVE, STOP_L1, STOP_NONE,
-                   STORE_BF16, STORE_F16, STORE_F32, PPR_UNIFORM, GraphInfo, PlanParams,
-                   PrOptions, PrResult)
+                   STORE_BF16, STORE_F16, STORE_F32, PPR_UNIFORM, SRC_BLOCK_AUTO,
+                   SRC_BLOCK_OFF, GraphInfo, PlanParams, PrOptions, PrResult)
 
 __all__ = ["lib", "AGError", "Graph", "Device", "DeviceGraph", "PrRun", "Batch", "MLP",
            "plan_params",

@@ -22,6 +22,7 @@ AG_ERR_INVALID, AG_ERR_NO_DEVICE, AG_ERR_HIP, AG_ERR_OOM, AG_ERR_TIMEOUT, AG_ERR
     1, 2, 3, 4, 5, 6
 
 AG_NBINS = 5
+SRC_BLOCK_AUTO, SRC_BLOCK_OFF = 0, 255
 BIN_NAMES = ["thread", "group16", "wave", "block", "hub"]
 STOP_ACTIVE, STOP_L1, STOP_NONE = 0, 1, 2
 INIT_REFERENCE, INIT_UNIFORM, INIT_USER = 0, 1, 2
@@ -34,15 +35,16 @@ PPR_UNIFORM = 0xFFFFFFFF
 class PlanParams(C.Structure):
     _fields_ = [("thread_max_deg", u32), ("group16_max_deg", u32), ("wave_max_deg", u32),
                 ("block_max_deg", u32), ("tile_rows", u32), ("tile_nnz", u32),
-                ("segment_nnz", u32), ("order", u32), ("cold_below_bin", u32),
-                ("wg_per_cu", u32)]
+                ("segment_nnz", u32), ("order", u32), ("src_block_bits", u32),
+                ("wg_per_cu", u32), ("blocked_min_bin", u32)]
 
 
 class GraphInfo(C.Structure):
     _fields_ = [("num_vertices", u32), ("num_edges", u32), ("bin_rows", u32 * AG_NBINS),
                 ("bin_edges", u64 * AG_NBINS), ("bin_blocks", u32 * AG_NBINS),
                 ("num_tiles", u32), ("num_segments", u32), ("has_weights", u32),
-                ("device_bytes", u64), ("params", PlanParams)]
+                ("device_bytes", u64), ("params", PlanParams), ("src_block_bits", u32),
+                ("src_blocks", u32), ("virtual_rows", u32), ("blocked_row0", u32)]
 
 
 class PrOptions(C.Structure):

@@ -1,12 +1,13 @@
 """Plan autotuner: picks the pull plan's bin thresholds and launch shape for a graph.
 
-Coordinate descent over a small grid: starting from the defaults, each knob in turn is set to
-every value of its grid while the others stay fixed, each candidate plan is timed on the GPU by
+Coordinate descent over a small grid (source blocking, bin thresholds, occupancy, row order):
+starting from the defaults, each knob in turn is set to every value of its grid while the
+others stay fixed, each candidate plan is timed on the GPU by
 ag_autotune_plan (median of `repeats` runs of `iterations` FP32 iterations), and the fastest
 value is kept; two passes. Results are cached in a JSON table keyed by a graph signature
 (vertex and edge counts plus the log2 histogram of in-degrees), so the same graph shape is
 tuned once. A plan changes only the schedule: integer results are identical under every
-candidate and FP32 results differ only by summation order outside the thread bin.
+candidate and FP32 results differ only by summation order.
 """
 from __future__ import annotations
 
@@ -20,11 +21,13 @@ import numpy as np
 from . import _abi, _check, lib, plan_params
 
 GRID = {
+    "src_block_bits": [255, 18, 19, 20],   # 255: no source blocking
+    "blocked_min_bin": [0, 1],
     "thread_max_deg": [8, 16, 32],
     "group16_max_deg": [64, 128, 256],
     "wave_max_deg": [1024, 2048, 4096],
     "block_max_deg": [8192, 16384, 65536],
-    "wg_per_cu": [4, 6, 8],
+    "wg_per_cu": [3, 4, 5, 6, 8],
     "order": [0, 1],
 }
 
@@ -102,6 +105,8 @@ def autotune(dev, graph, iterations: int = 5, repeats: int = 3, passes: int = 2,
                 if not (d["thread_max_deg"] <= d["group16_max_deg"] <= d["wave_max_deg"]
                         <= d["block_max_deg"]):
                     continue
+                if knob == "src_block_bits" and v != 255 and (graph.n + (1 << v) - 1) >> v > 4096:
+                    continue   # more source blocks than a plan allows
                 dicts.append(d)
                 cands.append(plan_params(**d))
             ms, b = _time(dev, arrays, cands, iterations, repeats)

@@ -44,6 +44,10 @@ struct ag_dgraph {
     uint32_t *seg_row, *seg_lo, *seg_hi;  // [n_segs]
     uint32_t *hub_seg0;  // [n_hubs+1]
     uint64_t *hub_part;    // [n_segs] 8-byte partial sums (float or u64 bits)
+    // source blocking of the PageRank pull (blk.bits == 0: none); host copies of the
+    // schedule scalars and HBM arrays of the block plan
+    ag_block_plan blk;   // pointer members are device pointers here
+    uint64_t *vpart;     // [blk.nv] partial sum of every virtual row (float or u64 bits)
     // scratch for permuted vectors (x, y) of SpMV / gather, sized lazily
     void *vec_a;
     void *vec_b;
@@ -66,9 +70,19 @@ struct PullGeom {
     uint32_t slot_blk0[AG_NSLOTS + 1];
 };
 
+// Source-blocked pull: virtual rows, the combine schedule and the work items of each slice.
+struct BlockGeom {
+    PullGeom V;              // off = voff, src = vsrc (virtual rows)
+    PullGeom C;              // off = coff, src = cidx, combine bins, tiles and slots
+    const uint32_t *item;    // [2 * n_items]
+    uint64_t *part;          // [nv]
+    uint32_t item_off[AG_NXCD + 1];
+};
+
 // Persistent-grid size of a pull launch: resident workgroups per CU x CUs, capped by the
-// number of work items.
-uint32_t ag_pull_grid(const ag_dgraph *g, uint32_t lds_bytes);
+// number of work items. Per CU: the plan's wg_per_cu, else default_per_cu, never more than
+// LDS allows.
+uint32_t ag_pull_grid(const ag_dgraph *g, uint32_t lds_bytes, uint32_t default_per_cu = 4u);
 
 // Device-side iteration state of a PageRank session.
 struct PrState {
@@ -125,4 +139,10 @@ int ag_wait_stream(ag_device *dev);
     } while (0)
 
 PullGeom ag_make_geom(const ag_dgraph *g);
+// Regular schedule of the rows a source-blocked graph leaves unblocked (AG_ITEM_REG items).
+PullGeom ag_make_unblocked_geom(const ag_dgraph *g);
+BlockGeom ag_make_block_geom(const ag_dgraph *g);
+// Grids of the two passes of a source-blocked iteration (the first a multiple of AG_NXCD).
+uint32_t ag_blocked_grid(const ag_dgraph *g, uint32_t lds_bytes);
+uint32_t ag_combine_grid(const ag_dgraph *g, uint32_t lds_bytes);
 int ag_ensure_vec(ag_dgraph *g, size_t bytes);

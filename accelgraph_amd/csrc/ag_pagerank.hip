@@ -34,14 +34,6 @@ struct bf16s {
     uint16_t bits;
 };
 
-// Non-temporal loads of one stored contribution.
-__device__ __forceinline__ float ld_nt(const float *p) { return __builtin_nontemporal_load(p); }
-__device__ __forceinline__ bf16s ld_nt(const bf16s *p)
-{
-    return bf16s{__builtin_nontemporal_load((const uint16_t *)p)};
-}
-__device__ __forceinline__ _Float16 ld_nt(const _Float16 *p) { return __builtin_nontemporal_load(p); }
-
 __device__ __forceinline__ float ld_f32(float v) { return v; }
 __device__ __forceinline__ float ld_f32(bf16s v) { return __uint_as_float((uint32_t)v.bits << 16); }
 __device__ __forceinline__ float ld_f32(_Float16 v) { return (float)v; }
@@ -69,7 +61,8 @@ struct PrCtl {
     uint32_t *part_act;    // [nparts]
     double *hist_err;      // [max_iterations]
     uint32_t *hist_act;    // [max_iterations]
-    uint32_t nparts;
+    uint32_t nparts;       // partials of all passes of one iteration
+    uint32_t part_base;    // first partial of this launch's workgroups
     uint32_t stop_rule;
     uint32_t mixed;        // 16-bit storage with f32 refinement
     double l1_tol;
@@ -113,8 +106,8 @@ __device__ __forceinline__ void pr_store_partial(const PrCtl &C, Red red)
     __shared__ uint32_t s_a[AG_WG / AG_WAVE];
     Red t = block_reduce_red<AG_WG>(red, s_e, s_a);
     if (threadIdx.x == 0) {
-        C.part_err[blockIdx.x] = t.err;
-        C.part_act[blockIdx.x] = t.active;
+        C.part_err[C.part_base + blockIdx.x] = t.err;
+        C.part_act[C.part_base + blockIdx.x] = t.active;
     }
 }
 
@@ -144,12 +137,8 @@ struct PrF32Pol {
     const uint32_t *outdeg;
     float base, damp;
     double dn, eps;
-    uint32_t hot_lo;   // sources below this device id are cold: non-temporal gathers
 
-    __device__ __forceinline__ float edge(uint32_t, uint32_t s) const
-    {
-        return s < hot_lo ? ld_f32(ld_nt(&cin[s])) : ld_f32(cin[s]);
-    }
+    __device__ __forceinline__ float edge(uint32_t, uint32_t s) const { return ld_f32(cin[s]); }
     __device__ __forceinline__ void finish(uint32_t r, float sum, Red &red) const
     {
         const float nw = __fadd_rn(base, __fmul_rn(damp, sum));
@@ -170,7 +159,6 @@ struct PrF32Args {
     const uint32_t *outdeg;
     float base, damp;
     double dn, eps;
-    uint32_t hot_lo;
 };
 
 // Runs `body(policy)` with the policy of this iteration's storage mode.
@@ -181,7 +169,7 @@ __device__ __forceinline__ void pr_f32_dispatch(const PrState *st, uint32_t iter
     const uint32_t rd = iter & 1u, wr = rd ^ 1u;
     if (!mixed) {
         body(PrF32Pol<float, float>{(const float *)A.c32[rd], (float *)A.c32[wr], A.rank,
-                                    A.outdeg, A.base, A.damp, A.dn, A.eps, A.hot_lo});
+                                    A.outdeg, A.base, A.damp, A.dn, A.eps});
         return;
     }
     const int32_t ri = *(volatile const int32_t *)&st->refine_iter;
@@ -189,13 +177,13 @@ __device__ __forceinline__ void pr_f32_dispatch(const PrState *st, uint32_t iter
     const bool rf32 = ri >= 0 && (int32_t)iter >= ri + 1;
     if (rf32)
         body(PrF32Pol<float, float>{(const float *)A.c32[rd], (float *)A.c32[wr], A.rank,
-                                    A.outdeg, A.base, A.damp, A.dn, A.eps, A.hot_lo});
+                                    A.outdeg, A.base, A.damp, A.dn, A.eps});
     else if (wf32)
         body(PrF32Pol<HT, float>{(const HT *)A.c16[rd], (float *)A.c32[wr], A.rank, A.outdeg,
-                                 A.base, A.damp, A.dn, A.eps, A.hot_lo});
+                                 A.base, A.damp, A.dn, A.eps});
     else
         body(PrF32Pol<HT, HT>{(const HT *)A.c16[rd], (HT *)A.c16[wr], A.rank, A.outdeg, A.base,
-                              A.damp, A.dn, A.eps, A.hot_lo});
+                              A.damp, A.dn, A.eps});
 }
 
 template <typename HT>
@@ -225,6 +213,70 @@ __device__ __forceinline__ void pr_f32_fin(const PullGeom &G, const PrCtl &C,
     Red red{0.0, 0u};
     pr_f32_dispatch<HT>(st, iter, A, mixed, [&](const auto &p) { pull_hub_finish(G, p, red); });
     pr_close(C, iter, red);
+}
+
+// Source-blocked iteration: pass 1 (partials of the blocked rows, full update of the others),
+// then pass 2 (each blocked row's partials in block order, then the update).
+template <typename HT>
+__device__ __forceinline__ void pr_f32_blk(const PullGeom &U, const BlockGeom &B,
+                                           const PrCtl &C, const PrF32Args &A, bool mixed)
+{
+    __shared__ float s_vals[AG_TILE_NNZ_MAX];
+    __shared__ float s_wave[AG_WG / AG_WAVE];
+    const PrState *st = C.st;
+    if (*(volatile const uint32_t *)&st->done)
+        return;
+    const uint32_t iter = *(volatile const uint32_t *)&st->iter;
+    Red red{0.0, 0u};
+    pr_f32_dispatch<HT>(st, iter, A, mixed, [&](const auto &p) {
+        pull_blocked_items(U, B, p, p, red, s_vals, s_wave);
+    });
+    pr_store_partial(C, red);
+}
+
+template <typename HT>
+__device__ __forceinline__ void pr_f32_cmb(const BlockGeom &B, const PrCtl &C,
+                                           const PrF32Args &A, bool mixed)
+{
+    __shared__ float s_vals[AG_TILE_NNZ_MAX];
+    __shared__ float s_wave[AG_WG / AG_WAVE];
+    const PrState *st = C.st;
+    if (*(volatile const uint32_t *)&st->done)
+        return;
+    const uint32_t iter = *(volatile const uint32_t *)&st->iter;
+    Red red{0.0, 0u};
+    pr_f32_dispatch<HT>(st, iter, A, mixed, [&](const auto &p) {
+        pull_combine_items(B, p, red, s_vals, s_wave);
+    });
+    pr_store_partial(C, red);
+}
+
+__global__ __launch_bounds__(AG_WG) void pr_f32_blk_kernel(PullGeom U, BlockGeom B, PrCtl C,
+                                                           PrF32Args A)
+{
+    pr_f32_blk<float>(U, B, C, A, false);
+}
+__global__ __launch_bounds__(AG_WG) void pr_bf16_blk_kernel(PullGeom U, BlockGeom B, PrCtl C,
+                                                            PrF32Args A)
+{
+    pr_f32_blk<bf16s>(U, B, C, A, true);
+}
+__global__ __launch_bounds__(AG_WG) void pr_f16_blk_kernel(PullGeom U, BlockGeom B, PrCtl C,
+                                                           PrF32Args A)
+{
+    pr_f32_blk<_Float16>(U, B, C, A, true);
+}
+__global__ __launch_bounds__(AG_WG) void pr_f32_cmb_kernel(BlockGeom B, PrCtl C, PrF32Args A)
+{
+    pr_f32_cmb<float>(B, C, A, false);
+}
+__global__ __launch_bounds__(AG_WG) void pr_bf16_cmb_kernel(BlockGeom B, PrCtl C, PrF32Args A)
+{
+    pr_f32_cmb<bf16s>(B, C, A, true);
+}
+__global__ __launch_bounds__(AG_WG) void pr_f16_cmb_kernel(BlockGeom B, PrCtl C, PrF32Args A)
+{
+    pr_f32_cmb<_Float16>(B, C, A, true);
 }
 
 __global__ __launch_bounds__(AG_WG) void pr_f32_kernel(PullGeom G, PrCtl C, PrF32Args A)
@@ -342,6 +394,36 @@ __global__ __launch_bounds__(AG_WG) void pr_fixed_kernel(PullGeom G, PrCtl C, Pr
 }
 
 template <int BITS>
+__global__ __launch_bounds__(AG_WG) void pr_fixed_blk_kernel(PullGeom U, BlockGeom B, PrCtl C,
+                                                             PrF32Args A)
+{
+    __shared__ uint64_t s_vals[AG_TILE_NNZ_MAX];
+    __shared__ uint64_t s_wave[AG_WG / AG_WAVE];
+    const PrState *st = C.st;
+    if (*(volatile const uint32_t *)&st->done)
+        return;
+    const uint32_t iter = *(volatile const uint32_t *)&st->iter;
+    Red red{0.0, 0u};
+    const PrFixPol<BITS> p = fix_policy<BITS>(iter, A);
+    pull_blocked_items(U, B, p, p, red, s_vals, s_wave);
+    pr_store_partial(C, red);
+}
+
+template <int BITS>
+__global__ __launch_bounds__(AG_WG) void pr_fixed_cmb_kernel(BlockGeom B, PrCtl C, PrF32Args A)
+{
+    __shared__ uint64_t s_vals[AG_TILE_NNZ_MAX];
+    __shared__ uint64_t s_wave[AG_WG / AG_WAVE];
+    const PrState *st = C.st;
+    if (*(volatile const uint32_t *)&st->done)
+        return;
+    const uint32_t iter = *(volatile const uint32_t *)&st->iter;
+    Red red{0.0, 0u};
+    pull_combine_items(B, fix_policy<BITS>(iter, A), red, s_vals, s_wave);
+    pr_store_partial(C, red);
+}
+
+template <int BITS>
 __global__ __launch_bounds__(AG_FIN_WG) void pr_fixed_finish(PullGeom G, PrCtl C, PrF32Args A)
 {
     const PrState *st = C.st;
@@ -447,6 +529,16 @@ extern "C" void ag_pr_options_default(ag_pr_options *o)
     o->record_history = 0;
 }
 
+// Upper bound of the per-workgroup partials of one iteration (every pass, every kind).
+static uint32_t ag_pr_max_parts(const ag_dgraph *g)
+{
+    const uint32_t lds = AG_TILE_NNZ_MAX * 4u;   // the smallest LDS use gives the largest grid
+    uint32_t n = ag_pull_grid(g, lds);
+    if (g->blk.bits)
+        n = ag_blocked_grid(g, lds) + ag_combine_grid(g, lds);
+    return n ? n : 1u;
+}
+
 static void session_free(ag_pr_session *s)
 {
     if (!s)
@@ -534,7 +626,7 @@ extern "C" int ag_pr_begin(ag_dgraph *g, const ag_pr_options *opts, ag_pr_sessio
     hipStream_t stream = g->dev->stream;
     const uint32_t n = g->n;
     const uint32_t maxit = o.max_iterations ? o.max_iterations : 1;
-    const uint32_t nblk = ag_pull_grid(g, AG_TILE_NNZ_MAX * 4u) ? ag_pull_grid(g, AG_TILE_NNZ_MAX * 4u) : 1;
+    const uint32_t nblk = ag_pr_max_parts(g);
     const uint32_t fblk = (n + AG_WG - 1) / AG_WG ? (n + AG_WG - 1) / AG_WG : 1;
 
     AG_TRY(salloc(s, (void **)&s->st, sizeof(PrState)));
@@ -618,17 +710,34 @@ extern "C" int ag_pr_begin(ag_dgraph *g, const ag_pr_options *opts, ag_pr_sessio
     return AG_OK;
 }
 
+#define AG_PR_LAUNCH_REGULAR(MAIN, FIN)                      \
+    do {                                                     \
+        MAIN<<<grid, block, 0, stream>>>(G, C, A);           \
+        FIN<<<1, fin, 0, stream>>>(G, C, A);                 \
+    } while (0)
+
+#define AG_PR_LAUNCH_BLOCKED(BLK, CMB, FIN)                  \
+    do {                                                     \
+        C.part_base = 0;                                     \
+        BLK<<<g1, block, 0, stream>>>(G, B, C, A);           \
+        C.part_base = g1;                                    \
+        CMB<<<g2, block, 0, stream>>>(B, C, A);              \
+        FIN<<<1, fin, 0, stream>>>(G, C, A);                 \
+    } while (0)
+
 int ag_pr_launch_iteration(ag_pr_session *s)
 {
     ag_dgraph *g = s->g;
     hipStream_t stream = g->dev->stream;
-    PullGeom G = ag_make_geom(g);
+    const bool blocked = g->blk.bits != 0;
+    PullGeom G = blocked ? ag_make_unblocked_geom(g) : ag_make_geom(g);
     PrCtl C;
     C.st = s->st;
     C.part_err = s->part_err;
     C.part_act = s->part_act;
     C.hist_err = s->hist_err;
     C.hist_act = s->hist_act;
+    C.part_base = 0;
     C.stop_rule = s->o.stop_rule;
     C.mixed = (s->kind_code == 1 || s->kind_code == 2) ? 1u : 0u;
     C.l1_tol = s->o.l1_tolerance;
@@ -644,40 +753,42 @@ int ag_pr_launch_iteration(ag_pr_session *s)
     A.damp = pr_damp();
     A.dn = (double)g->n;
     A.eps = s->o.epsilon;
-    A.hot_lo = g->params.cold_below_bin ? g->bin_row[g->params.cold_below_bin] : 0u;
     const uint32_t lds = (s->kind_code >= 8) ? AG_TILE_NNZ_MAX * 8u : AG_TILE_NNZ_MAX * 4u;
+    dim3 block(AG_WG), fin(AG_FIN_WG);
+    if (blocked) {
+        const BlockGeom B = ag_make_block_geom(g);
+        const uint32_t g1 = ag_blocked_grid(g, lds), g2 = ag_combine_grid(g, lds);
+        C.nparts = g1 + g2;
+        switch (s->kind_code) {
+        case 0: AG_PR_LAUNCH_BLOCKED(pr_f32_blk_kernel, pr_f32_cmb_kernel, pr_f32_finish); break;
+        case 1: AG_PR_LAUNCH_BLOCKED(pr_bf16_blk_kernel, pr_bf16_cmb_kernel, pr_bf16_finish); break;
+        case 2: AG_PR_LAUNCH_BLOCKED(pr_f16_blk_kernel, pr_f16_cmb_kernel, pr_f16_finish); break;
+        case 64:
+            AG_PR_LAUNCH_BLOCKED(pr_fixed_blk_kernel<64>, pr_fixed_cmb_kernel<64>, pr_fixed_finish<64>);
+            break;
+        case 32:
+            AG_PR_LAUNCH_BLOCKED(pr_fixed_blk_kernel<32>, pr_fixed_cmb_kernel<32>, pr_fixed_finish<32>);
+            break;
+        case 16:
+            AG_PR_LAUNCH_BLOCKED(pr_fixed_blk_kernel<16>, pr_fixed_cmb_kernel<16>, pr_fixed_finish<16>);
+            break;
+        default:
+            AG_PR_LAUNCH_BLOCKED(pr_fixed_blk_kernel<8>, pr_fixed_cmb_kernel<8>, pr_fixed_finish<8>);
+            break;
+        }
+        AG_HIP(hipGetLastError());
+        return AG_OK;
+    }
     const uint32_t grid = ag_pull_grid(g, lds);
     C.nparts = grid;
-    dim3 block(AG_WG), fin(AG_FIN_WG);
     switch (s->kind_code) {
-    case 0:
-        pr_f32_kernel<<<grid, block, 0, stream>>>(G, C, A);
-        pr_f32_finish<<<1, fin, 0, stream>>>(G, C, A);
-        break;
-    case 1:
-        pr_bf16_kernel<<<grid, block, 0, stream>>>(G, C, A);
-        pr_bf16_finish<<<1, fin, 0, stream>>>(G, C, A);
-        break;
-    case 2:
-        pr_f16_kernel<<<grid, block, 0, stream>>>(G, C, A);
-        pr_f16_finish<<<1, fin, 0, stream>>>(G, C, A);
-        break;
-    case 64:
-        pr_fixed_kernel<64><<<grid, block, 0, stream>>>(G, C, A);
-        pr_fixed_finish<64><<<1, fin, 0, stream>>>(G, C, A);
-        break;
-    case 32:
-        pr_fixed_kernel<32><<<grid, block, 0, stream>>>(G, C, A);
-        pr_fixed_finish<32><<<1, fin, 0, stream>>>(G, C, A);
-        break;
-    case 16:
-        pr_fixed_kernel<16><<<grid, block, 0, stream>>>(G, C, A);
-        pr_fixed_finish<16><<<1, fin, 0, stream>>>(G, C, A);
-        break;
-    default:
-        pr_fixed_kernel<8><<<grid, block, 0, stream>>>(G, C, A);
-        pr_fixed_finish<8><<<1, fin, 0, stream>>>(G, C, A);
-        break;
+    case 0: AG_PR_LAUNCH_REGULAR(pr_f32_kernel, pr_f32_finish); break;
+    case 1: AG_PR_LAUNCH_REGULAR(pr_bf16_kernel, pr_bf16_finish); break;
+    case 2: AG_PR_LAUNCH_REGULAR(pr_f16_kernel, pr_f16_finish); break;
+    case 64: AG_PR_LAUNCH_REGULAR(pr_fixed_kernel<64>, pr_fixed_finish<64>); break;
+    case 32: AG_PR_LAUNCH_REGULAR(pr_fixed_kernel<32>, pr_fixed_finish<32>); break;
+    case 16: AG_PR_LAUNCH_REGULAR(pr_fixed_kernel<16>, pr_fixed_finish<16>); break;
+    default: AG_PR_LAUNCH_REGULAR(pr_fixed_kernel<8>, pr_fixed_finish<8>); break;
     }
     AG_HIP(hipGetLastError());
     return AG_OK;

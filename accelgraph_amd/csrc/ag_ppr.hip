@@ -360,8 +360,9 @@ extern "C" int ag_ppr_pull(ag_dgraph *g, uint32_t k, const uint32_t *seeds,
     AG_HIP(hipSetDevice(g->dev->ordinal));
     hipStream_t s = g->dev->stream;
     const uint32_t maxit = o.max_iterations ? o.max_iterations : 1;
-    const uint32_t grid = ag_pull_grid(g, (AG_WG / PPR_HW) * PPR_W * 12u) ?
-                          ag_pull_grid(g, (AG_WG / PPR_HW) * PPR_W * 12u) : 1;
+    // 64-column rows are wide coalesced reads: this kernel keeps 8 workgroups per CU
+    const uint32_t grid = ag_pull_grid(g, (AG_WG / PPR_HW) * PPR_W * 12u, 8u) ?
+                          ag_pull_grid(g, (AG_WG / PPR_HW) * PPR_W * 12u, 8u) : 1;
     const size_t nv = (size_t)n * PPR_W;
     void *buf[12] = {0};
     int nb = 0;

@@ -101,6 +101,8 @@ __device__ __forceinline__ uint32_t ld_stream(const uint32_t *p)
 }
 
 #define AG_UNROLL 4
+// source-id loads (then gathers) in flight per lane in an LDS tile
+#define AG_TILE_UNROLL 4
 
 // Lane-strided partial sum of in-edges lo, lo+S, ... < hi. Each round issues the source-id
 // loads of AG_UNROLL elements, then their gathers, all predicated, so a lane has up to
@@ -156,15 +158,16 @@ __device__ __forceinline__ void pull_block_row(const PullGeom &G, const P &pol, 
         pol.finish(row, acc, red);
 }
 
+// Rows [rbase + k*(AG_WG/GS) ...]: one row per GS lanes; rows at or past rend are idle.
 template <class P, uint32_t GS>
-__device__ __forceinline__ void pull_group_rows(const PullGeom &G, const P &pol, uint32_t bin,
-                                                uint32_t item, Red &red)
+__device__ __forceinline__ void group_rows(const PullGeom &G, const P &pol, uint32_t rbase,
+                                           uint32_t rend, Red &red)
 {
     using Acc = typename P::Acc;
     const uint32_t tid = threadIdx.x;
-    const uint32_t row = G.bin_row[bin] + item * (AG_WG / GS) + tid / GS;
+    const uint32_t row = rbase + tid / GS;
     const uint32_t lane = tid % GS;
-    const bool valid = row < G.bin_row[bin + 1];
+    const bool valid = row < rend;
     Acc acc = Acc(0);
     if (valid)
         acc = strided_sum<P, GS>(G, pol, G.off[row] + lane, G.off[row + 1]);
@@ -173,24 +176,33 @@ __device__ __forceinline__ void pull_group_rows(const PullGeom &G, const P &pol,
         pol.finish(row, acc, red);
 }
 
+template <class P, uint32_t GS>
+__device__ __forceinline__ void pull_group_rows(const PullGeom &G, const P &pol, uint32_t bin,
+                                                uint32_t item, Red &red)
+{
+    group_rows<P, GS>(G, pol, G.bin_row[bin] + item * (AG_WG / GS), G.bin_row[bin + 1], red);
+}
+
+// Rows [r0, r1) with at most AG_TILE_NNZ_MAX in-edges in total: the in-edge values are
+// gathered cooperatively (coalesced source-id reads) into LDS, then each lane sums whole rows
+// from LDS in stream order -- the same order and rounding as the reference's sequential sum.
 template <class P>
-__device__ __forceinline__ void pull_thread_tile(const PullGeom &G, const P &pol, uint32_t tile,
-                                                 Red &red, typename P::Acc *s_vals)
+__device__ __forceinline__ void tile_rows(const PullGeom &G, const P &pol, uint32_t r0,
+                                          uint32_t r1, Red &red, typename P::Acc *s_vals)
 {
     using Acc = typename P::Acc;
     const uint32_t tid = threadIdx.x;
-    const uint32_t r0 = G.tile_row[tile], r1 = G.tile_row[tile + 1];
     const uint32_t e0 = G.off[r0], e1 = G.off[r1];
     const uint32_t nnz = e1 - e0;  // <= AG_TILE_NNZ_MAX, guaranteed by the plan
-    for (uint32_t k = tid; k < nnz; k += AG_UNROLL * AG_WG) {
-        uint32_t sidx[AG_UNROLL];
+    for (uint32_t k = tid; k < nnz; k += AG_TILE_UNROLL * AG_WG) {
+        uint32_t sidx[AG_TILE_UNROLL];
 #pragma unroll
-        for (int u = 0; u < AG_UNROLL; u++) {
+        for (int u = 0; u < AG_TILE_UNROLL; u++) {
             const uint32_t kk = k + u * AG_WG;
             sidx[u] = kk < nnz ? ld_stream(&G.src[e0 + kk]) : 0u;
         }
 #pragma unroll
-        for (int u = 0; u < AG_UNROLL; u++) {
+        for (int u = 0; u < AG_TILE_UNROLL; u++) {
             const uint32_t kk = k + u * AG_WG;
             if (kk < nnz)
                 s_vals[kk] = pol.edge(e0 + kk, sidx[u]);
@@ -210,24 +222,111 @@ __device__ __forceinline__ void pull_thread_tile(const PullGeom &G, const P &pol
     __syncthreads();  // s_vals is reused by this workgroup's next tile
 }
 
+template <class P>
+__device__ __forceinline__ void pull_thread_tile(const PullGeom &G, const P &pol, uint32_t tile,
+                                                 Red &red, typename P::Acc *s_vals)
+{
+    tile_rows(G, pol, G.tile_row[tile], G.tile_row[tile + 1], red, s_vals);
+}
+
+// Work item `it` of a regular schedule (launch slots of G.slot_blk0).
+template <class P>
+__device__ __forceinline__ void pull_one_item(const PullGeom &G, const P &pol, uint32_t it,
+                                              Red &red, typename P::Acc *s_vals,
+                                              typename P::Acc *s_wave)
+{
+    if (it < G.slot_blk0[1])
+        pull_hub_segment(G, pol, it - G.slot_blk0[0], s_wave);
+    else if (it < G.slot_blk0[2])
+        pull_block_row(G, pol, G.bin_row[AG_BIN_BLOCK] + (it - G.slot_blk0[1]), red, s_wave);
+    else if (it < G.slot_blk0[3])
+        pull_group_rows<P, AG_WAVE>(G, pol, AG_BIN_WAVE, it - G.slot_blk0[2], red);
+    else if (it < G.slot_blk0[4])
+        pull_group_rows<P, 16>(G, pol, AG_BIN_GROUP16, it - G.slot_blk0[3], red);
+    else
+        pull_thread_tile(G, pol, it - G.slot_blk0[4], red, s_vals);
+}
+
 // All work items of one workgroup of the persistent grid.
 template <class P>
 __device__ __forceinline__ void pull_items(const PullGeom &G, const P &pol, Red &red,
                                            typename P::Acc *s_vals, typename P::Acc *s_wave)
 {
     const uint32_t total = G.slot_blk0[AG_NSLOTS];
-    for (uint32_t it = blockIdx.x; it < total; it += gridDim.x) {
-        if (it < G.slot_blk0[1])
-            pull_hub_segment(G, pol, it - G.slot_blk0[0], s_wave);
-        else if (it < G.slot_blk0[2])
-            pull_block_row(G, pol, G.bin_row[AG_BIN_BLOCK] + (it - G.slot_blk0[1]), red, s_wave);
-        else if (it < G.slot_blk0[3])
-            pull_group_rows<P, AG_WAVE>(G, pol, AG_BIN_WAVE, it - G.slot_blk0[2], red);
-        else if (it < G.slot_blk0[4])
-            pull_group_rows<P, 16>(G, pol, AG_BIN_GROUP16, it - G.slot_blk0[3], red);
-        else
-            pull_thread_tile(G, pol, it - G.slot_blk0[4], red, s_vals);
+    for (uint32_t it = blockIdx.x; it < total; it += gridDim.x)
+        pull_one_item(G, pol, it, red, s_vals, s_wave);
+}
+
+// ---------------------------------------------------------------- source-blocked pull
+
+// First pass: a virtual row's sum is stored as its partial; the update waits for the combine.
+template <class P>
+struct PartPol {
+    using Acc = typename P::Acc;
+    P base;
+    Acc *part;
+    __device__ __forceinline__ Acc edge(uint32_t j, uint32_t s) const { return base.edge(j, s); }
+    __device__ __forceinline__ void finish(uint32_t v, Acc acc, Red &) const { part[v] = acc; }
+};
+
+// Second pass: a row's "in-edges" are its partial slots, added in source-block order.
+template <class P>
+struct CombPol {
+    using Acc = typename P::Acc;
+    P base;
+    const Acc *part;
+    __device__ __forceinline__ Acc edge(uint32_t, uint32_t v) const { return part[v]; }
+    __device__ __forceinline__ void finish(uint32_t r, Acc acc, Red &red) const
+    {
+        base.finish(r, acc, red);
     }
+};
+
+// First pass of a source-blocked iteration. Workgroup b serves slice b % AG_NXCD (the XCD it
+// is dispatched to) and walks that slice's items, which go source block by source block, so
+// the gathers of the workgroups of one XCD stay inside about one block of the contribution
+// vector, which that XCD's L2 holds. `pol` gathers through virtual-row edge positions (its
+// per-edge arrays, if any, are in virtual-row order); AG_ITEM_REG items run the unblocked
+// rows' regular schedule U with `upol`.
+template <class P, class PU>
+__device__ __forceinline__ void pull_blocked_items(const PullGeom &U, const BlockGeom &B,
+                                                   const P &pol, const PU &upol, Red &red,
+                                                   typename P::Acc *s_vals,
+                                                   typename P::Acc *s_wave)
+{
+    using Acc = typename P::Acc;
+    const PartPol<P> pp{pol, (Acc *)B.part};
+    const uint32_t x = blockIdx.x % AG_NXCD;
+    const uint32_t step = gridDim.x / AG_NXCD;   // the grid is a multiple of AG_NXCD
+    const uint32_t end = B.item_off[x + 1];
+    for (uint32_t i = B.item_off[x] + blockIdx.x / AG_NXCD; i < end; i += step) {
+        const uint32_t first = B.item[2 * i], kc = B.item[2 * i + 1];
+        const uint32_t cnt = kc >> 4;
+        switch (kc & 15u) {
+        case AG_ITEM_REG:
+            pull_one_item(U, upol, first, red, s_vals, s_wave);
+            break;
+        case AG_ITEM_VTILE:
+            tile_rows(B.V, pp, first, first + cnt, red, s_vals);
+            break;
+        case AG_ITEM_VG16:
+            group_rows<PartPol<P>, 16>(B.V, pp, first, first + cnt, red);
+            break;
+        default:
+            group_rows<PartPol<P>, AG_WAVE>(B.V, pp, first, first + cnt, red);
+            break;
+        }
+    }
+}
+
+// Second pass: the regular schedule of the combine CSR, finishing every blocked row.
+template <class P>
+__device__ __forceinline__ void pull_combine_items(const BlockGeom &B, const P &pol, Red &red,
+                                                   typename P::Acc *s_vals,
+                                                   typename P::Acc *s_wave)
+{
+    using Acc = typename P::Acc;
+    pull_items(B.C, CombPol<P>{pol, (const Acc *)B.part}, red, s_vals, s_wave);
 }
 
 // Finish step of the hub rows: add each row's segment partials in segment order.

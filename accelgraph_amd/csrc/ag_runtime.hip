@@ -205,7 +205,9 @@ extern "C" int ag_graph_free(ag_dgraph *g)
     hipSetDevice(g->dev->ordinal);
     hipStreamSynchronize(g->dev->stream);
     void *ptrs[] = {g->d2o, g->off, g->src, g->w, g->wq, g->outdeg, g->tile_row, g->seg_row,
-                    g->seg_lo, g->seg_hi, g->hub_seg0, g->hub_part, g->vec_a, g->vec_b};
+                    g->seg_lo, g->seg_hi, g->hub_seg0, g->hub_part, g->vec_a, g->vec_b,
+                    g->blk.voff, g->blk.vsrc, g->blk.vw, g->blk.item, g->blk.coff, g->blk.cidx,
+                    g->blk.c_tile_row, g->vpart};
     for (void *p : ptrs)
         if (p)
             hipFree(p);
@@ -236,16 +238,73 @@ PullGeom ag_make_geom(const ag_dgraph *g)
     return G;
 }
 
-uint32_t ag_pull_grid(const ag_dgraph *g, uint32_t lds_bytes)
+PullGeom ag_make_unblocked_geom(const ag_dgraph *g)
 {
-    // 160 KB of LDS and 8 four-wave workgroups (32 waves) per CU
-    uint32_t by_lds = lds_bytes ? (160u * 1024u) / lds_bytes : 8u;
-    uint32_t per_cu = by_lds < 8u ? by_lds : 8u;
-    if (g->params.wg_per_cu && g->params.wg_per_cu < per_cu)
-        per_cu = g->params.wg_per_cu;
-    if (per_cu == 0)
-        per_cu = 1;
-    uint32_t grid = g->dev->num_cus * per_cu;
+    PullGeom G = ag_make_geom(g);
+    for (int s = 0; s <= AG_NSLOTS; s++)
+        G.slot_blk0[s] = g->blk.r_slot_blk0[s];
+    if (g->params.blocked_min_bin <= AG_BIN_HUB)
+        G.n_hubs = 0;   // hub rows are blocked: their sums come from the combine pass
+    return G;
+}
+
+BlockGeom ag_make_block_geom(const ag_dgraph *g)
+{
+    const ag_block_plan &B = g->blk;
+    BlockGeom K;
+    memset(&K, 0, sizeof(K));
+    K.V.off = B.voff;
+    K.V.src = B.vsrc;
+    K.V.n = B.nv;
+    K.C.off = B.coff;
+    K.C.src = B.cidx;
+    K.C.tile_row = B.c_tile_row;
+    K.C.n = g->n;
+    K.C.n_hubs = 0;
+    for (int b = 0; b <= AG_NBINS; b++)
+        K.C.bin_row[b] = B.c_bin_row[b];
+    for (int s = 0; s <= AG_NSLOTS; s++)
+        K.C.slot_blk0[s] = B.c_slot_blk0[s];
+    K.item = B.item;
+    K.part = g->vpart;
+    for (uint32_t x = 0; x <= AG_NXCD; x++)
+        K.item_off[x] = B.item_off[x];
+    return K;
+}
+
+// Resident four-wave workgroups per CU: the plan's wg_per_cu, by default 4 (16 waves per
+// CU), never more than 160 KB of LDS allows. The gathers are bound by the rate of random L2
+// requests, and more requests in flight only lengthen the queues (8 per CU measured 5-40%
+// slower than 4 on RMAT-25 and ER-25).
+static uint32_t wg_per_cu(const ag_dgraph *g, uint32_t lds_bytes, uint32_t dflt = 4u)
+{
+    uint32_t by_lds = lds_bytes ? (160u * 1024u) / lds_bytes : 32u;
+    uint32_t per_cu = g->params.wg_per_cu ? g->params.wg_per_cu : dflt;
+    if (per_cu > by_lds)
+        per_cu = by_lds;
+    return per_cu ? per_cu : 1u;
+}
+
+uint32_t ag_blocked_grid(const ag_dgraph *g, uint32_t lds_bytes)
+{
+    // workgroup i runs on XCD i % AG_NXCD (round-robin dispatch), so slice x is served by the
+    // workgroups with that residue; the grid is a multiple of AG_NXCD
+    uint32_t grid = g->dev->num_cus * wg_per_cu(g, lds_bytes);
+    grid = (grid + AG_NXCD - 1) / AG_NXCD * AG_NXCD;
+    return grid ? grid : AG_NXCD;
+}
+
+uint32_t ag_combine_grid(const ag_dgraph *g, uint32_t lds_bytes)
+{
+    uint32_t grid = g->dev->num_cus * wg_per_cu(g, lds_bytes);
+    uint32_t total = g->blk.c_slot_blk0[AG_NSLOTS];
+    grid = grid < total ? grid : total;
+    return grid ? grid : 1u;
+}
+
+uint32_t ag_pull_grid(const ag_dgraph *g, uint32_t lds_bytes, uint32_t default_per_cu)
+{
+    uint32_t grid = g->dev->num_cus * wg_per_cu(g, lds_bytes, default_per_cu);
     return grid < g->total_blocks ? grid : g->total_blocks;
 }
 
@@ -331,6 +390,28 @@ extern "C" int ag_graph_upload(ag_device *dev, uint32_t n, uint32_t m, const uin
             st = ag_hip_fail(he, "hub scratch allocation", __FILE__, __LINE__);
         g->device_bytes += pb;
     }
+    if (st == AG_OK && plan.blk.bits) {
+        const ag_block_plan &B = plan.blk;
+        g->blk = B;   // scalars; the pointers are replaced by device copies below
+        g->blk.voff = g->blk.vsrc = g->blk.item = g->blk.coff = g->blk.cidx = NULL;
+        g->blk.c_tile_row = NULL;
+        g->blk.vw = NULL;
+        st = dev_upload(g, &g->blk.voff, B.voff, (size_t)B.nv + 1);
+        if (st == AG_OK) st = dev_upload(g, &g->blk.vsrc, B.vsrc, B.mb);
+        if (st == AG_OK && B.vw) st = dev_upload(g, &g->blk.vw, B.vw, B.mb);
+        if (st == AG_OK) st = dev_upload(g, &g->blk.item, B.item, (size_t)B.n_items * 2);
+        if (st == AG_OK) st = dev_upload(g, &g->blk.coff, B.coff, (size_t)plan.n + 1);
+        if (st == AG_OK) st = dev_upload(g, &g->blk.cidx, B.cidx, B.nv);
+        if (st == AG_OK)
+            st = dev_upload(g, &g->blk.c_tile_row, B.c_tile_row, (size_t)B.c_n_tiles + 1);
+        if (st == AG_OK) {
+            size_t pb = (size_t)(B.nv ? B.nv : 1) * sizeof(uint64_t);
+            he = hipMalloc((void **)&g->vpart, pb);
+            if (he != hipSuccess)
+                st = ag_hip_fail(he, "partial-sum allocation", __FILE__, __LINE__);
+            g->device_bytes += pb;
+        }
+    }
     // the host plan arrays must outlive the asynchronous copies
     if (st == AG_OK)
         st = ag_wait_stream(dev);
@@ -396,6 +477,10 @@ extern "C" int ag_graph_get_info(const ag_dgraph *g, ag_graph_info *info)
     info->has_weights = g->w != NULL;
     info->device_bytes = g->device_bytes;
     info->params = g->params;
+    info->src_block_bits = g->blk.bits;
+    info->src_blocks = g->blk.nblocks;
+    info->virtual_rows = g->blk.nv;
+    info->blocked_row0 = g->blk.bits ? g->blk.row0 : g->n;
     return AG_OK;
 }
 

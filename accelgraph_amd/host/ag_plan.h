@@ -27,6 +27,52 @@ extern "C" {
 #define AG_NSLOTS AG_NBINS
 extern const uint32_t ag_slot_bin[AG_NSLOTS];
 
+/* Source blocking (PageRank pull on large graphs). The blocked rows (bins >= blocked_min_bin,
+ * the device-row suffix [row0, n)) are cut into AG_NXCD contiguous slices of about equal
+ * in-edge counts, one per accelerator complex (XCD, which owns its own L2). Every blocked row's
+ * in-edges are regrouped by source block (device id >> bits), keeping stream order inside a
+ * block; each (row, block) group, split into pieces of at most AG_PIECE_NNZ edges, is a
+ * "virtual row" whose sum is one partial. Virtual rows are ordered by (slice, block, class,
+ * row, piece) and each slice's work items walk the blocks in order, so at any time the
+ * gathers of one XCD fall in one block of the contribution vector. A second pass adds each
+ * row's partials in block order (a CSR over the partials, coff/cidx) and applies the update. */
+#define AG_NXCD 8u
+#define AG_PIECE_NNZ 4096u
+#define AG_MAX_SRC_BLOCKS 4096u
+#define AG_AUTO_BLOCK_BITS 19u
+#define AG_AUTO_BLOCK_MIN_N (1u << 21)
+
+/* work item kinds of the blocked pass: item = {first, kind | count << 4} */
+enum { AG_ITEM_REG = 0,     /* first = item of the unblocked rows' regular schedule      */
+       AG_ITEM_VTILE = 1,   /* virtual rows [first, first+count): LDS tile, lane per row  */
+       AG_ITEM_VG16 = 2,    /* virtual rows [first, first+count), count <= 16: 16 lanes each */
+       AG_ITEM_VWAVE = 3 }; /* virtual rows [first, first+count), count <= 4: a wave each */
+
+typedef struct ag_block_plan {
+    uint32_t bits;          /* 0: no source blocking (the fields below are empty)       */
+    uint32_t nblocks;
+    uint32_t row0;          /* first blocked device row                                 */
+    uint32_t nv;            /* virtual rows                                             */
+    uint32_t mb;            /* in-edges of the blocked rows                             */
+    uint32_t slice_row[AG_NXCD + 1];
+    uint32_t *voff;         /* [nv+1] offsets of the virtual rows in vsrc               */
+    uint32_t *vsrc;         /* [mb]   device source ids, virtual-row order              */
+    float *vw;              /* [mb]   weights in the same order, or NULL                */
+    uint32_t n_items;
+    uint32_t *item;         /* [2*n_items]                                              */
+    uint32_t item_off[AG_NXCD + 1]; /* items of slice x: [item_off[x], item_off[x+1])   */
+    uint32_t *coff;         /* [n+1]  partial slots of each row (empty below row0)      */
+    uint32_t *cidx;         /* [nv]   virtual row of each slot, block order within a row */
+    /* combine pass: the regular bin schedule over the blocked rows, with coff/cidx as the
+     * CSR; hub rows join the block bin (a row has at most a few hundred partials) */
+    uint32_t c_bin_row[AG_NBINS + 1];
+    uint32_t c_n_tiles;
+    uint32_t *c_tile_row;   /* [c_n_tiles+1]                                            */
+    uint32_t c_slot_blk0[AG_NSLOTS + 1];
+    /* regular schedule restricted to the unblocked rows (the AG_ITEM_REG items)         */
+    uint32_t r_slot_blk0[AG_NSLOTS + 1];
+} ag_block_plan;
+
 typedef struct ag_plan {
     uint32_t n, m;
     ag_plan_params params;
@@ -49,6 +95,7 @@ typedef struct ag_plan {
     uint32_t bin_blocks[AG_NBINS];
     uint32_t slot_blk0[AG_NSLOTS + 1]; /* first block of each launch slot       */
     uint32_t total_blocks;
+    ag_block_plan blk;
 } ag_plan;
 
 /* Validates the inputs (every offset/degree in range, every source < n) and builds the

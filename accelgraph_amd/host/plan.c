@@ -24,8 +24,9 @@ void ag_plan_params_default(ag_plan_params *p)
     p->tile_nnz = 4096;
     p->segment_nnz = 16384;
     p->order = 0;
-    p->cold_below_bin = 0;
+    p->src_block_bits = AG_SRC_BLOCK_AUTO;
     p->wg_per_cu = 0;
+    p->blocked_min_bin = AG_BIN_THREAD;
 }
 
 int ag_plan_params_check(const ag_plan_params *p, const char **err)
@@ -55,8 +56,13 @@ int ag_plan_params_check(const ag_plan_params *p, const char **err)
         *err = "wg_per_cu must be in [0, 32]";
         return AG_ERR_INVALID;
     }
-    if (p->cold_below_bin > AG_NBINS) {
-        *err = "cold_below_bin must be in [0, 5]";
+    if (p->src_block_bits != AG_SRC_BLOCK_AUTO && p->src_block_bits != AG_SRC_BLOCK_OFF &&
+        (p->src_block_bits < 8 || p->src_block_bits > 30)) {
+        *err = "src_block_bits must be AG_SRC_BLOCK_AUTO, AG_SRC_BLOCK_OFF or 8..30";
+        return AG_ERR_INVALID;
+    }
+    if (p->blocked_min_bin > AG_NBINS) {
+        *err = "blocked_min_bin must be in [0, 5]";
         return AG_ERR_INVALID;
     }
     if (p->order > 1) {
@@ -81,6 +87,13 @@ void ag_plan_free(ag_plan *p)
     free(p->seg_lo);
     free(p->seg_hi);
     free(p->hub_seg0);
+    free(p->blk.voff);
+    free(p->blk.vsrc);
+    free(p->blk.vw);
+    free(p->blk.item);
+    free(p->blk.coff);
+    free(p->blk.cidx);
+    free(p->blk.c_tile_row);
     memset(p, 0, sizeof(*p));
 }
 
@@ -106,6 +119,396 @@ static int cmp_u64(const void *a, const void *b)
 static void *zalloc(size_t count, size_t size)
 {
     return calloc(count ? count : 1, size);
+}
+
+/* Greedy cut of rows [lo, hi) into tiles of at most rows_max rows and nnz_max in-edges
+ * (a single row above nnz_max is never cut; callers keep such rows out). Returns the
+ * [*count + 1] tile start rows, or NULL when out of memory. */
+static uint32_t *cut_tiles(const uint32_t *off, uint32_t lo, uint32_t hi, uint32_t rows_max,
+                           uint32_t nnz_max, uint32_t *count)
+{
+    uint32_t *tr = (uint32_t *)zalloc((size_t)(hi - lo) + 2, sizeof(uint32_t));
+    if (!tr)
+        return NULL;
+    uint32_t t = 0, r = lo;
+    while (r < hi) {
+        tr[t++] = r;
+        uint32_t rows = 0, nnz = 0;
+        while (r < hi && rows < rows_max) {
+            uint32_t d = off[r + 1] - off[r];
+            if (rows > 0 && nnz + d > nnz_max)
+                break;
+            nnz += d;
+            rows++;
+            r++;
+        }
+    }
+    tr[t] = hi;
+    *count = t;
+    return tr;
+}
+
+/* ---------------------------------------------------------------- source blocking */
+
+/* Effective block bits (0: no blocking). *err is set when explicit bits give too many blocks. */
+static uint32_t block_bits_for(const ag_plan *p, const char **err)
+{
+    uint32_t b = p->params.src_block_bits;
+    if (b == AG_SRC_BLOCK_OFF || p->params.blocked_min_bin >= AG_NBINS)
+        return 0;
+    if (b == AG_SRC_BLOCK_AUTO) {
+        if (p->n <= AG_AUTO_BLOCK_MIN_N)
+            return 0;
+        b = AG_AUTO_BLOCK_BITS;
+        while ((((uint64_t)p->n + (1ull << b) - 1) >> b) > AG_MAX_SRC_BLOCKS)
+            b++;
+        return b;
+    }
+    if ((((uint64_t)p->n + (1ull << b) - 1) >> b) > AG_MAX_SRC_BLOCKS)
+        *err = "src_block_bits too small for this graph: more than 4096 source blocks";
+    return b;
+}
+
+/* class of a virtual row by length: 0 = wave, 1 = 16-lane group, 2 = tile (lane per row) */
+static inline uint32_t vclass(const ag_plan_params *q, uint32_t len)
+{
+    return len <= q->thread_max_deg ? 2u : (len <= q->group16_max_deg ? 1u : 0u);
+}
+
+/* Stable sort of one row's in-edges by source block into out_s/out_w. keys: scratch of d. */
+static void sort_row_by_block(const uint32_t *src, const float *w, uint32_t d, uint32_t bits,
+                              uint32_t *out_s, float *out_w, uint64_t *keys)
+{
+    if (d <= 32) {
+        for (uint32_t i = 0; i < d; i++) {
+            uint32_t s = src[i];
+            float x = w ? w[i] : 0.0f;
+            uint32_t j = i;
+            while (j > 0 && (out_s[j - 1] >> bits) > (s >> bits)) {
+                out_s[j] = out_s[j - 1];
+                if (w)
+                    out_w[j] = out_w[j - 1];
+                j--;
+            }
+            out_s[j] = s;
+            if (w)
+                out_w[j] = x;
+        }
+        return;
+    }
+    for (uint32_t i = 0; i < d; i++)
+        keys[i] = ((uint64_t)(src[i] >> bits) << 32) | i;
+    qsort(keys, d, sizeof(uint64_t), cmp_u64);
+    for (uint32_t i = 0; i < d; i++) {
+        uint32_t k = (uint32_t)keys[i];
+        out_s[i] = src[k];
+        if (w)
+            out_w[i] = w[k];
+    }
+}
+
+/* Calls emit(start, len, block) for every virtual row of a block-sorted row: one per run of
+ * equal source block, split into pieces of at most AG_PIECE_NNZ. Returns the count. */
+static uint32_t row_pieces(const uint32_t *ss, uint32_t d, uint32_t bits, uint32_t *st,
+                           uint32_t *len, uint32_t *blk)
+{
+    uint32_t cnt = 0, i = 0;
+    while (i < d) {
+        uint32_t b = ss[i] >> bits, j = i + 1;
+        while (j < d && (ss[j] >> bits) == b)
+            j++;
+        for (uint32_t a = i; a < j; a += AG_PIECE_NNZ) {
+            if (st) {
+                st[cnt] = a;
+                len[cnt] = (j - a > AG_PIECE_NNZ) ? AG_PIECE_NNZ : j - a;
+                blk[cnt] = b;
+            }
+            cnt++;
+        }
+        i = j;
+    }
+    return cnt;
+}
+
+/* Work items of every slice; item == NULL only counts them. */
+static uint32_t emit_items(const ag_plan *p, const uint32_t *kstart, const uint32_t *vlen,
+                           uint32_t reg_total, uint32_t *item, uint32_t *item_off)
+{
+    const ag_block_plan *B = &p->blk;
+    const uint32_t nb = B->nblocks;
+    uint32_t k = 0;
+#define AG_EMIT(first, kind, cnt)                                        \
+    do {                                                                 \
+        if (item) {                                                      \
+            item[2 * k] = (first);                                       \
+            item[2 * k + 1] = (uint32_t)(kind) | ((uint32_t)(cnt) << 4); \
+        }                                                                \
+        k++;                                                             \
+    } while (0)
+    for (uint32_t x = 0; x < AG_NXCD; x++) {
+        if (item_off)
+            item_off[x] = k;
+        for (uint32_t b = 0; b < nb; b++) {
+            const uint32_t K = (x * nb + b) * 3;
+            for (uint32_t v = kstart[K]; v < kstart[K + 1]; v += 4)
+                AG_EMIT(v, AG_ITEM_VWAVE, (kstart[K + 1] - v < 4) ? kstart[K + 1] - v : 4);
+            for (uint32_t v = kstart[K + 1]; v < kstart[K + 2]; v += 16)
+                AG_EMIT(v, AG_ITEM_VG16, (kstart[K + 2] - v < 16) ? kstart[K + 2] - v : 16);
+            uint32_t v = kstart[K + 2];
+            const uint32_t e = kstart[K + 3];
+            while (v < e) {
+                uint32_t t0 = v, rows = 0, nnz = 0;
+                while (v < e && rows < p->params.tile_rows) {
+                    if (rows > 0 && nnz + vlen[v] > p->params.tile_nnz)
+                        break;
+                    nnz += vlen[v];
+                    rows++;
+                    v++;
+                }
+                AG_EMIT(t0, AG_ITEM_VTILE, rows);
+            }
+        }
+        for (uint32_t r = x; r < reg_total; r += AG_NXCD)
+            AG_EMIT(r, AG_ITEM_REG, 0);
+    }
+    if (item_off)
+        item_off[AG_NXCD] = k;
+#undef AG_EMIT
+    return k;
+}
+
+static int build_block_plan(ag_plan *p, const char **err)
+{
+    ag_block_plan *B = &p->blk;
+    memset(B, 0, sizeof(*B));
+    const char *e = NULL;
+    const uint32_t bits = block_bits_for(p, &e);
+    if (e) {
+        *err = e;
+        return AG_ERR_INVALID;
+    }
+    const uint32_t n = p->n, min_bin = p->params.blocked_min_bin;
+    if (!bits || min_bin >= AG_NBINS)
+        return AG_OK;
+    const uint32_t row0 = p->bin_row[min_bin];
+    if (row0 >= n || p->off[n] == p->off[row0])
+        return AG_OK;
+    const uint32_t e0 = p->off[row0], mb = p->off[n] - e0;
+    const uint32_t nb = (uint32_t)((((uint64_t)n) + (1ull << bits) - 1) >> bits);
+    const uint32_t nK = AG_NXCD * nb * 3;
+    const ag_plan_params *q = &p->params;
+    int oom = 0;
+
+    B->bits = bits;
+    B->nblocks = nb;
+    B->row0 = row0;
+    B->mb = mb;
+    /* slices: contiguous blocked rows with about mb/AG_NXCD in-edges each */
+    B->slice_row[0] = row0;
+    for (uint32_t x = 1; x < AG_NXCD; x++) {
+        uint64_t target = e0 + (uint64_t)mb * x / AG_NXCD;
+        uint32_t lo = B->slice_row[x - 1], hi = n;
+        while (lo < hi) {
+            uint32_t mid = lo + (hi - lo) / 2;
+            if (p->off[mid] < target)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        B->slice_row[x] = lo;
+    }
+    B->slice_row[AG_NXCD] = n;
+
+    uint32_t *ss = (uint32_t *)zalloc(mb, sizeof(uint32_t));
+    float *sw = p->w ? (float *)zalloc(mb, sizeof(float)) : NULL;
+    B->coff = (uint32_t *)zalloc((size_t)n + 1, sizeof(uint32_t));
+    uint32_t *nvr = (uint32_t *)zalloc((size_t)(n - row0), sizeof(uint32_t));
+    if (!ss || (p->w && !sw) || !B->coff || !nvr) {
+        free(ss);
+        free(sw);
+        free(nvr);
+        *err = "host allocation failed";
+        return AG_ERR_OOM;
+    }
+    /* 1: sort each blocked row by source block; count its virtual rows */
+    #pragma omp parallel
+    {
+        uint64_t *keys = NULL;
+        size_t cap = 0;
+        #pragma omp for schedule(dynamic, 256)
+        for (uint32_t r = row0; r < n; r++) {
+            const uint32_t d = p->off[r + 1] - p->off[r], o = p->off[r] - e0;
+            if (d > 32 && d > cap) {
+                free(keys);
+                cap = d;
+                keys = (uint64_t *)malloc(cap * sizeof(uint64_t));
+                if (!keys) {
+                    cap = 0;
+                    #pragma omp atomic write
+                    oom = 1;
+                    continue;
+                }
+            }
+            sort_row_by_block(p->src + p->off[r], p->w ? p->w + p->off[r] : NULL, d, bits,
+                              ss + o, sw ? sw + o : NULL, keys);
+            nvr[r - row0] = row_pieces(ss + o, d, bits, NULL, NULL, NULL);
+        }
+        free(keys);
+    }
+    if (oom) {
+        free(ss);
+        free(sw);
+        free(nvr);
+        *err = "host allocation failed";
+        return AG_ERR_OOM;
+    }
+    {
+        uint64_t acc = 0;
+        for (uint32_t r = 0; r <= row0; r++)
+            B->coff[r] = 0;
+        for (uint32_t r = row0; r < n; r++) {
+            acc += nvr[r - row0];
+            B->coff[r + 1] = (uint32_t)acc;
+        }
+        B->nv = (uint32_t)acc;
+    }
+    free(nvr);
+    const uint32_t nv = B->nv;
+
+    /* 2: every virtual row in (row, block, piece) slot order: key, start in ss, length */
+    uint32_t *vkey = (uint32_t *)zalloc(nv, sizeof(uint32_t));
+    uint32_t *vst = (uint32_t *)zalloc(nv, sizeof(uint32_t));
+    uint32_t *vln = (uint32_t *)zalloc(nv, sizeof(uint32_t));
+    B->cidx = (uint32_t *)zalloc(nv, sizeof(uint32_t));
+    B->voff = (uint32_t *)zalloc((size_t)nv + 1, sizeof(uint32_t));
+    B->vsrc = (uint32_t *)zalloc(mb, sizeof(uint32_t));
+    if (p->w)
+        B->vw = (float *)zalloc(mb, sizeof(float));
+    const int nt = omp_get_max_threads() > 0 ? omp_get_max_threads() : 1;
+    uint32_t *hist = (uint32_t *)zalloc((size_t)nt * nK, sizeof(uint32_t));
+    uint32_t *kstart = (uint32_t *)zalloc((size_t)nK + 1, sizeof(uint32_t));
+    uint32_t *vlen = (uint32_t *)zalloc(nv, sizeof(uint32_t));
+    uint32_t *vfrom = (uint32_t *)zalloc(nv, sizeof(uint32_t));
+    if (!vkey || !vst || !vln || !B->cidx || !B->voff || !B->vsrc || (p->w && !B->vw) || !hist ||
+        !kstart || !vlen || !vfrom) {
+        oom = 1;
+        goto done;
+    }
+    #pragma omp parallel for schedule(dynamic, 256)
+    for (uint32_t r = row0; r < n; r++) {
+        const uint32_t d = p->off[r + 1] - p->off[r], o = p->off[r] - e0, c = B->coff[r];
+        uint32_t x = 0;
+        while (x + 1 < AG_NXCD && r >= B->slice_row[x + 1])
+            x++;
+        uint32_t cnt = row_pieces(ss + o, d, bits, vst + c, vln + c, vkey + c);
+        for (uint32_t j = 0; j < cnt; j++) {
+            vkey[c + j] = (x * nb + vkey[c + j]) * 3 + vclass(q, vln[c + j]);
+            vst[c + j] += o;
+        }
+    }
+    /* 3: stable counting sort of the slots by key: slot -> virtual row */
+    #pragma omp parallel num_threads(nt)
+    {
+        const int t = omp_get_thread_num(), T = omp_get_num_threads();
+        const uint32_t a = (uint32_t)((uint64_t)nv * t / T), b = (uint32_t)((uint64_t)nv * (t + 1) / T);
+        uint32_t *h = hist + (size_t)t * nK;
+        for (uint32_t i = a; i < b; i++)
+            h[vkey[i]]++;
+        #pragma omp barrier
+        #pragma omp single
+        {
+            uint32_t run = 0;
+            for (uint32_t K = 0; K < nK; K++) {
+                kstart[K] = run;
+                for (int u = 0; u < T; u++) {
+                    uint32_t c = hist[(size_t)u * nK + K];
+                    hist[(size_t)u * nK + K] = run;
+                    run += c;
+                }
+            }
+            kstart[nK] = run;
+        }
+        for (uint32_t i = a; i < b; i++) {
+            uint32_t v = h[vkey[i]]++;
+            B->cidx[i] = v;
+            vlen[v] = vln[i];
+            vfrom[v] = vst[i];
+        }
+    }
+    {
+        uint64_t acc = 0;
+        for (uint32_t v = 0; v < nv; v++) {
+            B->voff[v] = (uint32_t)acc;
+            acc += vlen[v];
+        }
+        B->voff[nv] = (uint32_t)acc;
+    }
+    #pragma omp parallel for schedule(dynamic, 4096)
+    for (uint32_t v = 0; v < nv; v++) {
+        memcpy(B->vsrc + B->voff[v], ss + vfrom[v], (size_t)vlen[v] * sizeof(uint32_t));
+        if (B->vw)
+            memcpy(B->vw + B->voff[v], sw + vfrom[v], (size_t)vlen[v] * sizeof(float));
+    }
+
+    /* 4: combine schedule over the blocked rows. A row has at most in-degree partials, so
+     * when the wave bin's rows fit a tile every row below the block bin is combined by a lane
+     * from an LDS tile (sequential, in block order); block and hub rows get a workgroup. */
+    for (int k = 0; k < AG_BIN_HUB; k++)
+        B->c_bin_row[k] = p->bin_row[k] > row0 ? p->bin_row[k] : row0;
+    B->c_bin_row[AG_BIN_HUB] = n;
+    B->c_bin_row[AG_NBINS] = n;
+    if (q->wave_max_deg <= q->tile_nnz) {
+        B->c_bin_row[AG_BIN_GROUP16] = B->c_bin_row[AG_BIN_BLOCK];
+        B->c_bin_row[AG_BIN_WAVE] = B->c_bin_row[AG_BIN_BLOCK];
+    }
+    B->c_tile_row = cut_tiles(B->coff, B->c_bin_row[AG_BIN_THREAD],
+                              B->c_bin_row[AG_BIN_THREAD + 1], q->tile_rows, q->tile_nnz,
+                              &B->c_n_tiles);
+    if (!B->c_tile_row) {
+        oom = 1;
+        goto done;
+    }
+    {
+        uint32_t cb[AG_NBINS];
+        cb[AG_BIN_THREAD] = B->c_n_tiles;
+        cb[AG_BIN_GROUP16] = (B->c_bin_row[2] - B->c_bin_row[1] + 15) / 16;
+        cb[AG_BIN_WAVE] = (B->c_bin_row[3] - B->c_bin_row[2] + 3) / 4;
+        cb[AG_BIN_BLOCK] = B->c_bin_row[4] - B->c_bin_row[3];
+        cb[AG_BIN_HUB] = 0;
+        uint32_t rb[AG_NBINS];
+        for (int k = 0; k < AG_NBINS; k++)
+            rb[k] = ((uint32_t)k < min_bin) ? p->bin_blocks[k] : 0u;
+        B->c_slot_blk0[0] = B->r_slot_blk0[0] = 0;
+        for (int sl = 0; sl < AG_NSLOTS; sl++) {
+            B->c_slot_blk0[sl + 1] = B->c_slot_blk0[sl] + cb[ag_slot_bin[sl]];
+            B->r_slot_blk0[sl + 1] = B->r_slot_blk0[sl] + rb[ag_slot_bin[sl]];
+        }
+    }
+
+    /* 5: work items, slice by slice, block-major */
+    B->n_items = emit_items(p, kstart, vlen, B->r_slot_blk0[AG_NSLOTS], NULL, NULL);
+    B->item = (uint32_t *)zalloc((size_t)B->n_items * 2, sizeof(uint32_t));
+    if (!B->item) {
+        oom = 1;
+        goto done;
+    }
+    emit_items(p, kstart, vlen, B->r_slot_blk0[AG_NSLOTS], B->item, B->item_off);
+
+done:
+    free(ss);
+    free(sw);
+    free(vkey);
+    free(vst);
+    free(vln);
+    free(hist);
+    free(kstart);
+    free(vlen);
+    free(vfrom);
+    if (oom) {
+        *err = "host allocation failed";
+        return AG_ERR_OOM;
+    }
+    return AG_OK;
 }
 
 int ag_plan_build(uint32_t n, uint32_t m, const uint32_t *out_degree,
@@ -244,30 +647,12 @@ int ag_plan_build(uint32_t n, uint32_t m, const uint32_t *out_degree,
     }
 
     /* thread-bin tiles: greedy cut on rows and in-edges */
-    {
-        uint32_t lo = p->bin_row[AG_BIN_THREAD], hi = p->bin_row[AG_BIN_THREAD + 1];
-        uint32_t cap = hi - lo + 1;
-        p->tile_row = (uint32_t *)zalloc((size_t)cap + 1, sizeof(uint32_t));
-        if (!p->tile_row) {
-            ag_plan_free(p);
-            *err = "host allocation failed";
-            return AG_ERR_OOM;
-        }
-        uint32_t t = 0, r = lo;
-        while (r < hi) {
-            p->tile_row[t++] = r;
-            uint32_t rows = 0, nnz = 0;
-            while (r < hi && rows < p->params.tile_rows) {
-                uint32_t d = p->off[r + 1] - p->off[r];
-                if (rows > 0 && nnz + d > p->params.tile_nnz)
-                    break;
-                nnz += d;
-                rows++;
-                r++;
-            }
-        }
-        p->tile_row[t] = hi;
-        p->n_tiles = t;
+    p->tile_row = cut_tiles(p->off, p->bin_row[AG_BIN_THREAD], p->bin_row[AG_BIN_THREAD + 1],
+                            p->params.tile_rows, p->params.tile_nnz, &p->n_tiles);
+    if (!p->tile_row) {
+        ag_plan_free(p);
+        *err = "host allocation failed";
+        return AG_ERR_OOM;
     }
 
     /* hub segments */
@@ -311,5 +696,11 @@ int ag_plan_build(uint32_t n, uint32_t m, const uint32_t *out_degree,
     for (int s = 0; s < AG_NSLOTS; s++)
         p->slot_blk0[s + 1] = p->slot_blk0[s] + p->bin_blocks[ag_slot_bin[s]];
     p->total_blocks = p->slot_blk0[AG_NSLOTS];
+
+    st = build_block_plan(p, err);
+    if (st != AG_OK) {
+        ag_plan_free(p);
+        return st;
+    }
     return AG_OK;
 }

@@ -84,10 +84,23 @@ typedef struct ag_plan_params {
     uint32_t tile_nnz;         /* max in-edges per thread-bin tile (<= 4096)    */
     uint32_t segment_nnz;      /* in-edges per hub segment                      */
     uint32_t order;            /* row order inside a bin: 0 = vertex id, 1 = out-degree desc */
-    uint32_t cold_below_bin;   /* 0: off; k: gathers of sources in rows of bins < k use
-                                  non-temporal loads (they stop evicting hot L2 lines)    */
-    uint32_t wg_per_cu;        /* resident workgroups per CU of the persistent grid (0 = auto) */
+    uint32_t src_block_bits;   /* source blocking of the PageRank pull: sources are cut into
+                                  blocks of 2^bits device ids and the in-edges of every row of
+                                  a blocked bin are regrouped block by block, so the gathers of
+                                  one pass stay inside one block (an L2-sized window of the
+                                  contribution vector). AG_SRC_BLOCK_AUTO: 19 bits when the
+                                  contribution vector exceeds 8 MB (n > 2^21), else off;
+                                  AG_SRC_BLOCK_OFF: off; else 8..30                      */
+    uint32_t wg_per_cu;        /* resident workgroups per CU of the persistent grid (0 = auto:
+                                  4, fewer if LDS limits; more in flight only queues at L2) */
+    uint32_t blocked_min_bin;  /* with source blocking, rows of bins >= this are blocked.
+                                  Default AG_BIN_THREAD (every row): on a blocked graph the
+                                  thread bin then loses its stream-order FP32 sums (FP32
+                                  results stay within rounding; integer results are exact) */
 } ag_plan_params;
+
+#define AG_SRC_BLOCK_AUTO 0u
+#define AG_SRC_BLOCK_OFF 255u
 
 void ag_plan_params_default(ag_plan_params *p);
 
@@ -102,6 +115,10 @@ typedef struct ag_graph_info {
     uint32_t has_weights;
     uint64_t device_bytes;
     ag_plan_params params;
+    uint32_t src_block_bits;   /* blocking in effect (0: none)                          */
+    uint32_t src_blocks;       /* source blocks                                         */
+    uint32_t virtual_rows;     /* (row, source block) pieces of the blocked rows        */
+    uint32_t blocked_row0;     /* first blocked device row                              */
 } ag_graph_info;
 
 /* Upload one directed graph. Arrays are host arrays in the reference's layout:

@@ -24,7 +24,14 @@ FP32_TOL = 1e-5
 SMALL = dict(thread_max_deg=2, group16_max_deg=6, wave_max_deg=20, block_max_deg=60,
              tile_rows=7, tile_nnz=9, segment_nnz=256, order=0)
 PARAMS = {"default": None, "small": SMALL, "small-order1": dict(SMALL, order=1),
-          "tiles": dict(thread_max_deg=16, tile_rows=1024, tile_nnz=4096)}
+          "tiles": dict(thread_max_deg=16, tile_rows=1024, tile_nnz=4096),
+          # source blocking forced on small graphs: 2^10-source blocks, rows of in-degree > 16
+          # blocked; every row blocked with 2^8-source blocks; small thresholds (every virtual
+          # row class and every combine bin populated)
+          "blocked": dict(src_block_bits=10, blocked_min_bin=1),
+          "blocked-all": dict(src_block_bits=8, blocked_min_bin=0),
+          "blocked-small": dict(SMALL, src_block_bits=9, blocked_min_bin=1)}
+BLOCKED = ["blocked", "blocked-all", "blocked-small"]
 
 
 def say(*a):
@@ -239,7 +246,7 @@ def test_pagerank_f32_user_init_and_determinism(device, rmat12):
 # ------------------------------------------------------------------ fixed point
 
 @pytest.mark.parametrize("bits", [64, 32, 16, 8])
-@pytest.mark.parametrize("pname", ["default", "small"])
+@pytest.mark.parametrize("pname", ["default", "small"] + BLOCKED)
 def test_pagerank_fixed_bit_exact(device, rmat16, bits, pname):
     say(f"pagerank fixed{bits} {pname}")
     g = rmat16
@@ -252,6 +259,32 @@ def test_pagerank_fixed_bit_exact(device, rmat16, bits, pname):
     np.testing.assert_array_equal(run.ranks.view(np.uint32), ref.ranks.view(np.uint32))
     np.testing.assert_allclose(run.hist_err, ref.hist_err, rtol=1e-9, atol=1e-15)
     dg.close()
+
+
+def test_pagerank_blocked_auto_rmat22(device):
+    """Above 2^21 vertices the default plan blocks sources (2^19 per block). RMAT-22: fixed32
+    bit-exact and FP32 within tolerance of the oracle for 20 iterations, and the blocked and
+    unblocked FP32 runs agree."""
+    say("pagerank blocked auto rmat-22")
+    g = ag.Graph.rmat(22)
+    dg = device.upload(g)
+    info = dg.info()
+    assert info.src_block_bits == 19 and info.src_blocks == 8 and info.virtual_rows > 0
+    kw = dict(max_iterations=20, stop_rule=ag.STOP_NONE)
+    fx = dg.pagerank(kind=32, **kw)
+    ref = ref_pr(g, kind=32, max_iterations=20, stop_rule=2, threads=8)
+    np.testing.assert_array_equal(fx.ranks.view(np.uint32), ref.ranks.view(np.uint32))
+    run = dg.pagerank(**kw)
+    ref = ref_pr(g, max_iterations=20, stop_rule=2, threads=8)
+    err = np.abs(run.ranks - ref.ranks).max()
+    say(f"  fixed32 bit-exact; f32 maxabs={err:.3e}")
+    assert err <= FP32_TOL
+    dg.close()
+    plain = device.upload(g, params=ag.plan_params(src_block_bits=ag.SRC_BLOCK_OFF))
+    assert plain.info().src_block_bits == 0
+    run2 = plain.pagerank(**kw)
+    assert np.abs(run.ranks - run2.ranks).max() <= FP32_TOL
+    plain.close()
 
 
 # ------------------------------------------------------------------ SpMV
@@ -307,13 +340,14 @@ def test_edge_cases(device):
 # ------------------------------------------------------------------ 16-bit storage
 
 @pytest.mark.parametrize("storage", [ag.STORE_BF16, ag.STORE_F16])
-def test_pagerank_16bit_storage_refined(device, rmat16, storage):
+@pytest.mark.parametrize("pname", ["default", "blocked"])
+def test_pagerank_16bit_storage_refined(device, rmat16, storage, pname):
     """16-bit contributions (FP32 accumulate) until the L1 delta drops below refine_l1, then
     FP32 contributions to L1 <= 1e-6: the result must be as accurate as the plain FP32 run
     against the FP64 oracle."""
-    say(f"pagerank 16-bit storage {storage}")
+    say(f"pagerank 16-bit storage {storage} {pname}")
     g = rmat16
-    dg = device.upload(g)
+    dg = upload(device, g, pname)
     kw = dict(max_iterations=300, stop_rule=ag.STOP_L1, l1_tolerance=1e-6,
               init=ag.INIT_UNIFORM)
     plain = dg.pagerank(**kw)

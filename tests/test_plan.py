@@ -12,6 +12,16 @@ from accelgraph_amd._abi import PlanParams
 u32p = C.POINTER(C.c_uint32)
 
 
+class BlockPlan(C.Structure):
+    _fields_ = [("bits", C.c_uint32), ("nblocks", C.c_uint32), ("row0", C.c_uint32),
+                ("nv", C.c_uint32), ("mb", C.c_uint32), ("slice_row", C.c_uint32 * 9),
+                ("voff", u32p), ("vsrc", u32p), ("vw", C.POINTER(C.c_float)),
+                ("n_items", C.c_uint32), ("item", u32p), ("item_off", C.c_uint32 * 9),
+                ("coff", u32p), ("cidx", u32p), ("c_bin_row", C.c_uint32 * 6),
+                ("c_n_tiles", C.c_uint32), ("c_tile_row", u32p), ("c_slot_blk0", C.c_uint32 * 6),
+                ("r_slot_blk0", C.c_uint32 * 6)]
+
+
 class Plan(C.Structure):
     _fields_ = [("n", C.c_uint32), ("m", C.c_uint32), ("params", PlanParams),
                 ("d2o", u32p), ("o2d", u32p), ("off", u32p), ("src", u32p),
@@ -20,7 +30,7 @@ class Plan(C.Structure):
                 ("n_hubs", C.c_uint32), ("n_segs", C.c_uint32), ("seg_row", u32p),
                 ("seg_lo", u32p), ("seg_hi", u32p), ("hub_seg0", u32p),
                 ("bin_blocks", C.c_uint32 * 5), ("slot_blk0", C.c_uint32 * 6),
-                ("total_blocks", C.c_uint32)]
+                ("total_blocks", C.c_uint32), ("blk", BlockPlan)]
 
 
 def build(g, params=None, n=None, outdeg=None, idx=None, deg=None, src=None):
@@ -98,8 +108,8 @@ def check_plan(g, p):
 
 
 SMALL = dict(thread_max_deg=2, group16_max_deg=6, wave_max_deg=20, block_max_deg=60,
-             tile_rows=7, tile_nnz=9, segment_nnz=256, order=0, cold_below_bin=0,
-             wg_per_cu=0)
+             tile_rows=7, tile_nnz=9, segment_nnz=256, order=0, src_block_bits=255,
+             wg_per_cu=0, blocked_min_bin=1)
 
 
 @pytest.mark.parametrize("params", [None, SMALL, dict(SMALL, order=1)],
@@ -141,3 +151,168 @@ def test_plan_rejects_bad_input():
     assert st == 1
     st, p, err = build(g, ag.plan_params(thread_max_deg=100, group16_max_deg=50))
     assert st == 1
+
+
+# ---------------------------------------------------------------- source blocking
+
+PIECE = 4096
+NXCD = 8
+ITEM_REG, ITEM_VTILE, ITEM_VG16, ITEM_VWAVE = 0, 1, 2, 3
+
+
+def check_block_plan(g, p):
+    """Every blocked row's in-edges appear exactly once among its virtual rows, grouped by
+    source block in stream order; virtual rows are ordered (slice, block, class, row, piece);
+    the items of each slice cover that slice's virtual rows exactly once, block-major, within
+    the kernels' limits; the combine schedule covers every blocked row once."""
+    B, prm, n = p.blk, p.params, g.n
+    bits, nb, row0, nv = B.bits, B.nblocks, B.row0, B.nv
+    off = view(p.off, n + 1)
+    srcd = view(p.src, p.m)
+    br = list(p.bin_row)
+    assert row0 == br[prm.blocked_min_bin] and B.mb == off[n] - off[row0]
+    assert nb == -(-n // (1 << bits)) and nb <= 4096
+    sl = list(B.slice_row)
+    assert sl[0] == row0 and sl[-1] == n and sl == sorted(sl)
+    voff, vsrc = view(B.voff, nv + 1), view(B.vsrc, B.mb)
+    coff, cidx = view(B.coff, n + 1), view(B.cidx, nv)
+    assert voff[0] == 0 and voff[-1] == B.mb
+    vlen = np.diff(voff.astype(np.int64))
+    assert np.all(vlen >= 1) and np.all(vlen <= PIECE)
+    assert np.all(coff[:row0 + 1] == 0) and coff[n] == nv and np.all(np.diff(coff.astype(np.int64)) >= 0)
+    assert sorted(cidx) == list(range(nv))
+    vrow = np.empty(nv, np.int64)
+    vblk = np.empty(nv, np.int64)
+    for r in range(row0, n):
+        vs = cidx[coff[r]:coff[r + 1]]
+        vrow[vs] = r
+        seq = [vsrc[voff[v]:voff[v + 1]] for v in vs]
+        for v, sq in zip(vs, seq):
+            blks = np.unique(sq >> bits)
+            assert blks.size == 1
+            vblk[v] = blks[0]
+        got = np.concatenate(seq) if seq else np.zeros(0, np.uint32)
+        row = srcd[off[r]:off[r + 1]]
+        want = row[np.argsort(row >> bits, kind="stable")]
+        np.testing.assert_array_equal(got, want)
+        # pieces: consecutive virtual rows of one block are full pieces except the last
+        bl = vblk[vs]
+        assert np.all(np.diff(bl) >= 0)
+        for k in range(len(vs) - 1):
+            if bl[k] == bl[k + 1]:
+                assert vlen[vs[k]] == PIECE
+    cls = np.where(vlen <= prm.thread_max_deg, 2, np.where(vlen <= prm.group16_max_deg, 1, 0))
+    xs = np.searchsorted(np.array(sl[1:]), vrow, side="right")
+    key = (xs * nb + vblk) * 3 + cls
+    assert np.all(np.diff(key) >= 0), "virtual rows ordered by (slice, block, class)"
+    same = np.diff(key) == 0
+    assert np.all(np.diff(vrow)[same] >= 0), "then by row"
+    # items
+    it = view(B.item, 2 * B.n_items).reshape(-1, 2).astype(np.int64)
+    io = list(B.item_off)
+    assert io[0] == 0 and io[-1] == B.n_items and io == sorted(io)
+    covered = np.zeros(nv, np.int64)
+    reg_total = B.r_slot_blk0[5]
+    reg_seen = []
+    for x in range(NXCD):
+        last_blk = -1
+        for first, kc in it[io[x]:io[x + 1]]:
+            kind, cnt = kc & 15, kc >> 4
+            if kind == ITEM_REG:
+                assert cnt == 0 and first % NXCD == x
+                reg_seen.append(first)
+                continue
+            vs = np.arange(first, first + cnt)
+            assert cnt >= 1 and vs[-1] < nv
+            covered[vs] += 1
+            assert np.all(xs[vs] == x) and np.unique(vblk[vs]).size == 1
+            assert vblk[vs[0]] >= last_blk
+            last_blk = vblk[vs[0]]
+            if kind == ITEM_VWAVE:
+                assert cnt <= 4 and np.all(cls[vs] == 0)
+            elif kind == ITEM_VG16:
+                assert cnt <= 16 and np.all(cls[vs] == 1)
+            else:
+                assert kind == ITEM_VTILE and np.all(cls[vs] == 2)
+                assert cnt <= prm.tile_rows and vlen[vs].sum() <= prm.tile_nnz
+    assert np.all(covered == 1)
+    assert sorted(reg_seen) == list(range(reg_total))
+    # regular schedule of the unblocked rows: bins below blocked_min_bin only
+    blk_counts = list(p.bin_blocks)
+    rb = [blk_counts[b] if b < prm.blocked_min_bin else 0 for b in range(5)]
+    order = [4, 3, 2, 1, 0]
+    assert list(B.r_slot_blk0) == list(np.concatenate([[0], np.cumsum([rb[b] for b in order])]))
+    # combine schedule: blocked rows only, hub rows in the block bin, tiles within limits
+    cb = list(B.c_bin_row)
+    want = [max(br[k], row0) for k in range(4)]
+    if prm.wave_max_deg <= prm.tile_nnz:   # rows below the block bin combine in LDS tiles
+        want[1] = want[2] = want[3]
+    assert cb[:4] == want and cb[4] == n and cb[5] == n
+    assert np.all(np.diff(coff[cb[0]:cb[1] + 1].astype(np.int64)) <= prm.tile_nnz)
+    ct = view(B.c_tile_row, B.c_n_tiles + 1)
+    assert ct[0] == cb[0] and ct[-1] == cb[1]
+    assert np.all(np.diff(ct) <= prm.tile_rows)
+    assert np.all(coff[ct[1:]] - coff[ct[:-1]] <= prm.tile_nnz)
+    cnts = [B.c_n_tiles, -(-(cb[2] - cb[1]) // 16), -(-(cb[3] - cb[2]) // 4), cb[4] - cb[3], 0]
+    assert list(B.c_slot_blk0) == list(np.concatenate([[0], np.cumsum([cnts[b] for b in order])]))
+
+
+@pytest.mark.parametrize("min_bin", [0, 1, 3])
+@pytest.mark.parametrize("small", [False, True], ids=["default", "small"])
+def test_block_plan_invariants(min_bin, small):
+    g = ag.Graph.rmat(12)
+    base = dict(SMALL) if small else {}
+    prm = ag.plan_params(**dict(base, src_block_bits=8, blocked_min_bin=min_bin))
+    st, p, err = build(g, prm)
+    assert st == 0, err
+    try:
+        row0 = p.bin_row[min_bin]
+        if row0 == g.n:   # no row in the blocked bins: nothing to block
+            assert p.blk.bits == 0 and p.blk.nv == 0
+            return
+        assert p.blk.bits == 8 and p.blk.nblocks == 16 and p.blk.nv > 0
+        check_plan(g, p)
+        check_block_plan(g, p)
+        kinds = set(view(p.blk.item, 2 * p.blk.n_items)[1::2] & 15)
+        if small:
+            assert {ITEM_VTILE, ITEM_VG16, ITEM_VWAVE} <= kinds
+    finally:
+        ag.lib().ag_plan_free(C.byref(p))
+
+
+def test_block_plan_long_pieces():
+    # one row with 10000 in-edges from one source block is cut into pieces of 4096
+    n = 10002
+    src = np.concatenate([np.arange(1, 10001), [0, 1]]).astype(np.uint32)
+    dst = np.concatenate([np.zeros(10000, np.uint32), [10001, 10001]]).astype(np.uint32)
+    g = ag.Graph.from_edges(src, dst, n=n)
+    st, p, err = build(g, ag.plan_params(src_block_bits=14, blocked_min_bin=0))
+    assert st == 0, err
+    try:
+        check_block_plan(g, p)
+        lens = np.diff(view(p.blk.voff, p.blk.nv + 1).astype(np.int64))
+        assert sorted(lens.tolist())[-3:] == [1808, 4096, 4096]
+    finally:
+        ag.lib().ag_plan_free(C.byref(p))
+
+
+def test_block_plan_auto_and_off():
+    g = ag.Graph.rmat(10)
+    for bits in (0, 255):   # auto leaves graphs of at most 2^21 vertices unblocked
+        st, p, err = build(g, ag.plan_params(src_block_bits=bits))
+        assert st == 0 and p.blk.bits == 0 and p.blk.nv == 0
+        ag.lib().ag_plan_free(C.byref(p))
+    st, p, err = build(g, ag.plan_params(src_block_bits=8, blocked_min_bin=5))
+    assert st == 0 and p.blk.bits == 0
+    ag.lib().ag_plan_free(C.byref(p))
+
+
+def test_block_plan_rejects_bad_bits():
+    g = ag.Graph.rmat(6)
+    st, p, err = build(g, ag.plan_params(src_block_bits=7))
+    assert st == 1 and b"src_block_bits" in err
+    n = (1 << 20) + 1   # 4097 blocks of 256 sources
+    st, p, err = build(None, n=n, outdeg=np.zeros(n, np.uint32), idx=np.zeros(n, np.uint32),
+                       deg=np.zeros(n, np.uint32), src=np.zeros(0, np.uint32),
+                       params=ag.plan_params(src_block_bits=8))
+    assert st == 1 and b"4096" in err
