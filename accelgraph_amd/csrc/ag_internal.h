@@ -48,6 +48,7 @@ struct ag_dgraph {
     // schedule scalars and HBM arrays of the block plan
     ag_block_plan blk;   // pointer members are device pointers here
     uint64_t *vpart;     // [blk.nv] partial sum of every virtual row (float or u64 bits)
+    uint32_t *vwq;       // [blk.mb] FloatToFixed32(vw), built on the first fixed SpMV
     // scratch for permuted vectors (x, y) of SpMV / gather, sized lazily
     void *vec_a;
     void *vec_b;

@@ -353,6 +353,27 @@ __global__ __launch_bounds__(AG_WG) void pull_kernel(PullGeom G, P pol)
     pull_items(G, pol, red, s_vals, s_wave);
 }
 
+// Plain source-blocked pull (SpMV): pass 1 with `pol` reading per-edge data in virtual-row
+// order and `upol` in regular order for the unblocked rows, then the combine pass.
+template <class P, class PU>
+__global__ __launch_bounds__(AG_WG) void pull_blocked_kernel(PullGeom U, BlockGeom B, P pol,
+                                                             PU upol)
+{
+    __shared__ typename P::Acc s_vals[AG_TILE_NNZ_MAX];
+    __shared__ typename P::Acc s_wave[AG_WG / AG_WAVE];
+    Red red{0.0, 0u};
+    pull_blocked_items(U, B, pol, upol, red, s_vals, s_wave);
+}
+
+template <class P>
+__global__ __launch_bounds__(AG_WG) void pull_combine_kernel(BlockGeom B, P pol)
+{
+    __shared__ typename P::Acc s_vals[AG_TILE_NNZ_MAX];
+    __shared__ typename P::Acc s_wave[AG_WG / AG_WAVE];
+    Red red{0.0, 0u};
+    pull_combine_items(B, pol, red, s_vals, s_wave);
+}
+
 template <class P>
 __global__ __launch_bounds__(AG_FIN_WG) void pull_finish_kernel(PullGeom G, P pol)
 {

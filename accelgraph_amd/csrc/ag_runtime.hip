@@ -207,7 +207,7 @@ extern "C" int ag_graph_free(ag_dgraph *g)
     void *ptrs[] = {g->d2o, g->off, g->src, g->w, g->wq, g->outdeg, g->tile_row, g->seg_row,
                     g->seg_lo, g->seg_hi, g->hub_seg0, g->hub_part, g->vec_a, g->vec_b,
                     g->blk.voff, g->blk.vsrc, g->blk.vw, g->blk.item, g->blk.coff, g->blk.cidx,
-                    g->blk.c_tile_row, g->vpart};
+                    g->blk.c_tile_row, g->vpart, g->vwq};
     for (void *p : ptrs)
         if (p)
             hipFree(p);

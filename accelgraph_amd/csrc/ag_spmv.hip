@@ -119,6 +119,46 @@ static int run_pull(ag_dgraph *g, const P &pol, uint32_t reps, double *avg_ms)
     return st;
 }
 
+// Source-blocked form of run_pull: `pv` reads per-edge data in virtual-row order, `pu` in
+// regular order (unblocked rows); pass 1 then the combine pass per repetition.
+template <class P>
+static int run_pull_blocked(ag_dgraph *g, const P &pv, const P &pu, uint32_t reps,
+                            double *avg_ms)
+{
+    hipStream_t stream = g->dev->stream;
+    const PullGeom U = ag_make_unblocked_geom(g);
+    const BlockGeom B = ag_make_block_geom(g);
+    const uint32_t lds = AG_TILE_NNZ_MAX * (uint32_t)sizeof(typename P::Acc);
+    const uint32_t g1 = ag_blocked_grid(g, lds), g2 = ag_combine_grid(g, lds);
+    hipEvent_t e0 = NULL, e1 = NULL;
+    if (avg_ms) {
+        AG_HIP(hipEventCreate(&e0));
+        AG_HIP(hipEventCreate(&e1));
+        AG_HIP(hipEventRecord(e0, stream));
+    }
+    for (uint32_t i = 0; i < reps; i++) {
+        pull_blocked_kernel<P, P><<<g1, AG_WG, 0, stream>>>(U, B, pv, pu);
+        pull_combine_kernel<P><<<g2, AG_WG, 0, stream>>>(B, pu);
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && avg_ms)
+        e = hipEventRecord(e1, stream);
+    int st = (e == hipSuccess) ? ag_wait_stream(g->dev)
+                               : ag_hip_fail(e, "blocked pull launch", __FILE__, __LINE__);
+    if (st == AG_OK && avg_ms) {
+        float ms = 0.0f;
+        e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess)
+            st = ag_hip_fail(e, "hipEventElapsedTime", __FILE__, __LINE__);
+        *avg_ms = reps ? (double)ms / reps : 0.0;
+    }
+    if (e0)
+        hipEventDestroy(e0);
+    if (e1)
+        hipEventDestroy(e1);
+    return st;
+}
+
 extern "C" int ag_spmv_pull(ag_dgraph *g, const float *x, float *y, uint32_t reps, double *avg_ms)
 {
     if (!g || (g->n && (!x || !y)))
@@ -142,7 +182,13 @@ extern "C" int ag_spmv_pull(ag_dgraph *g, const float *x, float *y, uint32_t rep
     AG_HIP(hipMemcpyAsync(xo, x, (size_t)n * sizeof(float), hipMemcpyHostToDevice, stream));
     permute_in_kernel<float><<<blk, AG_WG, 0, stream>>>(n, xo, g->d2o, xd);
     AG_HIP(hipGetLastError());
-    if (g->w)
+    if (g->blk.bits && g->w)
+        st = run_pull_blocked(g, SpmvF32Pol<true>{xd, g->blk.vw, yd},
+                              SpmvF32Pol<true>{xd, g->w, yd}, reps, avg_ms);
+    else if (g->blk.bits)
+        st = run_pull_blocked(g, SpmvF32Pol<false>{xd, NULL, yd}, SpmvF32Pol<false>{xd, NULL, yd},
+                              reps, avg_ms);
+    else if (g->w)
         st = run_pull(g, SpmvF32Pol<true>{xd, g->w, yd}, reps, avg_ms);
     else
         st = run_pull(g, SpmvF32Pol<false>{xd, NULL, yd}, reps, avg_ms);
@@ -187,6 +233,16 @@ extern "C" int ag_spmv_pull_fixed(ag_dgraph *g, const uint32_t *xq, uint64_t *yq
         quantize_weights_kernel<<<(g->m + AG_WG - 1) / AG_WG, AG_WG, 0, stream>>>(g->m, g->w, g->wq);
         AG_HIP(hipGetLastError());
     }
+    // Blocked graphs read weight codes in virtual-row order: quantised from vw. Codes set by
+    // the caller exist only in regular order, so such a graph keeps the regular schedule.
+    const bool blocked = g->blk.bits && !g->wq_user;
+    if (blocked && g->w && !g->vwq && g->blk.mb) {
+        AG_HIP(hipMalloc((void **)&g->vwq, (size_t)g->blk.mb * sizeof(uint32_t)));
+        g->device_bytes += (size_t)g->blk.mb * sizeof(uint32_t);
+        quantize_weights_kernel<<<(g->blk.mb + AG_WG - 1) / AG_WG, AG_WG, 0, stream>>>(
+            g->blk.mb, g->blk.vw, g->vwq);
+        AG_HIP(hipGetLastError());
+    }
     uint32_t *xo = (uint32_t *)g->vec_a;
     uint64_t *yo = (uint64_t *)g->vec_a;              // reused after x is permuted
     uint32_t *xd = (uint32_t *)g->vec_b;
@@ -195,10 +251,17 @@ extern "C" int ag_spmv_pull_fixed(ag_dgraph *g, const uint32_t *xq, uint64_t *yq
     AG_HIP(hipMemcpyAsync(xo, xq, (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
     permute_in_kernel<uint32_t><<<blk, AG_WG, 0, stream>>>(n, xo, g->d2o, xd);
     AG_HIP(hipGetLastError());
-    if (g->wq)
+    const uint32_t wc = host_fixed32(0.0001f);
+    if (blocked && g->wq)
+        st = run_pull_blocked(g, SpmvFixPol<true>{xd, g->vwq, 0u, yd},
+                              SpmvFixPol<true>{xd, g->wq, 0u, yd}, reps, avg_ms);
+    else if (blocked)
+        st = run_pull_blocked(g, SpmvFixPol<false>{xd, NULL, wc, yd},
+                              SpmvFixPol<false>{xd, NULL, wc, yd}, reps, avg_ms);
+    else if (g->wq)
         st = run_pull(g, SpmvFixPol<true>{xd, g->wq, 0u, yd}, reps, avg_ms);
     else
-        st = run_pull(g, SpmvFixPol<false>{xd, NULL, host_fixed32(0.0001f), yd}, reps, avg_ms);
+        st = run_pull(g, SpmvFixPol<false>{xd, NULL, wc, yd}, reps, avg_ms);
     if (st != AG_OK)
         return st;
     permute_out_kernel<uint64_t><<<blk, AG_WG, 0, stream>>>(n, yd, g->d2o, yo);

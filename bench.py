@@ -157,6 +157,66 @@ def ppr_section(ag, dev, scale: int, k: int) -> dict:
     return res
 
 
+def spmv_section(ag, dev, scale: int, reps: int, cpu: bool) -> dict:
+    """SpMV pull on a weighted RMAT graph: w = (k+1) 2^-24 and x = k 2^-24 with k uniform on
+    [0, 2^24) from a seeded stream; FP32 (binary32 products, FP32 sums) and the RTL's Q16.16
+    fixed-point datapath (u64 sums). Algorithmic bytes 12M + 8N + 4 (FP32), 12M + 12N + 4
+    (fixed). The CPU leg times the oracle's OpenMP restatement on the same inputs."""
+    import numpy as np
+    g = ag.Graph.rmat(scale)
+    rng = np.random.default_rng(ag.DEFAULT_SEED)
+    g.set_weights(((rng.integers(0, 1 << 24, g.m) + 1) * 2.0 ** -24).astype(np.float32))
+    x = (rng.integers(0, 1 << 24, g.n) * 2.0 ** -24).astype(np.float32)
+    dg = dev.upload(g, weights=True)
+    info = dg.info()
+    out = {"graph": f"RMAT scale {scale} (n={g.n}, m={g.m}), weighted",
+           "src_block_bits": info.src_block_bits}
+    xq = dev.fixed_encode(32, x).astype(np.uint32)
+    for name, fn, arg, nb in (("f32", dg.spmv, x, 12 * g.m + 8 * g.n + 4),
+                              ("fixed32", dg.spmv_fixed, xq, 12 * g.m + 12 * g.n + 4)):
+        fn(arg, reps=2)                                     # warm
+        _, ms = fn(arg, reps=reps)
+        gbs = nb / (ms * 1e-3) / 1e9
+        out[name] = {"ms": round(ms, 4), "gteps": round(g.m / (ms * 1e-3) / 1e9, 2),
+                     "algorithmic_bytes": nb, "gb_per_s": round(gbs, 1),
+                     "roofline_frac": round(gbs / HBM_PEAK_GBS, 4)}
+    y, _ = dg.spmv(x)
+    dg.close()
+    if cpu:   # CPU leg: the oracle's OpenMP restatement, also the FP32 result check
+        import oracle
+        threads = len(os.sched_getaffinity(0))
+        t0 = time.perf_counter()
+        ref = oracle.spmv_f32(g.inv_idx, g.in_degree, g.inv_src, g.inv_weight, x, threads)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"ms": round(dt * 1e3, 2), "cores": threads, "kind": "port",
+                               "gteps": round(g.m / dt / 1e9, 3)}
+        out["f32"]["max_rel_err_vs_cpu"] = float(np.max(np.abs(y - ref) / np.maximum(np.abs(ref), 1e-30)))
+    return out
+
+
+def er_section(ag, dev, scale: int, steps: int) -> dict:
+    """PageRank-pull FP32 iteration on an Erdos-Renyi graph with the bench graph's vertex and
+    edge counts (uniform sources: no hot set, every gather is a cold random read)."""
+    n = 1 << scale
+    g = ag.Graph.erdos_renyi(n, 16 * n)
+    dg = dev.upload(g)
+    sess = dg.session(max_iterations=steps + 3, stop_rule=ag.STOP_NONE, init=ag.INIT_UNIFORM)
+    sess.step(3)
+    dev.sync()
+    sess.step(steps)
+    dev.sync()
+    ms, _ = sess.last_step_kernel_ms()
+    sess.end()
+    info = dg.info()
+    dg.close()
+    nb = algorithmic_bytes(g.n, g.m, "f32")
+    gbs = nb / (ms * 1e-3) / 1e9
+    return {"graph": f"Erdos-Renyi n=2^{scale}, 16n draws, simplified (n={g.n}, m={g.m})",
+            "ms_per_iteration": round(ms, 4), "gteps": round(g.m / (ms * 1e-3) / 1e9, 2),
+            "gb_per_s": round(gbs, 1), "roofline_frac": round(gbs / HBM_PEAK_GBS, 4),
+            "src_block_bits": info.src_block_bits}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,6 +236,9 @@ def main() -> int:
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--no-ppr", action="store_true", help="skip the 64-column PPR")
     ap.add_argument("--ppr-scale", type=int, default=22)
+    ap.add_argument("--no-spmv", action="store_true", help="skip the RMAT-22 SpMV")
+    ap.add_argument("--spmv-scale", type=int, default=22)
+    ap.add_argument("--no-er", action="store_true", help="skip the ER PageRank iteration")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -252,6 +315,11 @@ def main() -> int:
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "algorithmic_bytes": nbytes,
                      "kernel_ms": round(kern_ms, 4),
+                     "kernel": ("one iteration = pr_f32_blk_kernel (source-blocked gather) + "
+                                "pr_f32_cmb_kernel (combine + update) + pr_f32_finish"
+                                if info.src_block_bits else
+                                "one iteration = pr_f32_kernel + pr_f32_finish"),
+                     "src_block_bits": info.src_block_bits,
                      "measured_copy_gb_per_s": round(copy_gbs, 1)},
     }
 
@@ -282,6 +350,14 @@ def main() -> int:
         t0 = time.perf_counter()
         out["ppr_64"] = ppr_section(ag, dev, args.ppr_scale, 64)
         log(f"ppr: {out['ppr_64']} ({time.perf_counter() - t0:.1f} s)")
+    if not args.no_spmv:
+        t0 = time.perf_counter()
+        out["spmv"] = spmv_section(ag, dev, args.spmv_scale, 20, not args.no_cpu)
+        log(f"spmv: {out['spmv']} ({time.perf_counter() - t0:.1f} s)")
+    if not args.no_er and args.graph == "rmat":
+        t0 = time.perf_counter()
+        out["er_pagerank"] = er_section(ag, dev, args.scale, 20)
+        log(f"er: {out['er_pagerank']} ({time.perf_counter() - t0:.1f} s)")
     dev.close()
     if not args.no_cpu:
         t0 = time.perf_counter()

@@ -289,32 +289,60 @@ def test_pagerank_blocked_auto_rmat22(device):
 
 # ------------------------------------------------------------------ SpMV
 
-def test_spmv_f32_and_fixed(device):
-    say("spmv rmat-14 random weights")
+@pytest.mark.parametrize("pname", ["default", "small"] + BLOCKED)
+def test_spmv_f32_and_fixed(device, pname):
+    say(f"spmv rmat-14 random weights {pname}")
     g = ag.Graph.rmat(14)
     rng = np.random.default_rng(11)
     fw = ((rng.integers(0, 2 ** 24, g.m) + 1) * 2.0 ** -24).astype(np.float32)  # (0, 1]
     g.set_weights(fw)
     x = (rng.integers(0, 2 ** 24, g.n) * 2.0 ** -24).astype(np.float32)  # [0, 1)
-    for pname in ("default", "small"):
-        dg = upload(device, g, pname, weights=True)
-        y, _ = dg.spmv(x, reps=3)
-        exp = oracle.spmv_f32(g.inv_idx, g.in_degree, g.inv_src, g.inv_weight, x)
-        np.testing.assert_allclose(y, exp, rtol=2e-6, atol=1e-6)
-        tb = g.in_degree <= dg.info().params.thread_max_deg
+    dg = upload(device, g, pname, weights=True)
+    info = dg.info()
+    assert (info.src_block_bits != 0) == (pname in BLOCKED)
+    y, _ = dg.spmv(x, reps=3)
+    exp = oracle.spmv_f32(g.inv_idx, g.in_degree, g.inv_src, g.inv_weight, x)
+    np.testing.assert_allclose(y, exp, rtol=2e-6, atol=1e-6)
+    if info.src_block_bits == 0 or info.params.blocked_min_bin > 0:
+        # thread-bin rows that are not source-blocked keep the stream-order sum
+        tb = g.in_degree <= info.params.thread_max_deg
         np.testing.assert_array_equal(y[tb], exp[tb])
-        xq = oracle.fixed_encode(32, x).astype(np.uint32)
-        wq = oracle.fixed_encode(32, g.inv_weight).astype(np.uint32)
-        yq, _ = dg.spmv_fixed(xq, reps=2)
-        np.testing.assert_array_equal(yq, oracle.spmv_fixed(g.inv_idx, g.in_degree, g.inv_src,
-                                                            wq, 0, xq))
-        dg.close()
-    # unweighted graph: constant weight 0.0001f
-    dg = device.upload(g)
-    y, _ = dg.spmv(x)
-    exp = oracle.spmv_f32(g.inv_idx, g.in_degree, g.inv_src, None, x)
-    np.testing.assert_allclose(y, exp, rtol=2e-6, atol=1e-9)
+    xq = oracle.fixed_encode(32, x).astype(np.uint32)
+    wq = oracle.fixed_encode(32, g.inv_weight).astype(np.uint32)
+    want = oracle.spmv_fixed(g.inv_idx, g.in_degree, g.inv_src, wq, 0, xq)
+    yq, _ = dg.spmv_fixed(xq, reps=2)
+    np.testing.assert_array_equal(yq, want)
+    # caller-supplied weight codes (regular order): same result through the fallback schedule
+    import ctypes as C
+    u32p = C.POINTER(C.c_uint32)
+    idx, deg = np.ascontiguousarray(g.inv_idx, np.uint32), np.ascontiguousarray(g.in_degree, np.uint32)
+    wq2 = np.ascontiguousarray(wq[::-1])
+    assert ag.lib().ag_graph_set_fixed_weights(dg.handle, idx.ctypes.data_as(u32p),
+                                               deg.ctypes.data_as(u32p),
+                                               wq2.ctypes.data_as(u32p)) == 0
+    yq2, _ = dg.spmv_fixed(xq)
+    np.testing.assert_array_equal(yq2, oracle.spmv_fixed(g.inv_idx, g.in_degree, g.inv_src,
+                                                         wq2, 0, xq))
     dg.close()
+
+
+def test_spmv_unweighted(device):
+    say("spmv rmat-14 unweighted")
+    g = ag.Graph.rmat(14)
+    rng = np.random.default_rng(12)
+    x = (rng.integers(0, 2 ** 24, g.n) * 2.0 ** -24).astype(np.float32)  # [0, 1)
+    # unweighted graph: constant weight 0.0001f
+    for pname in ("default", "blocked-all"):
+        dg = upload(device, g, pname)
+        y, _ = dg.spmv(x)
+        exp = oracle.spmv_f32(g.inv_idx, g.in_degree, g.inv_src, None, x)
+        np.testing.assert_allclose(y, exp, rtol=2e-6, atol=1e-9)
+        xq = oracle.fixed_encode(32, x).astype(np.uint32)
+        yq, _ = dg.spmv_fixed(xq)
+        wc = int(oracle.fixed_encode(32, [0.0001])[0])
+        np.testing.assert_array_equal(yq, oracle.spmv_fixed(g.inv_idx, g.in_degree, g.inv_src,
+                                                            None, wc, xq))
+        dg.close()
 
 
 # ------------------------------------------------------------------ edge cases
