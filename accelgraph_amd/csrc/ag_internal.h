@@ -74,10 +74,17 @@ struct PullGeom {
 // Source-blocked pull: virtual rows, the combine schedule and the work items of each slice.
 struct BlockGeom {
     PullGeom V;              // off = voff, src = vsrc (virtual rows)
-    PullGeom C;              // off = coff, src = cidx, combine bins, tiles and slots
-    const uint32_t *item;    // [2 * n_items]
+    PullGeom C;              // off = coff, src = cidx (a row's partial slots)
+    const uint32_t *item;    // [2 * n_items] pass-1 items
+    const uint32_t *c_item;  // [2 * c_n_items] combine items
+    const uint32_t *ct_row;  // [2 * c_n_tiles]
+    const uint32_t *ct_start;
+    const uint32_t *ct_pre;
+    const uint16_t *lpos;
     uint64_t *part;          // [nv]
+    uint32_t nblocks;
     uint32_t item_off[AG_NXCD + 1];
+    uint32_t c_item_off[AG_NXCD + 1];
 };
 
 // Persistent-grid size of a pull launch: resident workgroups per CU x CUs, capped by the
@@ -143,7 +150,23 @@ PullGeom ag_make_geom(const ag_dgraph *g);
 // Regular schedule of the rows a source-blocked graph leaves unblocked (AG_ITEM_REG items).
 PullGeom ag_make_unblocked_geom(const ag_dgraph *g);
 BlockGeom ag_make_block_geom(const ag_dgraph *g);
-// Grids of the two passes of a source-blocked iteration (the first a multiple of AG_NXCD).
-uint32_t ag_blocked_grid(const ag_dgraph *g, uint32_t lds_bytes);
-uint32_t ag_combine_grid(const ag_dgraph *g, uint32_t lds_bytes);
+// Grid of either pass of a source-blocked iteration: CUs x workgroups per CU (the plan's
+// wg_per_cu, else default_per_cu, never more than resident_per_cu), a multiple of AG_NXCD. A
+// persistent grid must fit on the device at once: a workgroup that waits for a free slot runs
+// its whole share of the items after the others have finished. The gather pass defaults to 4
+// per CU (it is bound by the rate of random L2 requests: more in flight only queue); the
+// combine pass, whose reads are coalesced and whose time goes to dependent round trips,
+// defaults to as many as fit, up to 8.
+uint32_t ag_blocked_grid(const ag_dgraph *g, uint32_t resident_per_cu, uint32_t default_per_cu);
+
+// Workgroups of AG_WG lanes of `kernel` that fit on one CU (registers and LDS).
+template <typename K>
+static inline uint32_t ag_resident_per_cu(K kernel)
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, AG_WG, 0) != hipSuccess ||
+        nb < 1)
+        return 1u;
+    return (uint32_t)nb;
+}
 int ag_ensure_vec(ag_dgraph *g, size_t bytes);

@@ -223,13 +223,14 @@ __device__ __forceinline__ void pr_f32_blk(const PullGeom &U, const BlockGeom &B
 {
     __shared__ float s_vals[AG_TILE_NNZ_MAX];
     __shared__ float s_wave[AG_WG / AG_WAVE];
+    __shared__ BlkLds L;
     const PrState *st = C.st;
     if (*(volatile const uint32_t *)&st->done)
         return;
     const uint32_t iter = *(volatile const uint32_t *)&st->iter;
     Red red{0.0, 0u};
     pr_f32_dispatch<HT>(st, iter, A, mixed, [&](const auto &p) {
-        pull_blocked_items(U, B, p, p, red, s_vals, s_wave);
+        pull_blocked_items(U, B, p, p, red, s_vals, s_wave, L);
     });
     pr_store_partial(C, red);
 }
@@ -240,13 +241,14 @@ __device__ __forceinline__ void pr_f32_cmb(const BlockGeom &B, const PrCtl &C,
 {
     __shared__ float s_vals[AG_TILE_NNZ_MAX];
     __shared__ float s_wave[AG_WG / AG_WAVE];
+    __shared__ BlkLds L;
     const PrState *st = C.st;
     if (*(volatile const uint32_t *)&st->done)
         return;
     const uint32_t iter = *(volatile const uint32_t *)&st->iter;
     Red red{0.0, 0u};
     pr_f32_dispatch<HT>(st, iter, A, mixed, [&](const auto &p) {
-        pull_combine_items(B, p, red, s_vals, s_wave);
+        pull_combine_items(B, p, red, s_vals, s_wave, L);
     });
     pr_store_partial(C, red);
 }
@@ -399,13 +401,14 @@ __global__ __launch_bounds__(AG_WG) void pr_fixed_blk_kernel(PullGeom U, BlockGe
 {
     __shared__ uint64_t s_vals[AG_TILE_NNZ_MAX];
     __shared__ uint64_t s_wave[AG_WG / AG_WAVE];
+    __shared__ BlkLds L;
     const PrState *st = C.st;
     if (*(volatile const uint32_t *)&st->done)
         return;
     const uint32_t iter = *(volatile const uint32_t *)&st->iter;
     Red red{0.0, 0u};
     const PrFixPol<BITS> p = fix_policy<BITS>(iter, A);
-    pull_blocked_items(U, B, p, p, red, s_vals, s_wave);
+    pull_blocked_items(U, B, p, p, red, s_vals, s_wave, L);
     pr_store_partial(C, red);
 }
 
@@ -414,12 +417,13 @@ __global__ __launch_bounds__(AG_WG) void pr_fixed_cmb_kernel(BlockGeom B, PrCtl 
 {
     __shared__ uint64_t s_vals[AG_TILE_NNZ_MAX];
     __shared__ uint64_t s_wave[AG_WG / AG_WAVE];
+    __shared__ BlkLds L;
     const PrState *st = C.st;
     if (*(volatile const uint32_t *)&st->done)
         return;
     const uint32_t iter = *(volatile const uint32_t *)&st->iter;
     Red red{0.0, 0u};
-    pull_combine_items(B, fix_policy<BITS>(iter, A), red, s_vals, s_wave);
+    pull_combine_items(B, fix_policy<BITS>(iter, A), red, s_vals, s_wave, L);
     pr_store_partial(C, red);
 }
 
@@ -529,14 +533,11 @@ extern "C" void ag_pr_options_default(ag_pr_options *o)
     o->record_history = 0;
 }
 
-// Upper bound of the per-workgroup partials of one iteration (every pass, every kind).
+// Upper bound of the per-workgroup partials of one iteration (every pass, every kind): two
+// passes of at most 32 workgroups per CU.
 static uint32_t ag_pr_max_parts(const ag_dgraph *g)
 {
-    const uint32_t lds = AG_TILE_NNZ_MAX * 4u;   // the smallest LDS use gives the largest grid
-    uint32_t n = ag_pull_grid(g, lds);
-    if (g->blk.bits)
-        n = ag_blocked_grid(g, lds) + ag_combine_grid(g, lds);
-    return n ? n : 1u;
+    return 2u * g->dev->num_cus * 32u + AG_NXCD;
 }
 
 static void session_free(ag_pr_session *s)
@@ -716,13 +717,18 @@ extern "C" int ag_pr_begin(ag_dgraph *g, const ag_pr_options *opts, ag_pr_sessio
         FIN<<<1, fin, 0, stream>>>(G, C, A);                 \
     } while (0)
 
-#define AG_PR_LAUNCH_BLOCKED(BLK, CMB, FIN)                  \
-    do {                                                     \
-        C.part_base = 0;                                     \
-        BLK<<<g1, block, 0, stream>>>(G, B, C, A);           \
-        C.part_base = g1;                                    \
-        CMB<<<g2, block, 0, stream>>>(B, C, A);              \
-        FIN<<<1, fin, 0, stream>>>(G, C, A);                 \
+#define AG_PR_LAUNCH_BLOCKED(BLK, CMB, FIN)                                  \
+    do {                                                                     \
+        static const uint32_t res1 = ag_resident_per_cu(BLK);               \
+        static const uint32_t res2 = ag_resident_per_cu(CMB);               \
+        const uint32_t g1 = ag_blocked_grid(g, res1, 4u);                    \
+        const uint32_t g2 = ag_blocked_grid(g, res2, 8u);                    \
+        C.nparts = g1 + g2;                                                  \
+        C.part_base = 0;                                                     \
+        BLK<<<g1, block, 0, stream>>>(G, B, C, A);                           \
+        C.part_base = g1;                                                    \
+        CMB<<<g2, block, 0, stream>>>(B, C, A);                              \
+        FIN<<<1, fin, 0, stream>>>(G, C, A);                                 \
     } while (0)
 
 int ag_pr_launch_iteration(ag_pr_session *s)
@@ -757,8 +763,6 @@ int ag_pr_launch_iteration(ag_pr_session *s)
     dim3 block(AG_WG), fin(AG_FIN_WG);
     if (blocked) {
         const BlockGeom B = ag_make_block_geom(g);
-        const uint32_t g1 = ag_blocked_grid(g, lds), g2 = ag_combine_grid(g, lds);
-        C.nparts = g1 + g2;
         switch (s->kind_code) {
         case 0: AG_PR_LAUNCH_BLOCKED(pr_f32_blk_kernel, pr_f32_cmb_kernel, pr_f32_finish); break;
         case 1: AG_PR_LAUNCH_BLOCKED(pr_bf16_blk_kernel, pr_bf16_cmb_kernel, pr_bf16_finish); break;

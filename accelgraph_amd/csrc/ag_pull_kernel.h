@@ -259,6 +259,9 @@ __device__ __forceinline__ void pull_items(const PullGeom &G, const P &pol, Red 
 
 // ---------------------------------------------------------------- source-blocked pull
 
+#define AG_LANE_MAX 32u    // rows of a tile with at most this many elements: a lane each
+#define AG_LONG_MAX 128u   // longer rows of one tile: at most AG_TILE_NNZ_MAX / 33 < 128
+
 // First pass: a virtual row's sum is stored as its partial; the update waits for the combine.
 template <class P>
 struct PartPol {
@@ -282,17 +285,94 @@ struct CombPol {
     }
 };
 
+// Gathers in-edges [e0, e0 + nnz) (nnz <= AG_TILE_NNZ_MAX) into s_vals with coalesced
+// source-id reads, AG_TILE_UNROLL loads then gathers in flight per lane.
+template <class P>
+__device__ __forceinline__ void stage_edges(const uint32_t *src, uint32_t e0, uint32_t nnz,
+                                            const P &pol, typename P::Acc *s_vals)
+{
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t k = tid; k < nnz; k += AG_TILE_UNROLL * AG_WG) {
+        uint32_t sidx[AG_TILE_UNROLL];
+#pragma unroll
+        for (int u = 0; u < AG_TILE_UNROLL; u++) {
+            const uint32_t kk = k + u * AG_WG;
+            sidx[u] = kk < nnz ? ld_stream(&src[e0 + kk]) : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < AG_TILE_UNROLL; u++) {
+            const uint32_t kk = k + u * AG_WG;
+            if (kk < nnz)
+                s_vals[kk] = pol.edge(e0 + kk, sidx[u]);
+        }
+    }
+}
+
+// Rows [r0, r1) whose elements are s_vals[bnd[r] - base, bnd[r + 1] - base): a row of at
+// most AG_LANE_MAX elements is summed by one lane in element order (the first element starts
+// the sum); a longer row by a wavefront (lane-strided sums combined in a fixed tree). A row's
+// value depends only on its elements, whichever lane or wavefront computes it.
+template <class P>
+__device__ __forceinline__ void mixed_reduce(const uint32_t *bnd, uint32_t base, uint32_t r0,
+                                             uint32_t r1, const P &pol, Red &red,
+                                             const typename P::Acc *s_vals, uint32_t *s_long,
+                                             uint32_t *s_nlong)
+{
+    using Acc = typename P::Acc;
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0)
+        *s_nlong = 0u;
+    __syncthreads();
+    for (uint32_t r = r0 + tid; r < r1; r += AG_WG) {
+        const uint32_t b = bnd[r] - base, e = bnd[r + 1] - base;
+        if (e - b > AG_LANE_MAX) {
+            s_long[atomicAdd(s_nlong, 1u)] = r;
+            continue;
+        }
+        Acc acc = Acc(0);
+        if (e > b) {
+            acc = s_vals[b];
+            for (uint32_t q = b + 1; q < e; q++)
+                acc = acc + s_vals[q];
+        }
+        pol.finish(r, acc, red);
+    }
+    __syncthreads();
+    const uint32_t nlong = *s_nlong;
+    const uint32_t lane = tid % AG_WAVE;
+    for (uint32_t i = tid / AG_WAVE; i < nlong; i += AG_WG / AG_WAVE) {
+        const uint32_t r = s_long[i];
+        const uint32_t b = bnd[r] - base, e = bnd[r + 1] - base;
+        Acc acc = Acc(0);
+        for (uint32_t q = b + lane; q < e; q += AG_WAVE)
+            acc = acc + s_vals[q];
+        acc = group_reduce<Acc, AG_WAVE>(acc);
+        if (lane == 0)
+            pol.finish(r, acc, red);
+    }
+    __syncthreads();  // s_vals and s_long are reused by this workgroup's next tile
+}
+
 // First pass of a source-blocked iteration. Workgroup b serves slice b % AG_NXCD (the XCD it
-// is dispatched to) and walks that slice's items, which go source block by source block, so
+// is dispatched to) and walks that slice's tiles, which go source block by source block, so
 // the gathers of the workgroups of one XCD stay inside about one block of the contribution
 // vector, which that XCD's L2 holds. `pol` gathers through virtual-row edge positions (its
 // per-edge arrays, if any, are in virtual-row order); AG_ITEM_REG items run the unblocked
 // rows' regular schedule U with `upol`.
+// LDS of the blocked passes beyond the tile values, declared once per kernel (a kernel that
+// instantiates a pass for several policies shares it).
+struct BlkLds {
+    uint32_t pre[AG_MAX_SRC_BLOCKS + 1];   // combine: the tile's partials before each block
+    uint32_t start[AG_MAX_SRC_BLOCKS];     // combine: the tile's first virtual row per block
+    uint32_t lng[AG_LONG_MAX];             // rows of a tile summed by a wavefront
+    uint32_t nlong;
+};
+
 template <class P, class PU>
 __device__ __forceinline__ void pull_blocked_items(const PullGeom &U, const BlockGeom &B,
                                                    const P &pol, const PU &upol, Red &red,
                                                    typename P::Acc *s_vals,
-                                                   typename P::Acc *s_wave)
+                                                   typename P::Acc *s_wave, BlkLds &L)
 {
     using Acc = typename P::Acc;
     const PartPol<P> pp{pol, (Acc *)B.part};
@@ -301,32 +381,65 @@ __device__ __forceinline__ void pull_blocked_items(const PullGeom &U, const Bloc
     const uint32_t end = B.item_off[x + 1];
     for (uint32_t i = B.item_off[x] + blockIdx.x / AG_NXCD; i < end; i += step) {
         const uint32_t first = B.item[2 * i], kc = B.item[2 * i + 1];
-        const uint32_t cnt = kc >> 4;
-        switch (kc & 15u) {
-        case AG_ITEM_REG:
+        if ((kc & 15u) == AG_ITEM_REG) {
             pull_one_item(U, upol, first, red, s_vals, s_wave);
-            break;
-        case AG_ITEM_VTILE:
-            tile_rows(B.V, pp, first, first + cnt, red, s_vals);
-            break;
-        case AG_ITEM_VG16:
-            group_rows<PartPol<P>, 16>(B.V, pp, first, first + cnt, red);
-            break;
-        default:
-            group_rows<PartPol<P>, AG_WAVE>(B.V, pp, first, first + cnt, red);
-            break;
+        } else {
+            const uint32_t last = first + (kc >> 4);
+            const uint32_t e0 = B.V.off[first];
+            stage_edges(B.V.src, e0, B.V.off[last] - e0, pp, s_vals);
+            __syncthreads();
+            mixed_reduce(B.V.off, e0, first, last, pp, red, s_vals, L.lng, &L.nlong);
         }
     }
 }
 
-// Second pass: the regular schedule of the combine CSR, finishing every blocked row.
+// Second pass. A combine tile (rows [r0, r1) of one slice) streams its partials block by
+// block -- for each source block they are one contiguous run of virtual rows -- into LDS at
+// their row-major slots (lpos), then each row adds its partials in block order and applies
+// the update. A row with more partials than a tile holds is summed by a workgroup through
+// cidx. The items are spread over the whole grid: a slice's cost here is per row, and the
+// slices are balanced for the first pass, not for this one.
 template <class P>
 __device__ __forceinline__ void pull_combine_items(const BlockGeom &B, const P &pol, Red &red,
                                                    typename P::Acc *s_vals,
-                                                   typename P::Acc *s_wave)
+                                                   typename P::Acc *s_wave, BlkLds &L)
 {
     using Acc = typename P::Acc;
-    pull_items(B.C, CombPol<P>{pol, (const Acc *)B.part}, red, s_vals, s_wave);
+    uint32_t *s_pre = L.pre, *s_start = L.start;
+    const CombPol<P> cp{pol, (const Acc *)B.part};
+    const uint32_t nb = B.nblocks, tid = threadIdx.x;
+    const uint32_t end = B.c_item_off[AG_NXCD];
+    for (uint32_t i = blockIdx.x; i < end; i += gridDim.x) {
+        const uint32_t first = B.c_item[2 * i], kind = B.c_item[2 * i + 1] & 15u;
+        if (kind != AG_ITEM_CTILE) {
+            pull_block_row(B.C, cp, first, red, s_wave);
+            continue;
+        }
+        const uint32_t r0 = B.ct_row[2 * first], r1 = B.ct_row[2 * first + 1];
+        const uint32_t p0 = B.C.off[r0], np = B.C.off[r1] - p0;  // np <= AG_TILE_NNZ_MAX
+        const uint32_t *pre = B.ct_pre + (size_t)first * (nb + 1);
+        const uint32_t *start = B.ct_start + (size_t)first * nb;
+        for (uint32_t b = tid; b <= nb; b += AG_WG)
+            s_pre[b] = pre[b];
+        for (uint32_t b = tid; b < nb; b += AG_WG)
+            s_start[b] = start[b];
+        __syncthreads();
+        for (uint32_t k = tid; k < np; k += AG_WG) {
+            // block of the k-th partial: s_pre[lo] <= k < s_pre[lo + 1] (s_pre[nb] = np)
+            uint32_t lo = 0, hi = nb;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_pre[mid] <= k)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            const uint32_t v = s_start[lo] + (k - s_pre[lo]);
+            s_vals[B.lpos[v]] = cp.part[v];
+        }
+        __syncthreads();
+        mixed_reduce(B.C.off, p0, r0, r1, cp, red, s_vals, L.lng, &L.nlong);
+    }
 }
 
 // Finish step of the hub rows: add each row's segment partials in segment order.
@@ -361,8 +474,9 @@ __global__ __launch_bounds__(AG_WG) void pull_blocked_kernel(PullGeom U, BlockGe
 {
     __shared__ typename P::Acc s_vals[AG_TILE_NNZ_MAX];
     __shared__ typename P::Acc s_wave[AG_WG / AG_WAVE];
+    __shared__ BlkLds L;
     Red red{0.0, 0u};
-    pull_blocked_items(U, B, pol, upol, red, s_vals, s_wave);
+    pull_blocked_items(U, B, pol, upol, red, s_vals, s_wave, L);
 }
 
 template <class P>
@@ -370,8 +484,9 @@ __global__ __launch_bounds__(AG_WG) void pull_combine_kernel(BlockGeom B, P pol)
 {
     __shared__ typename P::Acc s_vals[AG_TILE_NNZ_MAX];
     __shared__ typename P::Acc s_wave[AG_WG / AG_WAVE];
+    __shared__ BlkLds L;
     Red red{0.0, 0u};
-    pull_combine_items(B, pol, red, s_vals, s_wave);
+    pull_combine_items(B, pol, red, s_vals, s_wave, L);
 }
 
 template <class P>

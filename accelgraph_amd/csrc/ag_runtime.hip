@@ -207,7 +207,8 @@ extern "C" int ag_graph_free(ag_dgraph *g)
     void *ptrs[] = {g->d2o, g->off, g->src, g->w, g->wq, g->outdeg, g->tile_row, g->seg_row,
                     g->seg_lo, g->seg_hi, g->hub_seg0, g->hub_part, g->vec_a, g->vec_b,
                     g->blk.voff, g->blk.vsrc, g->blk.vw, g->blk.item, g->blk.coff, g->blk.cidx,
-                    g->blk.c_tile_row, g->vpart, g->vwq};
+                    g->blk.lpos, g->blk.ct_row, g->blk.ct_start, g->blk.ct_pre, g->blk.c_item,
+                    g->vpart, g->vwq};
     for (void *p : ptrs)
         if (p)
             hipFree(p);
@@ -258,17 +259,19 @@ BlockGeom ag_make_block_geom(const ag_dgraph *g)
     K.V.n = B.nv;
     K.C.off = B.coff;
     K.C.src = B.cidx;
-    K.C.tile_row = B.c_tile_row;
     K.C.n = g->n;
-    K.C.n_hubs = 0;
-    for (int b = 0; b <= AG_NBINS; b++)
-        K.C.bin_row[b] = B.c_bin_row[b];
-    for (int s = 0; s <= AG_NSLOTS; s++)
-        K.C.slot_blk0[s] = B.c_slot_blk0[s];
     K.item = B.item;
+    K.c_item = B.c_item;
+    K.ct_row = B.ct_row;
+    K.ct_start = B.ct_start;
+    K.ct_pre = B.ct_pre;
+    K.lpos = B.lpos;
     K.part = g->vpart;
-    for (uint32_t x = 0; x <= AG_NXCD; x++)
+    K.nblocks = B.nblocks;
+    for (uint32_t x = 0; x <= AG_NXCD; x++) {
         K.item_off[x] = B.item_off[x];
+        K.c_item_off[x] = B.c_item_off[x];
+    }
     return K;
 }
 
@@ -285,21 +288,15 @@ static uint32_t wg_per_cu(const ag_dgraph *g, uint32_t lds_bytes, uint32_t dflt 
     return per_cu ? per_cu : 1u;
 }
 
-uint32_t ag_blocked_grid(const ag_dgraph *g, uint32_t lds_bytes)
+uint32_t ag_blocked_grid(const ag_dgraph *g, uint32_t resident_per_cu, uint32_t default_per_cu)
 {
     // workgroup i runs on XCD i % AG_NXCD (round-robin dispatch), so slice x is served by the
     // workgroups with that residue; the grid is a multiple of AG_NXCD
-    uint32_t grid = g->dev->num_cus * wg_per_cu(g, lds_bytes);
-    grid = (grid + AG_NXCD - 1) / AG_NXCD * AG_NXCD;
+    uint32_t per_cu = g->params.wg_per_cu ? g->params.wg_per_cu : default_per_cu;
+    if (per_cu > resident_per_cu)
+        per_cu = resident_per_cu;
+    uint32_t grid = g->dev->num_cus * (per_cu ? per_cu : 1u) / AG_NXCD * AG_NXCD;
     return grid ? grid : AG_NXCD;
-}
-
-uint32_t ag_combine_grid(const ag_dgraph *g, uint32_t lds_bytes)
-{
-    uint32_t grid = g->dev->num_cus * wg_per_cu(g, lds_bytes);
-    uint32_t total = g->blk.c_slot_blk0[AG_NSLOTS];
-    grid = grid < total ? grid : total;
-    return grid ? grid : 1u;
 }
 
 uint32_t ag_pull_grid(const ag_dgraph *g, uint32_t lds_bytes, uint32_t default_per_cu)
@@ -394,16 +391,21 @@ extern "C" int ag_graph_upload(ag_device *dev, uint32_t n, uint32_t m, const uin
         const ag_block_plan &B = plan.blk;
         g->blk = B;   // scalars; the pointers are replaced by device copies below
         g->blk.voff = g->blk.vsrc = g->blk.item = g->blk.coff = g->blk.cidx = NULL;
-        g->blk.c_tile_row = NULL;
+        g->blk.ct_row = g->blk.ct_start = g->blk.ct_pre = g->blk.c_item = NULL;
+        g->blk.lpos = NULL;
         g->blk.vw = NULL;
+        const size_t nt = B.c_n_tiles, nb = B.nblocks;
         st = dev_upload(g, &g->blk.voff, B.voff, (size_t)B.nv + 1);
         if (st == AG_OK) st = dev_upload(g, &g->blk.vsrc, B.vsrc, B.mb);
         if (st == AG_OK && B.vw) st = dev_upload(g, &g->blk.vw, B.vw, B.mb);
         if (st == AG_OK) st = dev_upload(g, &g->blk.item, B.item, (size_t)B.n_items * 2);
         if (st == AG_OK) st = dev_upload(g, &g->blk.coff, B.coff, (size_t)plan.n + 1);
         if (st == AG_OK) st = dev_upload(g, &g->blk.cidx, B.cidx, B.nv);
-        if (st == AG_OK)
-            st = dev_upload(g, &g->blk.c_tile_row, B.c_tile_row, (size_t)B.c_n_tiles + 1);
+        if (st == AG_OK) st = dev_upload(g, &g->blk.lpos, B.lpos, B.nv);
+        if (st == AG_OK) st = dev_upload(g, &g->blk.ct_row, B.ct_row, nt * 2);
+        if (st == AG_OK) st = dev_upload(g, &g->blk.ct_start, B.ct_start, nt * nb);
+        if (st == AG_OK) st = dev_upload(g, &g->blk.ct_pre, B.ct_pre, nt * (nb + 1));
+        if (st == AG_OK) st = dev_upload(g, &g->blk.c_item, B.c_item, (size_t)B.c_n_items * 2);
         if (st == AG_OK) {
             size_t pb = (size_t)(B.nv ? B.nv : 1) * sizeof(uint64_t);
             he = hipMalloc((void **)&g->vpart, pb);

@@ -128,8 +128,9 @@ static int run_pull_blocked(ag_dgraph *g, const P &pv, const P &pu, uint32_t rep
     hipStream_t stream = g->dev->stream;
     const PullGeom U = ag_make_unblocked_geom(g);
     const BlockGeom B = ag_make_block_geom(g);
-    const uint32_t lds = AG_TILE_NNZ_MAX * (uint32_t)sizeof(typename P::Acc);
-    const uint32_t g1 = ag_blocked_grid(g, lds), g2 = ag_combine_grid(g, lds);
+    static const uint32_t res1 = ag_resident_per_cu(pull_blocked_kernel<P, P>);
+    static const uint32_t res2 = ag_resident_per_cu(pull_combine_kernel<P>);
+    const uint32_t g1 = ag_blocked_grid(g, res1, 4u), g2 = ag_blocked_grid(g, res2, 8u);
     hipEvent_t e0 = NULL, e1 = NULL;
     if (avg_ms) {
         AG_HIP(hipEventCreate(&e0));

@@ -27,26 +27,30 @@ extern "C" {
 #define AG_NSLOTS AG_NBINS
 extern const uint32_t ag_slot_bin[AG_NSLOTS];
 
-/* Source blocking (PageRank pull on large graphs). The blocked rows (bins >= blocked_min_bin,
- * the device-row suffix [row0, n)) are cut into AG_NXCD contiguous slices of about equal
- * in-edge counts, one per accelerator complex (XCD, which owns its own L2). Every blocked row's
- * in-edges are regrouped by source block (device id >> bits), keeping stream order inside a
- * block; each (row, block) group, split into pieces of at most AG_PIECE_NNZ edges, is a
- * "virtual row" whose sum is one partial. Virtual rows are ordered by (slice, block, class,
- * row, piece) and each slice's work items walk the blocks in order, so at any time the
- * gathers of one XCD fall in one block of the contribution vector. A second pass adds each
- * row's partials in block order (a CSR over the partials, coff/cidx) and applies the update. */
+/* Source blocking (PageRank pull and SpMV on large graphs). The blocked rows (bins >=
+ * blocked_min_bin, the device-row suffix [row0, n)) are cut into AG_NXCD contiguous slices of
+ * about equal in-edge counts, one per accelerator complex (XCD, which owns its own L2). Every
+ * blocked row's in-edges are regrouped by source block (device id >> bits), keeping stream
+ * order inside a block; each (row, block) group, split into pieces of at most
+ * min(AG_PIECE_NNZ, tile_nnz) edges, is a "virtual row" whose sum is one partial. Virtual rows
+ * are ordered by (slice, block, row, piece) and each slice's pass-1 tiles walk the blocks in
+ * order, so at any time the gathers of one XCD fall in one block of the contribution vector.
+ * The combine pass cuts each slice's rows into tiles; a tile streams its partials block by
+ * block (rows are sorted inside a block, so they are contiguous), drops each into LDS at its
+ * row-major position lpos, and adds every row's partials in block order. */
 #define AG_NXCD 8u
 #define AG_PIECE_NNZ 4096u
-#define AG_MAX_SRC_BLOCKS 4096u
+#define AG_MAX_SRC_BLOCKS 1024u
+#define AG_CTILE_ROWS 4096u     /* rows of one combine tile */
 #define AG_AUTO_BLOCK_BITS 19u
 #define AG_AUTO_BLOCK_MIN_N (1u << 21)
 
-/* work item kinds of the blocked pass: item = {first, kind | count << 4} */
-enum { AG_ITEM_REG = 0,     /* first = item of the unblocked rows' regular schedule      */
-       AG_ITEM_VTILE = 1,   /* virtual rows [first, first+count): LDS tile, lane per row  */
-       AG_ITEM_VG16 = 2,    /* virtual rows [first, first+count), count <= 16: 16 lanes each */
-       AG_ITEM_VWAVE = 3 }; /* virtual rows [first, first+count), count <= 4: a wave each */
+/* work items of the blocked passes: item = {first, kind | count << 4} */
+enum { AG_ITEM_REG = 0,     /* pass 1: first = item of the unblocked rows' regular schedule */
+       AG_ITEM_VTILE = 1,   /* pass 1: virtual rows [first, first + count) as one LDS tile  */
+       AG_ITEM_CTILE = 2,   /* combine: tile `first`                                        */
+       AG_ITEM_CBIG = 3 };  /* combine: row `first` (more partials than a tile holds), one
+                               workgroup through cidx                                        */
 
 typedef struct ag_block_plan {
     uint32_t bits;          /* 0: no source blocking (the fields below are empty)       */
@@ -59,16 +63,18 @@ typedef struct ag_block_plan {
     uint32_t *vsrc;         /* [mb]   device source ids, virtual-row order              */
     float *vw;              /* [mb]   weights in the same order, or NULL                */
     uint32_t n_items;
-    uint32_t *item;         /* [2*n_items]                                              */
+    uint32_t *item;         /* [2*n_items] pass-1 items                                 */
     uint32_t item_off[AG_NXCD + 1]; /* items of slice x: [item_off[x], item_off[x+1])   */
     uint32_t *coff;         /* [n+1]  partial slots of each row (empty below row0)      */
     uint32_t *cidx;         /* [nv]   virtual row of each slot, block order within a row */
-    /* combine pass: the regular bin schedule over the blocked rows, with coff/cidx as the
-     * CSR; hub rows join the block bin (a row has at most a few hundred partials) */
-    uint32_t c_bin_row[AG_NBINS + 1];
+    uint16_t *lpos;         /* [nv]   slot of each virtual row relative to its combine tile */
     uint32_t c_n_tiles;
-    uint32_t *c_tile_row;   /* [c_n_tiles+1]                                            */
-    uint32_t c_slot_blk0[AG_NSLOTS + 1];
+    uint32_t *ct_row;       /* [2*c_n_tiles] first and end row of each combine tile     */
+    uint32_t *ct_start;     /* [c_n_tiles*nblocks] first virtual row of the tile in a block */
+    uint32_t *ct_pre;       /* [c_n_tiles*(nblocks+1)] the tile's partials before block b */
+    uint32_t c_n_items;
+    uint32_t *c_item;       /* [2*c_n_items] combine items                              */
+    uint32_t c_item_off[AG_NXCD + 1];
     /* regular schedule restricted to the unblocked rows (the AG_ITEM_REG items)         */
     uint32_t r_slot_blk0[AG_NSLOTS + 1];
 } ag_block_plan;

@@ -93,7 +93,11 @@ void ag_plan_free(ag_plan *p)
     free(p->blk.item);
     free(p->blk.coff);
     free(p->blk.cidx);
-    free(p->blk.c_tile_row);
+    free(p->blk.lpos);
+    free(p->blk.ct_row);
+    free(p->blk.ct_start);
+    free(p->blk.ct_pre);
+    free(p->blk.c_item);
     memset(p, 0, sizeof(*p));
 }
 
@@ -165,14 +169,8 @@ static uint32_t block_bits_for(const ag_plan *p, const char **err)
         return b;
     }
     if ((((uint64_t)p->n + (1ull << b) - 1) >> b) > AG_MAX_SRC_BLOCKS)
-        *err = "src_block_bits too small for this graph: more than 4096 source blocks";
+        *err = "src_block_bits too small for this graph: more than 1024 source blocks";
     return b;
-}
-
-/* class of a virtual row by length: 0 = wave, 1 = 16-lane group, 2 = tile (lane per row) */
-static inline uint32_t vclass(const ag_plan_params *q, uint32_t len)
-{
-    return len <= q->thread_max_deg ? 2u : (len <= q->group16_max_deg ? 1u : 0u);
 }
 
 /* Stable sort of one row's in-edges by source block into out_s/out_w. keys: scratch of d. */
@@ -207,20 +205,21 @@ static void sort_row_by_block(const uint32_t *src, const float *w, uint32_t d, u
     }
 }
 
-/* Calls emit(start, len, block) for every virtual row of a block-sorted row: one per run of
- * equal source block, split into pieces of at most AG_PIECE_NNZ. Returns the count. */
-static uint32_t row_pieces(const uint32_t *ss, uint32_t d, uint32_t bits, uint32_t *st,
-                           uint32_t *len, uint32_t *blk)
+/* The virtual rows of a block-sorted row: one per run of equal source block, split into
+ * pieces of at most `piece` edges. With st != NULL records start, length and block of each.
+ * Returns the count. */
+static uint32_t row_pieces(const uint32_t *ss, uint32_t d, uint32_t bits, uint32_t piece,
+                           uint32_t *st, uint32_t *len, uint32_t *blk)
 {
     uint32_t cnt = 0, i = 0;
     while (i < d) {
         uint32_t b = ss[i] >> bits, j = i + 1;
         while (j < d && (ss[j] >> bits) == b)
             j++;
-        for (uint32_t a = i; a < j; a += AG_PIECE_NNZ) {
+        for (uint32_t a = i; a < j; a += piece) {
             if (st) {
                 st[cnt] = a;
-                len[cnt] = (j - a > AG_PIECE_NNZ) ? AG_PIECE_NNZ : j - a;
+                len[cnt] = (j - a > piece) ? piece : j - a;
                 blk[cnt] = b;
             }
             cnt++;
@@ -230,51 +229,104 @@ static uint32_t row_pieces(const uint32_t *ss, uint32_t d, uint32_t bits, uint32
     return cnt;
 }
 
-/* Work items of every slice; item == NULL only counts them. */
+#define AG_EMIT(arr, k, first, kind, cnt)                                   \
+    do {                                                                    \
+        if (arr) {                                                          \
+            (arr)[2 * (k)] = (first);                                       \
+            (arr)[2 * (k) + 1] = (uint32_t)(kind) | ((uint32_t)(cnt) << 4); \
+        }                                                                   \
+        (k)++;                                                              \
+    } while (0)
+
+/* Pass-1 items of every slice: per block, greedy tiles of consecutive virtual rows (at most
+ * tile_nnz edges, so at most that many virtual rows), then the slice's share of the unblocked
+ * rows' items. item == NULL only counts them. */
 static uint32_t emit_items(const ag_plan *p, const uint32_t *kstart, const uint32_t *vlen,
                            uint32_t reg_total, uint32_t *item, uint32_t *item_off)
 {
-    const ag_block_plan *B = &p->blk;
-    const uint32_t nb = B->nblocks;
+    const uint32_t nb = p->blk.nblocks;
     uint32_t k = 0;
-#define AG_EMIT(first, kind, cnt)                                        \
-    do {                                                                 \
-        if (item) {                                                      \
-            item[2 * k] = (first);                                       \
-            item[2 * k + 1] = (uint32_t)(kind) | ((uint32_t)(cnt) << 4); \
-        }                                                                \
-        k++;                                                             \
-    } while (0)
     for (uint32_t x = 0; x < AG_NXCD; x++) {
         if (item_off)
             item_off[x] = k;
         for (uint32_t b = 0; b < nb; b++) {
-            const uint32_t K = (x * nb + b) * 3;
-            for (uint32_t v = kstart[K]; v < kstart[K + 1]; v += 4)
-                AG_EMIT(v, AG_ITEM_VWAVE, (kstart[K + 1] - v < 4) ? kstart[K + 1] - v : 4);
-            for (uint32_t v = kstart[K + 1]; v < kstart[K + 2]; v += 16)
-                AG_EMIT(v, AG_ITEM_VG16, (kstart[K + 2] - v < 16) ? kstart[K + 2] - v : 16);
-            uint32_t v = kstart[K + 2];
-            const uint32_t e = kstart[K + 3];
+            uint32_t v = kstart[x * nb + b];
+            const uint32_t e = kstart[x * nb + b + 1];
             while (v < e) {
                 uint32_t t0 = v, rows = 0, nnz = 0;
-                while (v < e && rows < p->params.tile_rows) {
+                while (v < e) {
                     if (rows > 0 && nnz + vlen[v] > p->params.tile_nnz)
                         break;
                     nnz += vlen[v];
                     rows++;
                     v++;
                 }
-                AG_EMIT(t0, AG_ITEM_VTILE, rows);
+                AG_EMIT(item, k, t0, AG_ITEM_VTILE, rows);
             }
         }
         for (uint32_t r = x; r < reg_total; r += AG_NXCD)
-            AG_EMIT(r, AG_ITEM_REG, 0);
+            AG_EMIT(item, k, r, AG_ITEM_REG, 0);
     }
     if (item_off)
         item_off[AG_NXCD] = k;
-#undef AG_EMIT
     return k;
+}
+
+/* Combine tiles and items of every slice: runs of rows with at most tile_nnz partials cut
+ * into tiles of at most AG_CTILE_ROWS rows and tile_nnz partials; a row with more partials is
+ * an AG_ITEM_CBIG item of its own. ct_row/item == NULL only count. */
+static uint32_t emit_combine(const ag_plan *p, uint32_t *ct_row, uint32_t *n_tiles,
+                             uint32_t *item, uint32_t *item_off)
+{
+    const ag_block_plan *B = &p->blk;
+    const uint32_t *coff = B->coff;
+    uint32_t k = 0, t = 0;
+    for (uint32_t x = 0; x < AG_NXCD; x++) {
+        if (item_off)
+            item_off[x] = k;
+        uint32_t r = B->slice_row[x];
+        const uint32_t hi = B->slice_row[x + 1];
+        while (r < hi) {
+            if (coff[r + 1] - coff[r] > p->params.tile_nnz) {
+                AG_EMIT(item, k, r, AG_ITEM_CBIG, 0);
+                r++;
+                continue;
+            }
+            uint32_t r0 = r, rows = 0, nnz = 0;
+            while (r < hi && rows < AG_CTILE_ROWS) {
+                uint32_t c = coff[r + 1] - coff[r];
+                if (c > p->params.tile_nnz || (rows > 0 && nnz + c > p->params.tile_nnz))
+                    break;
+                nnz += c;
+                rows++;
+                r++;
+            }
+            if (ct_row) {
+                ct_row[2 * t] = r0;
+                ct_row[2 * t + 1] = r;
+            }
+            AG_EMIT(item, k, t, AG_ITEM_CTILE, 0);
+            t++;
+        }
+    }
+    if (item_off)
+        item_off[AG_NXCD] = k;
+    *n_tiles = t;
+    return k;
+}
+#undef AG_EMIT
+
+/* first index i in [lo, hi) with key[i] >= r (key sorted on the range) */
+static uint32_t lower_bound_u32(const uint32_t *key, uint32_t lo, uint32_t hi, uint32_t r)
+{
+    while (lo < hi) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        if (key[mid] < r)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
 }
 
 static int build_block_plan(ag_plan *p, const char **err)
@@ -295,29 +347,15 @@ static int build_block_plan(ag_plan *p, const char **err)
         return AG_OK;
     const uint32_t e0 = p->off[row0], mb = p->off[n] - e0;
     const uint32_t nb = (uint32_t)((((uint64_t)n) + (1ull << bits) - 1) >> bits);
-    const uint32_t nK = AG_NXCD * nb * 3;
+    const uint32_t nK = AG_NXCD * nb;
     const ag_plan_params *q = &p->params;
+    const uint32_t piece = q->tile_nnz < AG_PIECE_NNZ ? q->tile_nnz : AG_PIECE_NNZ;
     int oom = 0;
 
     B->bits = bits;
     B->nblocks = nb;
     B->row0 = row0;
     B->mb = mb;
-    /* slices: contiguous blocked rows with about mb/AG_NXCD in-edges each */
-    B->slice_row[0] = row0;
-    for (uint32_t x = 1; x < AG_NXCD; x++) {
-        uint64_t target = e0 + (uint64_t)mb * x / AG_NXCD;
-        uint32_t lo = B->slice_row[x - 1], hi = n;
-        while (lo < hi) {
-            uint32_t mid = lo + (hi - lo) / 2;
-            if (p->off[mid] < target)
-                lo = mid + 1;
-            else
-                hi = mid;
-        }
-        B->slice_row[x] = lo;
-    }
-    B->slice_row[AG_NXCD] = n;
 
     uint32_t *ss = (uint32_t *)zalloc(mb, sizeof(uint32_t));
     float *sw = p->w ? (float *)zalloc(mb, sizeof(float)) : NULL;
@@ -351,7 +389,7 @@ static int build_block_plan(ag_plan *p, const char **err)
             }
             sort_row_by_block(p->src + p->off[r], p->w ? p->w + p->off[r] : NULL, d, bits,
                               ss + o, sw ? sw + o : NULL, keys);
-            nvr[r - row0] = row_pieces(ss + o, d, bits, NULL, NULL, NULL);
+            nvr[r - row0] = row_pieces(ss + o, d, bits, piece, NULL, NULL, NULL);
         }
         free(keys);
     }
@@ -374,12 +412,33 @@ static int build_block_plan(ag_plan *p, const char **err)
     }
     free(nvr);
     const uint32_t nv = B->nv;
+    /* slices: contiguous blocked rows of about equal pass-1 cost, in-edges + 2 x virtual
+     * rows (a virtual row costs its offset read and partial write, about two gathers) */
+    {
+        const uint64_t total = (uint64_t)mb + 2ull * nv;
+        B->slice_row[0] = row0;
+        for (uint32_t x = 1; x < AG_NXCD; x++) {
+            const uint64_t target = total * x / AG_NXCD;
+            uint32_t lo = B->slice_row[x - 1], hi = n;
+            while (lo < hi) {
+                uint32_t mid = lo + (hi - lo) / 2;
+                if ((uint64_t)(p->off[mid] - e0) + 2ull * B->coff[mid] < target)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            B->slice_row[x] = lo;
+        }
+        B->slice_row[AG_NXCD] = n;
+    }
 
-    /* 2: every virtual row in (row, block, piece) slot order: key, start in ss, length */
+    /* 2: every virtual row in (row, block, piece) slot order: key, start in ss, length, row */
     uint32_t *vkey = (uint32_t *)zalloc(nv, sizeof(uint32_t));
     uint32_t *vst = (uint32_t *)zalloc(nv, sizeof(uint32_t));
     uint32_t *vln = (uint32_t *)zalloc(nv, sizeof(uint32_t));
+    uint32_t *vrow = (uint32_t *)zalloc(nv, sizeof(uint32_t));   /* row of each virtual row */
     B->cidx = (uint32_t *)zalloc(nv, sizeof(uint32_t));
+    B->lpos = (uint16_t *)zalloc(nv, sizeof(uint16_t));
     B->voff = (uint32_t *)zalloc((size_t)nv + 1, sizeof(uint32_t));
     B->vsrc = (uint32_t *)zalloc(mb, sizeof(uint32_t));
     if (p->w)
@@ -389,8 +448,8 @@ static int build_block_plan(ag_plan *p, const char **err)
     uint32_t *kstart = (uint32_t *)zalloc((size_t)nK + 1, sizeof(uint32_t));
     uint32_t *vlen = (uint32_t *)zalloc(nv, sizeof(uint32_t));
     uint32_t *vfrom = (uint32_t *)zalloc(nv, sizeof(uint32_t));
-    if (!vkey || !vst || !vln || !B->cidx || !B->voff || !B->vsrc || (p->w && !B->vw) || !hist ||
-        !kstart || !vlen || !vfrom) {
+    if (!vkey || !vst || !vln || !vrow || !B->cidx || !B->lpos || !B->voff || !B->vsrc ||
+        (p->w && !B->vw) || !hist || !kstart || !vlen || !vfrom) {
         oom = 1;
         goto done;
     }
@@ -400,13 +459,13 @@ static int build_block_plan(ag_plan *p, const char **err)
         uint32_t x = 0;
         while (x + 1 < AG_NXCD && r >= B->slice_row[x + 1])
             x++;
-        uint32_t cnt = row_pieces(ss + o, d, bits, vst + c, vln + c, vkey + c);
+        uint32_t cnt = row_pieces(ss + o, d, bits, piece, vst + c, vln + c, vkey + c);
         for (uint32_t j = 0; j < cnt; j++) {
-            vkey[c + j] = (x * nb + vkey[c + j]) * 3 + vclass(q, vln[c + j]);
+            vkey[c + j] = x * nb + vkey[c + j];
             vst[c + j] += o;
         }
     }
-    /* 3: stable counting sort of the slots by key: slot -> virtual row */
+    /* 3: stable counting sort of the slots by (slice, block): slot -> virtual row */
     #pragma omp parallel num_threads(nt)
     {
         const int t = omp_get_thread_num(), T = omp_get_num_threads();
@@ -428,11 +487,27 @@ static int build_block_plan(ag_plan *p, const char **err)
             }
             kstart[nK] = run;
         }
+        /* rows of this thread's slots: the slots [a, b) belong to consecutive rows */
+        uint32_t r = row0;
+        {
+            uint32_t lo = row0, hi = n;   /* first row with coff[r + 1] > a */
+            while (lo < hi) {
+                uint32_t mid = lo + (hi - lo) / 2;
+                if (B->coff[mid + 1] <= a)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            r = lo;
+        }
         for (uint32_t i = a; i < b; i++) {
+            while (B->coff[r + 1] <= i)
+                r++;
             uint32_t v = h[vkey[i]]++;
             B->cidx[i] = v;
             vlen[v] = vln[i];
             vfrom[v] = vst[i];
+            vrow[v] = r;
         }
     }
     {
@@ -449,43 +524,16 @@ static int build_block_plan(ag_plan *p, const char **err)
         if (B->vw)
             memcpy(B->vw + B->voff[v], sw + vfrom[v], (size_t)vlen[v] * sizeof(float));
     }
-
-    /* 4: combine schedule over the blocked rows. A row has at most in-degree partials, so
-     * when the wave bin's rows fit a tile every row below the block bin is combined by a lane
-     * from an LDS tile (sequential, in block order); block and hub rows get a workgroup. */
-    for (int k = 0; k < AG_BIN_HUB; k++)
-        B->c_bin_row[k] = p->bin_row[k] > row0 ? p->bin_row[k] : row0;
-    B->c_bin_row[AG_BIN_HUB] = n;
-    B->c_bin_row[AG_NBINS] = n;
-    if (q->wave_max_deg <= q->tile_nnz) {
-        B->c_bin_row[AG_BIN_GROUP16] = B->c_bin_row[AG_BIN_BLOCK];
-        B->c_bin_row[AG_BIN_WAVE] = B->c_bin_row[AG_BIN_BLOCK];
-    }
-    B->c_tile_row = cut_tiles(B->coff, B->c_bin_row[AG_BIN_THREAD],
-                              B->c_bin_row[AG_BIN_THREAD + 1], q->tile_rows, q->tile_nnz,
-                              &B->c_n_tiles);
-    if (!B->c_tile_row) {
-        oom = 1;
-        goto done;
-    }
     {
-        uint32_t cb[AG_NBINS];
-        cb[AG_BIN_THREAD] = B->c_n_tiles;
-        cb[AG_BIN_GROUP16] = (B->c_bin_row[2] - B->c_bin_row[1] + 15) / 16;
-        cb[AG_BIN_WAVE] = (B->c_bin_row[3] - B->c_bin_row[2] + 3) / 4;
-        cb[AG_BIN_BLOCK] = B->c_bin_row[4] - B->c_bin_row[3];
-        cb[AG_BIN_HUB] = 0;
         uint32_t rb[AG_NBINS];
         for (int k = 0; k < AG_NBINS; k++)
             rb[k] = ((uint32_t)k < min_bin) ? p->bin_blocks[k] : 0u;
-        B->c_slot_blk0[0] = B->r_slot_blk0[0] = 0;
-        for (int sl = 0; sl < AG_NSLOTS; sl++) {
-            B->c_slot_blk0[sl + 1] = B->c_slot_blk0[sl] + cb[ag_slot_bin[sl]];
+        B->r_slot_blk0[0] = 0;
+        for (int sl = 0; sl < AG_NSLOTS; sl++)
             B->r_slot_blk0[sl + 1] = B->r_slot_blk0[sl] + rb[ag_slot_bin[sl]];
-        }
     }
 
-    /* 5: work items, slice by slice, block-major */
+    /* 4: pass-1 items, slice by slice, block-major */
     B->n_items = emit_items(p, kstart, vlen, B->r_slot_blk0[AG_NSLOTS], NULL, NULL);
     B->item = (uint32_t *)zalloc((size_t)B->n_items * 2, sizeof(uint32_t));
     if (!B->item) {
@@ -494,12 +542,51 @@ static int build_block_plan(ag_plan *p, const char **err)
     }
     emit_items(p, kstart, vlen, B->r_slot_blk0[AG_NSLOTS], B->item, B->item_off);
 
+    /* 5: combine tiles: per block, the tile's rows' virtual rows are one contiguous run of
+     * the (slice, block) list; lpos places each partial at its row-major slot in the tile */
+    {
+        uint32_t tiles = 0;
+        B->c_n_items = emit_combine(p, NULL, &tiles, NULL, NULL);
+        B->c_n_tiles = tiles;
+        B->ct_row = (uint32_t *)zalloc((size_t)tiles * 2, sizeof(uint32_t));
+        B->c_item = (uint32_t *)zalloc((size_t)B->c_n_items * 2, sizeof(uint32_t));
+        B->ct_start = (uint32_t *)zalloc((size_t)tiles * nb, sizeof(uint32_t));
+        B->ct_pre = (uint32_t *)zalloc((size_t)tiles * (nb + 1), sizeof(uint32_t));
+        if (!B->ct_row || !B->c_item || !B->ct_start || !B->ct_pre) {
+            oom = 1;
+            goto done;
+        }
+        emit_combine(p, B->ct_row, &tiles, B->c_item, B->c_item_off);
+        #pragma omp parallel for schedule(dynamic, 64)
+        for (uint32_t t = 0; t < tiles; t++) {
+            const uint32_t r0 = B->ct_row[2 * t], r1 = B->ct_row[2 * t + 1];
+            uint32_t x = 0;
+            while (x + 1 < AG_NXCD && r0 >= B->slice_row[x + 1])
+                x++;
+            uint32_t run = 0;
+            for (uint32_t b = 0; b < nb; b++) {
+                const uint32_t lo = kstart[x * nb + b], hi = kstart[x * nb + b + 1];
+                const uint32_t s = lower_bound_u32(vrow, lo, hi, r0);
+                const uint32_t f = lower_bound_u32(vrow, s, hi, r1);
+                B->ct_start[(size_t)t * nb + b] = s;
+                B->ct_pre[(size_t)t * (nb + 1) + b] = run;
+                run += f - s;
+            }
+            B->ct_pre[(size_t)t * (nb + 1) + nb] = run;
+            const uint32_t p0 = B->coff[r0];
+            for (uint32_t r = r0; r < r1; r++)
+                for (uint32_t j = B->coff[r]; j < B->coff[r + 1]; j++)
+                    B->lpos[B->cidx[j]] = (uint16_t)(j - p0);
+        }
+    }
+
 done:
     free(ss);
     free(sw);
     free(vkey);
     free(vst);
     free(vln);
+    free(vrow);
     free(hist);
     free(kstart);
     free(vlen);

@@ -17,9 +17,10 @@ class BlockPlan(C.Structure):
                 ("nv", C.c_uint32), ("mb", C.c_uint32), ("slice_row", C.c_uint32 * 9),
                 ("voff", u32p), ("vsrc", u32p), ("vw", C.POINTER(C.c_float)),
                 ("n_items", C.c_uint32), ("item", u32p), ("item_off", C.c_uint32 * 9),
-                ("coff", u32p), ("cidx", u32p), ("c_bin_row", C.c_uint32 * 6),
-                ("c_n_tiles", C.c_uint32), ("c_tile_row", u32p), ("c_slot_blk0", C.c_uint32 * 6),
-                ("r_slot_blk0", C.c_uint32 * 6)]
+                ("coff", u32p), ("cidx", u32p), ("lpos", C.POINTER(C.c_uint16)),
+                ("c_n_tiles", C.c_uint32), ("ct_row", u32p), ("ct_start", u32p),
+                ("ct_pre", u32p), ("c_n_items", C.c_uint32), ("c_item", u32p),
+                ("c_item_off", C.c_uint32 * 9), ("r_slot_blk0", C.c_uint32 * 6)]
 
 
 class Plan(C.Structure):
@@ -157,29 +158,33 @@ def test_plan_rejects_bad_input():
 
 PIECE = 4096
 NXCD = 8
-ITEM_REG, ITEM_VTILE, ITEM_VG16, ITEM_VWAVE = 0, 1, 2, 3
+ITEM_REG, ITEM_VTILE, ITEM_CTILE, ITEM_CBIG = 0, 1, 2, 3
 
 
 def check_block_plan(g, p):
     """Every blocked row's in-edges appear exactly once among its virtual rows, grouped by
-    source block in stream order; virtual rows are ordered (slice, block, class, row, piece);
-    the items of each slice cover that slice's virtual rows exactly once, block-major, within
-    the kernels' limits; the combine schedule covers every blocked row once."""
+    source block in stream order; virtual rows are ordered (slice, block, row, piece); the
+    pass-1 tiles of each slice cover that slice's virtual rows exactly once, block-major,
+    within the tile limits; the combine tiles and big rows cover every blocked row once, each
+    tile's per-block runs hold exactly its rows' partials and lpos places them row-major."""
     B, prm, n = p.blk, p.params, g.n
     bits, nb, row0, nv = B.bits, B.nblocks, B.row0, B.nv
+    piece = min(PIECE, prm.tile_nnz)
     off = view(p.off, n + 1)
     srcd = view(p.src, p.m)
     br = list(p.bin_row)
     assert row0 == br[prm.blocked_min_bin] and B.mb == off[n] - off[row0]
-    assert nb == -(-n // (1 << bits)) and nb <= 4096
+    assert nb == -(-n // (1 << bits)) and nb <= 1024
     sl = list(B.slice_row)
     assert sl[0] == row0 and sl[-1] == n and sl == sorted(sl)
     voff, vsrc = view(B.voff, nv + 1), view(B.vsrc, B.mb)
     coff, cidx = view(B.coff, n + 1), view(B.cidx, nv)
+    lpos = np.ctypeslib.as_array(B.lpos, shape=(nv,)).copy()
     assert voff[0] == 0 and voff[-1] == B.mb
     vlen = np.diff(voff.astype(np.int64))
-    assert np.all(vlen >= 1) and np.all(vlen <= PIECE)
-    assert np.all(coff[:row0 + 1] == 0) and coff[n] == nv and np.all(np.diff(coff.astype(np.int64)) >= 0)
+    assert np.all(vlen >= 1) and np.all(vlen <= piece)
+    assert np.all(coff[:row0 + 1] == 0) and coff[n] == nv
+    assert np.all(np.diff(coff.astype(np.int64)) >= 0)
     assert sorted(cidx) == list(range(nv))
     vrow = np.empty(nv, np.int64)
     vblk = np.empty(nv, np.int64)
@@ -200,14 +205,13 @@ def check_block_plan(g, p):
         assert np.all(np.diff(bl) >= 0)
         for k in range(len(vs) - 1):
             if bl[k] == bl[k + 1]:
-                assert vlen[vs[k]] == PIECE
-    cls = np.where(vlen <= prm.thread_max_deg, 2, np.where(vlen <= prm.group16_max_deg, 1, 0))
+                assert vlen[vs[k]] == piece and vs[k + 1] == vs[k] + 1
     xs = np.searchsorted(np.array(sl[1:]), vrow, side="right")
-    key = (xs * nb + vblk) * 3 + cls
-    assert np.all(np.diff(key) >= 0), "virtual rows ordered by (slice, block, class)"
+    key = xs * nb + vblk
+    assert np.all(np.diff(key) >= 0), "virtual rows ordered by (slice, block)"
     same = np.diff(key) == 0
     assert np.all(np.diff(vrow)[same] >= 0), "then by row"
-    # items
+    # pass-1 items
     it = view(B.item, 2 * B.n_items).reshape(-1, 2).astype(np.int64)
     io = list(B.item_off)
     assert io[0] == 0 and io[-1] == B.n_items and io == sorted(io)
@@ -222,19 +226,14 @@ def check_block_plan(g, p):
                 assert cnt == 0 and first % NXCD == x
                 reg_seen.append(first)
                 continue
+            assert kind == ITEM_VTILE
             vs = np.arange(first, first + cnt)
             assert cnt >= 1 and vs[-1] < nv
             covered[vs] += 1
             assert np.all(xs[vs] == x) and np.unique(vblk[vs]).size == 1
             assert vblk[vs[0]] >= last_blk
             last_blk = vblk[vs[0]]
-            if kind == ITEM_VWAVE:
-                assert cnt <= 4 and np.all(cls[vs] == 0)
-            elif kind == ITEM_VG16:
-                assert cnt <= 16 and np.all(cls[vs] == 1)
-            else:
-                assert kind == ITEM_VTILE and np.all(cls[vs] == 2)
-                assert cnt <= prm.tile_rows and vlen[vs].sum() <= prm.tile_nnz
+            assert vlen[vs].sum() <= prm.tile_nnz
     assert np.all(covered == 1)
     assert sorted(reg_seen) == list(range(reg_total))
     # regular schedule of the unblocked rows: bins below blocked_min_bin only
@@ -242,19 +241,41 @@ def check_block_plan(g, p):
     rb = [blk_counts[b] if b < prm.blocked_min_bin else 0 for b in range(5)]
     order = [4, 3, 2, 1, 0]
     assert list(B.r_slot_blk0) == list(np.concatenate([[0], np.cumsum([rb[b] for b in order])]))
-    # combine schedule: blocked rows only, hub rows in the block bin, tiles within limits
-    cb = list(B.c_bin_row)
-    want = [max(br[k], row0) for k in range(4)]
-    if prm.wave_max_deg <= prm.tile_nnz:   # rows below the block bin combine in LDS tiles
-        want[1] = want[2] = want[3]
-    assert cb[:4] == want and cb[4] == n and cb[5] == n
-    assert np.all(np.diff(coff[cb[0]:cb[1] + 1].astype(np.int64)) <= prm.tile_nnz)
-    ct = view(B.c_tile_row, B.c_n_tiles + 1)
-    assert ct[0] == cb[0] and ct[-1] == cb[1]
-    assert np.all(np.diff(ct) <= prm.tile_rows)
-    assert np.all(coff[ct[1:]] - coff[ct[:-1]] <= prm.tile_nnz)
-    cnts = [B.c_n_tiles, -(-(cb[2] - cb[1]) // 16), -(-(cb[3] - cb[2]) // 4), cb[4] - cb[3], 0]
-    assert list(B.c_slot_blk0) == list(np.concatenate([[0], np.cumsum([cnts[b] for b in order])]))
+    # combine items: tiles and big rows cover every blocked row once, inside their slice
+    ct = view(B.ct_row, 2 * B.c_n_tiles).reshape(-1, 2).astype(np.int64)
+    cs = view(B.ct_start, B.c_n_tiles * nb).reshape(-1, nb).astype(np.int64)
+    cp = view(B.ct_pre, B.c_n_tiles * (nb + 1)).reshape(-1, nb + 1).astype(np.int64)
+    ci = view(B.c_item, 2 * B.c_n_items).reshape(-1, 2).astype(np.int64)
+    co = list(B.c_item_off)
+    assert co[0] == 0 and co[-1] == B.c_n_items and co == sorted(co)
+    rows_seen = np.zeros(n, np.int64)
+    tiles_seen = []
+    for x in range(NXCD):
+        for first, kc in ci[co[x]:co[x + 1]]:
+            if kc & 15 == ITEM_CBIG:
+                assert sl[x] <= first < sl[x + 1]
+                assert coff[first + 1] - coff[first] > prm.tile_nnz
+                rows_seen[first] += 1
+                continue
+            assert kc & 15 == ITEM_CTILE
+            tiles_seen.append(first)
+            r0, r1 = ct[first]
+            assert sl[x] <= r0 < r1 <= sl[x + 1] and r1 - r0 <= 4096
+            rows_seen[r0:r1] += 1
+            p0, npart = coff[r0], coff[r1] - coff[r0]
+            assert npart <= prm.tile_nnz and cp[first, 0] == 0 and cp[first, nb] == npart
+            got = []
+            for b in range(nb):
+                vs = np.arange(cs[first, b], cs[first, b] + cp[first, b + 1] - cp[first, b])
+                assert np.all(vblk[vs] == b) and np.all((vrow[vs] >= r0) & (vrow[vs] < r1))
+                got.extend(vs.tolist())
+            want = cidx[coff[r0]:coff[r1]]
+            assert sorted(got) == sorted(want.tolist())
+            # lpos: the partial's slot relative to the tile, so that row r's partials are
+            # [coff[r] - p0, coff[r + 1] - p0) in block order
+            np.testing.assert_array_equal(lpos[want], np.arange(npart))
+    assert sorted(tiles_seen) == list(range(B.c_n_tiles))
+    assert np.all(rows_seen[row0:] == 1) and np.all(rows_seen[:row0] == 0)
 
 
 @pytest.mark.parametrize("min_bin", [0, 1, 3])
@@ -273,9 +294,9 @@ def test_block_plan_invariants(min_bin, small):
         assert p.blk.bits == 8 and p.blk.nblocks == 16 and p.blk.nv > 0
         check_plan(g, p)
         check_block_plan(g, p)
-        kinds = set(view(p.blk.item, 2 * p.blk.n_items)[1::2] & 15)
-        if small:
-            assert {ITEM_VTILE, ITEM_VG16, ITEM_VWAVE} <= kinds
+        if small:   # tiny tiles: some rows have more partials than a combine tile holds
+            kinds = set(view(p.blk.c_item, 2 * p.blk.c_n_items)[1::2] & 15)
+            assert {ITEM_CTILE, ITEM_CBIG} <= kinds
     finally:
         ag.lib().ag_plan_free(C.byref(p))
 
@@ -292,6 +313,7 @@ def test_block_plan_long_pieces():
         check_block_plan(g, p)
         lens = np.diff(view(p.blk.voff, p.blk.nv + 1).astype(np.int64))
         assert sorted(lens.tolist())[-3:] == [1808, 4096, 4096]
+        assert p.blk.c_n_tiles >= 1
     finally:
         ag.lib().ag_plan_free(C.byref(p))
 
@@ -311,8 +333,8 @@ def test_block_plan_rejects_bad_bits():
     g = ag.Graph.rmat(6)
     st, p, err = build(g, ag.plan_params(src_block_bits=7))
     assert st == 1 and b"src_block_bits" in err
-    n = (1 << 20) + 1   # 4097 blocks of 256 sources
+    n = (1 << 18) + 1   # 1025 blocks of 256 sources
     st, p, err = build(None, n=n, outdeg=np.zeros(n, np.uint32), idx=np.zeros(n, np.uint32),
                        deg=np.zeros(n, np.uint32), src=np.zeros(0, np.uint32),
                        params=ag.plan_params(src_block_bits=8))
-    assert st == 1 and b"4096" in err
+    assert st == 1 and b"1024" in err
