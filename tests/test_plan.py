@@ -34,7 +34,7 @@ class Plan(C.Structure):
                 ("total_blocks", C.c_uint32), ("blk", BlockPlan)]
 
 
-def build(g, params=None, n=None, outdeg=None, idx=None, deg=None, src=None):
+def build(g, params=None, n=None, outdeg=None, idx=None, deg=None, src=None, weights=None):
     L = ag.lib()
     L.ag_plan_build.restype = C.c_int
     L.ag_plan_build.argtypes = [C.c_uint32, C.c_uint32, u32p, u32p, u32p, u32p,
@@ -46,7 +46,9 @@ def build(g, params=None, n=None, outdeg=None, idx=None, deg=None, src=None):
     arrs = [np.ascontiguousarray(a, np.uint32) for a in (outdeg, idx, deg, src)]
     p = Plan()
     err = C.c_char_p()
-    st = L.ag_plan_build(n, arrs[3].size, *[a.ctypes.data_as(u32p) for a in arrs], None,
+    w = None if weights is None else np.ascontiguousarray(weights, np.float32)
+    st = L.ag_plan_build(n, arrs[3].size, *[a.ctypes.data_as(u32p) for a in arrs],
+                         None if w is None else w.ctypes.data_as(C.POINTER(C.c_float)),
                          C.byref(params) if params else None, C.byref(p), C.byref(err))
     return st, p, err.value
 
@@ -338,3 +340,33 @@ def test_block_plan_rejects_bad_bits():
                        deg=np.zeros(n, np.uint32), src=np.zeros(0, np.uint32),
                        params=ag.plan_params(src_block_bits=8))
     assert st == 1 and b"1024" in err
+
+
+def test_block_plan_weights_follow_sources():
+    """Weighted graph: the virtual-row weights are the row's weights regrouped exactly like its
+    sources (SpMV reads vw[j] beside vsrc[j])."""
+    g = ag.Graph.rmat(11)
+    w = (np.arange(g.m, dtype=np.float64) + 1).astype(np.float32)   # unique per inverse edge
+    st, p, err = build(g, ag.plan_params(src_block_bits=8, blocked_min_bin=0), weights=w)
+    assert st == 0, err
+    try:
+        B, n = p.blk, g.n
+        d2o = view(p.d2o, n)
+        voff, vsrc = view(B.voff, B.nv + 1), view(B.vsrc, B.mb)
+        vw = np.ctypeslib.as_array(B.vw, shape=(B.mb,)).copy()
+        coff, cidx = view(B.coff, n + 1), view(B.cidx, B.nv)
+        for r in range(B.row0, n):
+            v = d2o[r]
+            lo, hi = g.inv_idx[v], g.inv_idx[v] + g.in_degree[v]
+            if hi == lo:
+                assert coff[r + 1] == coff[r]
+                continue
+            osrc, ow = g.inv_src[lo:hi], w[lo:hi]
+            dsrc = view(p.o2d, n)[osrc]
+            order = np.argsort(dsrc >> B.bits, kind="stable")
+            got_s = np.concatenate([vsrc[voff[q]:voff[q + 1]] for q in cidx[coff[r]:coff[r + 1]]])
+            got_w = np.concatenate([vw[voff[q]:voff[q + 1]] for q in cidx[coff[r]:coff[r + 1]]])
+            np.testing.assert_array_equal(got_s, dsrc[order])
+            np.testing.assert_array_equal(got_w, ow[order])
+    finally:
+        ag.lib().ag_plan_free(C.byref(p))
