@@ -176,6 +176,38 @@ def test_bruteforce_dense_power_iteration(seed):
     np.testing.assert_allclose(f64, _dense_f64(n, edges, 400, 1.0) / n, rtol=1e-9)
 
 
+@pytest.mark.parametrize("seed", range(6))
+def test_ppr_f64_matches_linear_solve(seed):
+    """Personalised PageRank fixed point in closed form: r = (I - 0.85 P)^-1 p, with p the
+    restart vector (0.15 n on the seed vertex, or 0.15 everywhere for the uniform restart) and
+    P the column-stochastic transition matrix with dangling columns zero (pageRank.c:1026-1033).
+    The oracle iterates; the solve does not."""
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.integers(2, 48))
+    m = int(rng.integers(0, 4 * n))
+    edges = sorted({(int(rng.integers(n)), int(rng.integers(n))) for _ in range(m)})
+    edges = [(s, d) for s, d in edges if s != d]
+    A = np.zeros((n, n))
+    od = np.zeros(n)
+    for s, d in edges:
+        A[d, s] += 1.0
+        od[s] += 1
+    P = A * np.where(od > 0, 1.0 / np.maximum(od, 1), 0.0)[None, :]
+    M = np.eye(n) - 0.85 * P
+    odeg, idx, deg, src = _csr(n, edges)
+    for sv in [0xFFFFFFFF, int(rng.integers(n)), n - 1]:
+        p = np.full(n, 0.15) if sv == 0xFFFFFFFF else 0.15 * n * (np.arange(n) == sv)
+        want = np.linalg.solve(M, p) / n
+        got = oracle.ppr_f64(odeg, idx, deg, src, sv, tol=1e-15, max_iterations=20000)
+        # atol: what the iteration leaves of its all-ones start where the exact answer is 0
+        np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-12)
+    # the uniform restart is plain PageRank
+    np.testing.assert_allclose(oracle.ppr_f64(odeg, idx, deg, src, 0xFFFFFFFF, tol=1e-15,
+                                              max_iterations=20000),
+                               oracle.pagerank_f64(odeg, idx, deg, src, tol=1e-15,
+                                                   max_iterations=20000), rtol=1e-12)
+
+
 def test_openmp_matches_sequential():
     rng = np.random.default_rng(3)
     n = 3000
