@@ -242,9 +242,9 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the graph shards "
-              "by destination rows with an all-gather of contributions per iteration; see "
-              "DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (a sharded "
+              "version would split destination rows across GPUs and all-gather the "
+              "contribution vector every iteration). Run with --gpus 1.", file=sys.stderr)
         return 2
 
     import accelgraph_amd as ag

@@ -1,9 +1,9 @@
 """CPU oracle: ctypes wrapper of oracle/liboracle.so (a C restatement of the reference's
 PageRank-pull / SpMV-pull path; see ag_oracle.c for the file:line it follows).
 
-TEST INFRASTRUCTURE. Imported only by tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg, and only as the checker / CPU baseline. The product (accelgraph_amd) never
-imports it.
+TEST INFRASTRUCTURE. Imported only by tests/, __graft_entry__.smoke() and bench.py's CPU leg
+(the CPU baseline timings and the FP64 reference its error figures are measured against), and
+only as the checker / CPU baseline. The product (accelgraph_amd) never imports it.
 """
 from __future__ import annotations
 

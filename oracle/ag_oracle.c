@@ -1,8 +1,9 @@
 /*
  * ag_oracle.c -- CPU restatement of the reference's PageRank-pull / SpMV-pull path.
  *
- * TEST INFRASTRUCTURE ONLY. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
- * leg load this library, and only as the checker or the CPU baseline; the product library
+ * TEST INFRASTRUCTURE ONLY. Only tests/, __graft_entry__.smoke() and bench.py's CPU leg (the
+ * CPU baseline timings and the FP64 reference its error figures are measured against) load
+ * this library, and only as the checker or the CPU baseline; the product library
  * (accelgraph_amd/libaccelgraph.so) never links or calls it.
  *
  * Every function follows the reference step by step:

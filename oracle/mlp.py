@@ -1,4 +1,4 @@
-"""FP64 restatement of the warm-start MLP (TEST INFRASTRUCTURE: tests/ and bench only).
+"""FP64 restatement of the warm-start MLP (TEST INFRASTRUCTURE: used by tests/ only).
 
 Same network as accelgraph_amd/csrc/ag_mlp.hip: standardised 5 inputs (padded to 16) -> 64 ->
 64 -> 1 with ReLU, mean-squared loss on the standardised target. `forward_backward(...,
