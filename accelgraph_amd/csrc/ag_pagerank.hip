@@ -1,4 +1,4 @@
-// ag_pagerank.hip -- PageRank pull on the GPU: one fused launch per iteration.
+// ag_pagerank.hip -- PageRank pull on the GPU.
 //
 // Reference host loop (02_capi_graph/src/algorithms/capi/pageRank.c:942-1109), per iteration:
 //   contrib[v] = out_degree[v] ? rank[v] / out_degree[v] : 0          (:1026-1033)
@@ -8,12 +8,17 @@
 //   stop when active == 0                                              (:1072-1074)
 // and finally rank[v] /= N with an FP64 sum (:1079-1085).
 //
-// Here one launch does the gather, the damping update, the L1/active reduction and the next
-// iteration's contribution for every vertex; `next` never exists in memory. Contributions
-// ping-pong between two buffers by iteration parity. The launch's last workgroup (ticket
-// counter) reduces the per-workgroup partials in a fixed order, records the iteration's
-// error/active count, and sets a done flag that turns every later launch into a no-op, so
-// many iterations can be queued without a host round trip per iteration.
+// One iteration is either
+//   regular:        pr_*_kernel (gather + update for every row) then pr_*_finish, or
+//   source-blocked: pr_*_blk_kernel (per-block partial sums, XCD-sliced) then pr_*_cmb_kernel
+//                   (partials added per row in block order + update) then pr_*_finish.
+// The update is fused into the launch that completes a row's sum: the damping update, the
+// per-workgroup L1 / active partials and the next iteration's contribution; `next` never
+// exists in memory. Contributions ping-pong between two buffers by iteration parity. The
+// one-workgroup finish kernel adds the per-workgroup partials in a fixed order, records the
+// iteration's error and active count, decides the 16-bit -> FP32 refinement and sets a done
+// flag that turns every later launch into a no-op, so many iterations can be queued without
+// a host round trip per iteration.
 //
 // Fixed-point variants (pageRank.c:1276-1990) gather u64/u32/u16/u8 contributions into an
 // exact u64 sum (wraps modulo 2^64) and convert with the formats of fixedPoint.h.

@@ -19,10 +19,15 @@
 //           and rounding as the reference's sequential sum
 // The main kernel runs a persistent grid (compute units x resident workgroups); workgroup b
 // takes items b, b + grid, b + 2*grid, ... so the heavy items spread over the whole chip first.
-// No atomics: hub partials and per-workgroup reductions go to plain arrays that a one-workgroup
-// finish kernel reads after the launch boundary, in a fixed order, so every result is
-// deterministic from run to run. Lane partials use four independent accumulators (memory-level
-// parallelism) combined in a fixed tree.
+// No atomics on global memory: hub partials and per-workgroup reductions go to plain arrays
+// that a one-workgroup finish kernel reads after the launch boundary, in a fixed order, so
+// every result is deterministic from run to run. Lane partials use four independent
+// accumulators (memory-level parallelism) combined in a fixed tree.
+//
+// Source-blocked graphs (the second half of this file) replace the gather by two passes over
+// the plan's virtual rows -- (row, source block) pieces of in-edges -- so that the gathers of
+// each XCD stay inside one L2-sized block of the contribution vector at a time: pass 1 writes
+// one partial per piece, pass 2 adds each row's partials in block order and finishes the row.
 #pragma once
 
 #include "ag_internal.h"

@@ -293,6 +293,21 @@ def main() -> int:
     log(f"{K} steps: {ms_per_step:.3f} ms/step wall, kernel avg {kern_ms:.3f} ms over "
         f"{launches} launches, {gteps:.2f} GTEPS, {achieved:.0f} GB/s algorithmic")
 
+    # Q16.16 fixed-point iteration (the reference's pushpull 9 path: u32 contributions, exact
+    # u64 sums), same graph, same algorithmic bytes
+    fsess = dg.session(kind=ag.KIND_FIXED32, max_iterations=3 + 10, stop_rule=ag.STOP_NONE,
+                       init=ag.INIT_UNIFORM)
+    fsess.step(3)
+    dev.sync()
+    fsess.step(10)
+    dev.sync()
+    fix_ms, _ = fsess.last_step_kernel_ms()
+    fsess.end()
+    fix_gbs = algorithmic_bytes(n, m, "f32") / (fix_ms * 1e-3) / 1e9
+    fixed32 = {"ms_per_iteration": round(fix_ms, 4), "gteps": round(m / (fix_ms * 1e-3) / 1e9, 2),
+               "gb_per_s": round(fix_gbs, 1), "roofline_frac": round(fix_gbs / HBM_PEAK_GBS, 4)}
+    log(f"fixed-32 iteration: {fixed32}")
+
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     wl = f"pagerank_pull_{args.storage}_{args.graph}{args.scale}"
@@ -323,6 +338,7 @@ def main() -> int:
                      "measured_copy_gb_per_s": round(copy_gbs, 1)},
     }
 
+    out["fixed32_iteration"] = fixed32
     if not args.no_ttl:
         t0 = time.perf_counter()
         f64 = None
