@@ -14,7 +14,10 @@
 #include "ag_internal.h"
 #include "ag_pull_kernel.h"
 
-#define AG_BATCH_WG 512
+// 1024 lanes (16 wavefronts) per graph: the largest graphs of a batch set the makespan, and
+// their iterations are latency-bound chains of gathers that more wavefronts overlap (512 lanes
+// measured 1.9x slower on 512 graphs of scale 10-16)
+#define AG_BATCH_WG 1024
 
 struct BatchGraph {
     uint32_t v0, n, e0, m, ob, h0, nh, pad;
@@ -359,7 +362,7 @@ extern "C" int ag_batch_upload(ag_device *dev, uint32_t G, const uint32_t *vert_
 static int batch_enqueue(ag_batch *b)
 {
     hipStream_t s = b->dev->stream;
-    const uint32_t per_cu = 4;
+    static const uint32_t per_cu = ag_resident_per_cu(batch_kernel, AG_BATCH_WG);
     uint32_t grid = b->dev->num_cus * per_cu;
     if (grid > b->G)
         grid = b->G;

@@ -159,12 +159,12 @@ BlockGeom ag_make_block_geom(const ag_dgraph *g);
 // defaults to as many as fit, up to 8.
 uint32_t ag_blocked_grid(const ag_dgraph *g, uint32_t resident_per_cu, uint32_t default_per_cu);
 
-// Workgroups of AG_WG lanes of `kernel` that fit on one CU (registers and LDS).
+// Workgroups of `lanes` lanes of `kernel` that fit on one CU (registers and LDS).
 template <typename K>
-static inline uint32_t ag_resident_per_cu(K kernel)
+static inline uint32_t ag_resident_per_cu(K kernel, uint32_t lanes = AG_WG)
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, AG_WG, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, (int)lanes, 0) != hipSuccess ||
         nb < 1)
         return 1u;
     return (uint32_t)nb;
