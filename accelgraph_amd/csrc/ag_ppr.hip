@@ -88,37 +88,53 @@ __device__ __forceinline__ uint32_t pack16(float x, float y, bool f16)
     return to_bf16(x) | (to_bf16(y) << 16);
 }
 
+// Contribution lines in flight per half-wavefront: the 64-column rows are coalesced 128/256-byte
+// reads from HBM, so unlike the scalar gathers this kernel gains from more requests in flight.
+#define PPR_UNROLL 4
+
 // Sum of this lane's two columns over in-edges lo + start, lo + start + step, ... < hi.
+// PPR_UNROLL independent accumulators, combined in a fixed tree.
 __device__ __forceinline__ float2 ppr_edges(const PullGeom &G, const PprArgs &A, const Mode &m,
                                             uint32_t lo, uint32_t hi, uint32_t start,
                                             uint32_t step, uint32_t h)
 {
-    float2 a0 = make_float2(0.f, 0.f), a1 = a0, a2 = a0, a3 = a0;
+    float2 acc[PPR_UNROLL];
+#pragma unroll
+    for (int u = 0; u < PPR_UNROLL; u++)
+        acc[u] = make_float2(0.f, 0.f);
     const bool f16 = A.storage == AG_STORE_F16;
     const uint32_t *c16 = (const uint32_t *)A.c16[m.rd];
     const float2 *c32 = (const float2 *)A.c32[m.rd];
-    for (uint32_t j = lo + start; j < hi; j += 4 * step) {
-        uint32_t s[4];
+    for (uint32_t j = lo + start; j < hi; j += PPR_UNROLL * step) {
+        uint32_t s[PPR_UNROLL];
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
+        for (int u = 0; u < PPR_UNROLL; u++) {
             const uint32_t jj = j + u * step;
             s[u] = jj < hi ? __builtin_nontemporal_load(&G.src[jj]) : 0u;
         }
-        float2 v[4];
+        float2 v[PPR_UNROLL];
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
+        for (int u = 0; u < PPR_UNROLL; u++) {
             const uint32_t jj = j + u * step;
             v[u] = make_float2(0.f, 0.f);
             if (jj < hi)
                 v[u] = m.r16 ? unpack16(c16[(size_t)s[u] * PPR_HW + h], f16)
                              : c32[(size_t)s[u] * PPR_HW + h];
         }
-        a0.x += v[0].x; a0.y += v[0].y;
-        a1.x += v[1].x; a1.y += v[1].y;
-        a2.x += v[2].x; a2.y += v[2].y;
-        a3.x += v[3].x; a3.y += v[3].y;
+#pragma unroll
+        for (int u = 0; u < PPR_UNROLL; u++) {
+            acc[u].x += v[u].x;
+            acc[u].y += v[u].y;
+        }
     }
-    return make_float2((a0.x + a1.x) + (a2.x + a3.x), (a0.y + a1.y) + (a2.y + a3.y));
+#pragma unroll
+    for (int w = PPR_UNROLL / 2; w >= 1; w >>= 1)
+#pragma unroll
+        for (int u = 0; u < w; u++) {
+            acc[u].x += acc[u + w].x;
+            acc[u].y += acc[u + w].y;
+        }
+    return acc[0];
 }
 
 // Damping update of columns 2h, 2h+1 of row v; error accumulated per column.
