@@ -181,6 +181,20 @@ def spmv_section(ag, dev, scale: int, reps: int, cpu: bool) -> dict:
                      "algorithmic_bytes": nb, "gb_per_s": round(gbs, 1),
                      "roofline_frac": round(gbs / HBM_PEAK_GBS, 4)}
     y, _ = dg.spmv(x)
+    # fixed-32 PageRank iteration on the same graph (Q16.16 contributions, exact u64 sums)
+    fs = dg.session(kind=ag.KIND_FIXED32, max_iterations=3 + 20, stop_rule=ag.STOP_NONE,
+                    init=ag.INIT_UNIFORM)
+    fs.step(3)
+    dev.sync()
+    fs.step(20)
+    dev.sync()
+    fms, _ = fs.last_step_kernel_ms()
+    fs.end()
+    fgbs = algorithmic_bytes(g.n, g.m, "f32") / (fms * 1e-3) / 1e9
+    out["pagerank_fixed32_iteration"] = {"ms": round(fms, 4),
+                                         "gteps": round(g.m / (fms * 1e-3) / 1e9, 2),
+                                         "gb_per_s": round(fgbs, 1),
+                                         "roofline_frac": round(fgbs / HBM_PEAK_GBS, 4)}
     dg.close()
     if cpu:   # CPU leg: the oracle's OpenMP restatement, also the FP32 result check
         import oracle
@@ -191,6 +205,30 @@ def spmv_section(ag, dev, scale: int, reps: int, cpu: bool) -> dict:
         out["cpu_baseline"] = {"ms": round(dt * 1e3, 2), "cores": threads, "kind": "port",
                                "gteps": round(g.m / dt / 1e9, 3)}
         out["f32"]["max_rel_err_vs_cpu"] = float(np.max(np.abs(y - ref) / np.maximum(np.abs(ref), 1e-30)))
+    return out
+
+
+def config1_section(ag, dev, cpu: bool) -> dict:
+    """RMAT-16, damping 0.85, uniform 1/N start, stop at L1 <= 1e-4 or 100 iterations: time to
+    the stop (graph resident) and the error against the FP64 oracle (CPU leg)."""
+    import numpy as np
+    g = ag.Graph.rmat(16)
+    dg = dev.upload(g)
+    kw = dict(max_iterations=100, stop_rule=ag.STOP_L1, l1_tolerance=1e-4, init=ag.INIT_UNIFORM,
+              history=True)
+    dg.pagerank(**kw)                                    # warm the code paths
+    run = dg.pagerank(**kw)
+    dg.close()
+    out = {"graph": f"RMAT scale 16 (n={g.n}, m={g.m})", "iterations": run.executed,
+           "final_l1": run.error_total, "ms": round(float(run.hist_ms.sum()), 4),
+           "ms_per_iteration": round(float(run.hist_ms.mean()), 4)}
+    if cpu:
+        import oracle
+        # FP64 after the same number of iterations from the same start: the FP32 rounding error
+        f64 = oracle.pagerank_f64(g.out_degree, g.inv_idx, g.in_degree, g.inv_src, tol=0.0,
+                                  max_iterations=run.executed,
+                                  threads=len(os.sched_getaffinity(0)))
+        out["max_abs_err_vs_f64_same_iterations"] = float(np.abs(run.ranks - f64).max())
     return out
 
 
@@ -370,6 +408,10 @@ def main() -> int:
         t0 = time.perf_counter()
         out["spmv"] = spmv_section(ag, dev, args.spmv_scale, 20, not args.no_cpu)
         log(f"spmv: {out['spmv']} ({time.perf_counter() - t0:.1f} s)")
+    if not args.no_spmv:
+        t0 = time.perf_counter()
+        out["config1_rmat16"] = config1_section(ag, dev, not args.no_cpu)
+        log(f"config 1: {out['config1_rmat16']} ({time.perf_counter() - t0:.1f} s)")
     if not args.no_er and args.graph == "rmat":
         t0 = time.perf_counter()
         out["er_pagerank"] = er_section(ag, dev, args.scale, 20)
