@@ -92,6 +92,39 @@ class Graph:
                                               _ptr(d, C.c_uint32), _ptr(w, C.c_float),
                                               int(simplify)))
 
+    # construction on the GPU: the same arrays as the host builders above ----
+    @classmethod
+    def rmat_gpu(cls, dev: "Device", scale: int, edge_factor: int = 16, a: float = RMAT_A,
+                 b: float = RMAT_B, c: float = RMAT_C, seed: int = DEFAULT_SEED,
+                 simplify: bool = True) -> "Graph":
+        out = C.POINTER(_abi.GraphCSR)()
+        _check(lib().ag_graph_csr_generate_rmat(dev.handle, scale, edge_factor, a, b, c, seed,
+                                                int(simplify), C.byref(out)),
+               "ag_graph_csr_generate_rmat")
+        return cls(out)
+
+    @classmethod
+    def erdos_renyi_gpu(cls, dev: "Device", n: int, m: int, seed: int = DEFAULT_SEED,
+                        simplify: bool = True) -> "Graph":
+        out = C.POINTER(_abi.GraphCSR)()
+        _check(lib().ag_graph_csr_generate_er(dev.handle, n, m, seed, int(simplify),
+                                              C.byref(out)), "ag_graph_csr_generate_er")
+        return cls(out)
+
+    @classmethod
+    def from_edges_gpu(cls, dev: "Device", src, dst, n: int = 0, weights=None,
+                       simplify: bool = True) -> "Graph":
+        s = np.ascontiguousarray(src, dtype=np.uint32)
+        d = np.ascontiguousarray(dst, dtype=np.uint32)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+        if s.shape != d.shape:
+            raise ValueError("src and dst lengths differ")
+        out = C.POINTER(_abi.GraphCSR)()
+        _check(lib().ag_graph_csr_build(dev.handle, n, s.size, _ptr(s, C.c_uint32),
+                                        _ptr(d, C.c_uint32), _ptr(w, C.c_float), int(simplify),
+                                        C.byref(out)), "ag_graph_csr_build")
+        return cls(out)
+
     @classmethod
     def load_binary(cls, path: str, simplify: bool = True) -> "Graph":
         return cls(lib().graphCSRLoadBinaryEdgeList(path.encode(), int(simplify)))

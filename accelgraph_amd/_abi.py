@@ -187,6 +187,14 @@ SIGNATURES = {
     "graphCSRLoadBinaryEdgeList": (P(GraphCSR), [C.c_char_p, C.c_int]),
     "graphCSRSetWeights": (C.c_int, [P(GraphCSR), P(f32)]),
     "graphCSRFree": (None, [P(GraphCSR)]),
+    "graphCSRAlloc": (P(GraphCSR), [u32, u32, C.c_int]),
+    "ag_generate_rmat_edges": (C.c_int, [VP, u32, u64, f64, f64, f64, u32, P(u32), P(u32)]),
+    "ag_generate_er_edges": (C.c_int, [VP, u32, u64, u32, P(u32), P(u32)]),
+    "ag_graph_csr_build": (C.c_int, [VP, u32, u64, P(u32), P(u32), P(f32), C.c_int,
+                                     P(P(GraphCSR))]),
+    "ag_graph_csr_generate_rmat": (C.c_int, [VP, u32, u32, f64, f64, f64, u32, C.c_int,
+                                             P(P(GraphCSR))]),
+    "ag_graph_csr_generate_er": (C.c_int, [VP, u32, u64, u32, C.c_int, P(P(GraphCSR))]),
     # mt19937.h
     "initializeMersenneState": (None, [P(MT19937State), u32]),
     "generateRandInt": (u32, [P(MT19937State)]),

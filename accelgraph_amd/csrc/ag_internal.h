@@ -146,6 +146,10 @@ int ag_wait_stream(ag_device *dev);
             return ag_hip_fail(ag_e_, #call, __FILE__, __LINE__);      \
     } while (0)
 
+// graph_csr.c: parallel first touch of a new graph's host arrays (before bulk copies into them)
+struct GraphCSR;
+extern "C" void graphCSRPrefault(struct GraphCSR *g);
+
 PullGeom ag_make_geom(const ag_dgraph *g);
 // Regular schedule of the rows a source-blocked graph leaves unblocked (AG_ITEM_REG items).
 PullGeom ag_make_unblocked_geom(const ag_dgraph *g);

@@ -91,7 +91,7 @@ static int bucket_sort(uint32_t n, uint64_t m, const uint32_t *key, const uint32
     return 0;
 }
 
-static struct GraphCSR *alloc_graph(uint32_t n, uint32_t m, int weighted)
+struct GraphCSR *graphCSRAlloc(uint32_t n, uint32_t m, int weighted)
 {
     struct GraphCSR *g = (struct GraphCSR *)calloc(1, sizeof(struct GraphCSR));
     if (!g)
@@ -136,6 +136,40 @@ static struct GraphCSR *alloc_graph(uint32_t n, uint32_t m, int weighted)
         }
     }
     return g;
+}
+
+/* Touches every page of a freshly allocated graph's arrays from all threads (writing the
+ * zeros they already hold), so that later bulk writes -- device-to-host copies into pageable
+ * memory run on one thread -- do not take their page faults one page at a time. */
+void graphCSRPrefault(struct GraphCSR *g)
+{
+    if (!g)
+        return;
+    const size_t n = g->num_vertices, m = g->num_edges;
+    struct {
+        void *p;
+        size_t bytes;
+    } a[12];
+    int k = 0;
+    struct Vertex *vs[2] = {g->vertices, g->inverse_vertices};
+    for (int i = 0; i < 2; i++) {
+        a[k].p = vs[i]->out_degree; a[k++].bytes = n * 4;
+        a[k].p = vs[i]->in_degree;  a[k++].bytes = n * 4;
+        a[k].p = vs[i]->edges_idx;  a[k++].bytes = n * 4;
+    }
+    struct EdgeList *es[2] = {g->sorted_edges_array, g->inverse_sorted_edges_array};
+    for (int i = 0; i < 2; i++) {
+        a[k].p = es[i]->edges_array_src;    a[k++].bytes = m * 4;
+        a[k].p = es[i]->edges_array_dest;   a[k++].bytes = m * 4;
+        a[k].p = es[i]->edges_array_weight; a[k++].bytes = es[i]->edges_array_weight ? m * 4 : 0;
+    }
+    for (int i = 0; i < k; i++) {
+        volatile char *c = (volatile char *)a[i].p;
+        const size_t bytes = a[i].bytes;
+        #pragma omp parallel for schedule(static)
+        for (size_t q = 0; q < bytes; q += 4096)
+            c[q] = 0;
+    }
 }
 
 void graphCSRFree(struct GraphCSR *g)
@@ -223,7 +257,7 @@ struct GraphCSR *graphCSRFromEdgeList(uint32_t n, uint64_t m, const uint32_t *sr
     for (uint32_t v = 0; v < n; v++)
         total += fdeg[v];
 
-    struct GraphCSR *g = alloc_graph(n, (uint32_t)total, weight != NULL);
+    struct GraphCSR *g = graphCSRAlloc(n, (uint32_t)total, weight != NULL);
     if (!g) {
         free(foff);
         free(fvals);

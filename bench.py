@@ -300,8 +300,25 @@ def main() -> int:
     n, m = g.n, g.m
     log(f"graph built: n={n} m={m} in {time.perf_counter() - t0:.1f} s")
 
+    cpu_build_s = time.perf_counter() - t0
     dev = ag.Device(0)
     copy_gbs = dev.copy_bandwidth(1 << 30, 10)
+    # the same graph built on the GPU (generator + CSR + inverse CSR), checked array for array
+    construction = None
+    if args.graph == "rmat":
+        import numpy as np
+        t1 = time.perf_counter()
+        gg = ag.Graph.rmat_gpu(dev, args.scale, args.edge_factor)
+        gpu_build_s = time.perf_counter() - t1
+        same = (gg.n, gg.m) == (n, m) and all(
+            np.array_equal(getattr(gg, k), getattr(g, k))
+            for k in ("out_degree", "edges_idx", "fwd_dest", "in_degree", "inv_idx", "inv_src"))
+        gg.close()
+        construction = {"host_s": round(cpu_build_s, 3), "gpu_s": round(gpu_build_s, 3),
+                        "speedup": round(cpu_build_s / max(gpu_build_s, 1e-9), 2),
+                        "identical_arrays": bool(same),
+                        "host_cores": len(os.sched_getaffinity(0))}
+        log(f"graph construction: {construction}")
     log(f"device copy bandwidth {copy_gbs:.0f} GB/s (1 GiB, read + write)")
     t0 = time.perf_counter()
     dg = dev.upload(g)
@@ -377,6 +394,8 @@ def main() -> int:
     }
 
     out["fixed32_iteration"] = fixed32
+    if construction:
+        out["graph_construction"] = construction
     if not args.no_ttl:
         t0 = time.perf_counter()
         f64 = None

@@ -306,6 +306,32 @@ typedef struct ag_warm_result {
  * (else continues from the uniform run). Those iterations count in the session. */
 int ag_pr_warm_start(ag_pr_session *s, ag_mlp *m, ag_warm_result *res);
 
+/* ---------------------------------------------------------------- graph construction */
+
+/* The step before the path, on the GPU, with the same results as the host builders of
+ * include/graphCSR.h array for array (the reference takes its graphs from OpenGraph's
+ * builder, absent; options at 02_capi_graph/src/main/accel-graph.c:71-72,143-145):
+ * the chunked MT19937 RMAT / Erdos-Renyi edge streams of graphGenerateRMATEdges /
+ * graphCSRGenerateER, and the CSR + inverse CSR of graphCSRFromEdgeList (rows by source with
+ * destinations ascending, self-loops and duplicates dropped when `simplify`, the first
+ * occurrence keeping its weight; inverse rows by destination with sources ascending). The
+ * struct GraphCSR is allocated on the host (release with graphCSRFree). */
+struct GraphCSR;
+int ag_generate_rmat_edges(ag_device *dev, uint32_t scale, uint64_t num_edges, double a,
+                           double b, double c, uint32_t seed, uint32_t *src_out,
+                           uint32_t *dst_out);
+int ag_generate_er_edges(ag_device *dev, uint32_t num_vertices, uint64_t num_edges,
+                         uint32_t seed, uint32_t *src_out, uint32_t *dst_out);
+/* num_vertices = 0: 1 + the largest id. AG_ERR_INVALID for an id >= num_vertices. */
+int ag_graph_csr_build(ag_device *dev, uint32_t num_vertices, uint64_t num_edges,
+                       const uint32_t *src, const uint32_t *dst, const float *weight,
+                       int simplify, struct GraphCSR **out);
+int ag_graph_csr_generate_rmat(ag_device *dev, uint32_t scale, uint32_t edge_factor, double a,
+                               double b, double c, uint32_t seed, int simplify,
+                               struct GraphCSR **out);
+int ag_graph_csr_generate_er(ag_device *dev, uint32_t num_vertices, uint64_t num_edges,
+                             uint32_t seed, int simplify, struct GraphCSR **out);
+
 /* ---------------------------------------------------------------- many small graphs */
 
 /* A batch of independent graphs resident in HBM, solved together by one persistent launch

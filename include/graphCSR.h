@@ -90,6 +90,10 @@ int graphCSRSetWeights(struct GraphCSR *graph, const float *forward_weights);
 
 void graphCSRFree(struct GraphCSR *graph);
 
+/* An empty graph of n vertices and m edges with every array allocated and zeroed (weights
+ * when `weighted`); what the builders fill. NULL when out of memory. */
+struct GraphCSR *graphCSRAlloc(uint32_t num_vertices, uint32_t num_edges, int weighted);
+
 #ifdef __cplusplus
 }
 #endif
