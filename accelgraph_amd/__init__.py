@@ -8,6 +8,7 @@ raises if the shared library is not built.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import weakref
 from dataclasses import dataclass, field
 
@@ -18,7 +19,8 @@ from ._abi import (BIN_NAMES, GATHER_F32, GATHER_U8, GATHER_U16, GATHER_U32, GAT
                    INIT_REFERENCE, INIT_UNIFORM, INIT_USER, KIND_F32, KIND_FIXED8,
                    KIND_FIXED16, KIND_FIXED32, KIND_FIXED64, STOP_ACTIVE, STOP_L1, STOP_NONE,
                    STORE_BF16, STORE_F16, STORE_F32, PPR_UNIFORM, SRC_BLOCK_AUTO,
-                   SRC_BLOCK_OFF, GraphInfo, PlanParams, PrOptions, PrResult)
+                   SRC_BLOCK_OFF, FILE_TEXT_EDGE_LIST, FILE_BINARY_EDGE_LIST,
+                   FILE_CSR_SERIALIZED, GraphInfo, PlanParams, PrOptions, PrResult)
 
 __all__ = ["lib", "AGError", "Graph", "Device", "DeviceGraph", "PrRun", "Batch", "MLP",
            "plan_params",
@@ -128,6 +130,37 @@ class Graph:
     @classmethod
     def load_binary(cls, path: str, simplify: bool = True) -> "Graph":
         return cls(lib().graphCSRLoadBinaryEdgeList(path.encode(), int(simplify)))
+
+    # graph files ---------------------------------------------------------
+    @classmethod
+    def _loaded(cls, ptr, what: str) -> "Graph":
+        if not ptr:
+            raise AGError(_abi.AG_ERR_INVALID, what, lib().graphIOLastError().decode())
+        return cls(ptr)
+
+    @classmethod
+    def load_text(cls, path: str, weighted: bool = False, simplify: bool = True) -> "Graph":
+        """Text edge list: `src dst [weight]` per line (space, tab or comma separated)."""
+        return cls._loaded(lib().graphCSRLoadTextEdgeList(os.fsencode(path), int(weighted),
+                                                          int(simplify)),
+                           "graphCSRLoadTextEdgeList")
+
+    @classmethod
+    def load_serialized(cls, path: str) -> "Graph":
+        return cls._loaded(lib().graphCSRLoadSerialized(os.fsencode(path)),
+                           "graphCSRLoadSerialized")
+
+    @classmethod
+    def load_file(cls, path: str, file_format: int = FILE_BINARY_EDGE_LIST,
+                  simplify: bool = True) -> "Graph":
+        """`file_format`: 0 text edge list, 1 binary edge list, 2 serialised CSR."""
+        return cls._loaded(lib().graphCSRLoadFile(os.fsencode(path), int(file_format),
+                                                  int(simplify)), "graphCSRLoadFile")
+
+    def save(self, path: str, file_format: int = FILE_CSR_SERIALIZED) -> None:
+        if lib().graphCSRSaveFile(self.ptr, os.fsencode(path), int(file_format)) != 0:
+            raise AGError(_abi.AG_ERR_INVALID, "graphCSRSaveFile",
+                          lib().graphIOLastError().decode())
 
     def set_weights(self, forward_weights) -> None:
         w = np.ascontiguousarray(forward_weights, dtype=np.float32)

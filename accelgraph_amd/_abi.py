@@ -23,6 +23,8 @@ AG_ERR_INVALID, AG_ERR_NO_DEVICE, AG_ERR_HIP, AG_ERR_OOM, AG_ERR_TIMEOUT, AG_ERR
 
 AG_NBINS = 5
 SRC_BLOCK_AUTO, SRC_BLOCK_OFF = 0, 255
+# graphCSR.h GRAPH_FILE_*: the reference driver's -z codes
+FILE_TEXT_EDGE_LIST, FILE_BINARY_EDGE_LIST, FILE_CSR_SERIALIZED = 0, 1, 2
 BIN_NAMES = ["thread", "group16", "wave", "block", "hub"]
 STOP_ACTIVE, STOP_L1, STOP_NONE = 0, 1, 2
 INIT_REFERENCE, INIT_UNIFORM, INIT_USER = 0, 1, 2
@@ -186,6 +188,14 @@ SIGNATURES = {
     "graphGenerateRMATEdges": (C.c_int, [u32, u64, f64, f64, f64, u32, P(u32), P(u32)]),
     "graphCSRLoadBinaryEdgeList": (P(GraphCSR), [C.c_char_p, C.c_int]),
     "graphCSRSetWeights": (C.c_int, [P(GraphCSR), P(f32)]),
+    "graphCSRLoadTextEdgeList": (P(GraphCSR), [C.c_char_p, C.c_int, C.c_int]),
+    "graphCSRLoadSerialized": (P(GraphCSR), [C.c_char_p]),
+    "graphCSRLoadFile": (P(GraphCSR), [C.c_char_p, C.c_int, C.c_int]),
+    "graphCSRSaveTextEdgeList": (C.c_int, [P(GraphCSR), C.c_char_p]),
+    "graphCSRSaveBinaryEdgeList": (C.c_int, [P(GraphCSR), C.c_char_p]),
+    "graphCSRSaveSerialized": (C.c_int, [P(GraphCSR), C.c_char_p]),
+    "graphCSRSaveFile": (C.c_int, [P(GraphCSR), C.c_char_p, C.c_int]),
+    "graphIOLastError": (C.c_char_p, []),
     "graphCSRFree": (None, [P(GraphCSR)]),
     "graphCSRAlloc": (P(GraphCSR), [u32, u32, C.c_int]),
     "ag_generate_rmat_edges": (C.c_int, [VP, u32, u64, f64, f64, f64, u32, P(u32), P(u32)]),

@@ -26,7 +26,7 @@ CSRC_DIR = os.path.join(PKG, "csrc")
 BUILD_DIR = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libaccelgraph.so")
 
-HOST_SOURCES = ["mt19937.c", "graph_csr.c", "plan.c", "accel_context.c", "pageRank.c", "SPMV.c"]
+HOST_SOURCES = ["mt19937.c", "graph_csr.c", "graph_io.c", "plan.c", "accel_context.c", "pageRank.c", "SPMV.c"]
 HIP_SOURCES = ["ag_runtime.hip", "ag_pagerank.hip", "ag_spmv.hip", "ag_batch.hip", "ag_ppr.hip", "ag_tune.hip", "ag_mlp.hip", "ag_build.hip"]
 
 ARCH = "gfx950"

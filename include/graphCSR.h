@@ -84,6 +84,39 @@ int graphGenerateRMATEdges(uint32_t scale, uint64_t num_edges, double a, double 
  * reference's 04_test_graphs/TEST/graphbrew/graph.bin. */
 struct GraphCSR *graphCSRLoadBinaryEdgeList(const char *path, int simplify);
 
+/* ---- graph files (the reference driver's -z read formats, accel-graph.c:51-53, and its
+ * -c/-x conversion, accel-graph.c:115-128). On failure the loaders return NULL and the
+ * savers -1; graphIOLastError() then describes the failure (file, line, reason). ---- */
+
+enum {
+    GRAPH_FILE_TEXT_EDGE_LIST = 0,    /* "src dst [weight]" per line */
+    GRAPH_FILE_BINARY_EDGE_LIST = 1,  /* u32 LE (src,dst) pairs, no header */
+    GRAPH_FILE_CSR_SERIALIZED = 2     /* every array of struct GraphCSR, checksummed */
+};
+
+/* Text edge list: one edge per line, fields separated by spaces, tabs or commas, '#' and
+ * '%' lines and blank lines skipped, further columns ignored. With `weighted` every line
+ * must carry a third (float) column. Vertex count = 1 + largest id. */
+struct GraphCSR *graphCSRLoadTextEdgeList(const char *path, int weighted, int simplify);
+
+/* A graph saved by graphCSRSaveSerialized, every array as it was (no rebuild). The file is
+ * checked (checksum, offsets, id ranges, the two views against each other) before use. */
+struct GraphCSR *graphCSRLoadSerialized(const char *path);
+
+/* `file_format` is one of GRAPH_FILE_*; `simplify` applies to the edge-list formats. */
+struct GraphCSR *graphCSRLoadFile(const char *path, int file_format, int simplify);
+
+/* Writers. The edge-list forms write the forward edges in CSR order (with the weights as a
+ * third text column when the graph has them; the binary edge list has no weights), so
+ * loading a saved simplified graph rebuilds the same arrays. Return 0 on success. */
+int graphCSRSaveTextEdgeList(const struct GraphCSR *graph, const char *path);
+int graphCSRSaveBinaryEdgeList(const struct GraphCSR *graph, const char *path);
+int graphCSRSaveSerialized(const struct GraphCSR *graph, const char *path);
+int graphCSRSaveFile(const struct GraphCSR *graph, const char *path, int file_format);
+
+/* The last graph-file error of the calling thread ("" when none). */
+const char *graphIOLastError(void);
+
 /* Attach per-edge weights given in forward (sorted_edges_array) order; the inverse
  * weights are permuted to match. Returns 0 on success. */
 int graphCSRSetWeights(struct GraphCSR *graph, const float *forward_weights);

@@ -365,6 +365,29 @@ def test_edge_cases(device):
     dg.close()
 
 
+@pytest.mark.parametrize("fname,fmt", [("graphbrew_graph.bin", ag.FILE_BINARY_EDGE_LIST),
+                                       ("graphbrew_graph.org", ag.FILE_TEXT_EDGE_LIST),
+                                       ("graphbrew_graph.bin.txt", ag.FILE_TEXT_EDGE_LIST)])
+def test_reference_graph_files(device, tmp_path, fname, fmt):
+    """The reference's test graph (04_test_graphs/TEST/graphbrew/) read in each of its file
+    formats, and again after a serialised-CSR round trip, through the path."""
+    say(f"reference graph file {fname}")
+    g = ag.Graph.load_file(os.path.join(HERE, "golden", fname), fmt)
+    g.save(str(tmp_path / "g.csr"))
+    for h in (g, ag.Graph.load_serialized(str(tmp_path / "g.csr"))):
+        for pname in ("default", "blocked-all"):
+            dg = upload(device, h, pname)
+            run = dg.pagerank(kind=32, max_iterations=100, epsilon=1e-6, scaled=True)
+            ref = ref_pr(h, kind=32, max_iterations=100, epsilon=1e-6)
+            assert run.iterations == ref.iterations
+            np.testing.assert_array_equal(run.ranks.view(np.uint32), ref.ranks.view(np.uint32))
+            run = dg.pagerank(max_iterations=100)
+            ref = ref_pr(h, max_iterations=100)
+            assert run.iterations == ref.iterations
+            np.testing.assert_allclose(run.ranks, ref.ranks, rtol=0, atol=FP32_TOL)
+            dg.close()
+
+
 # ------------------------------------------------------------------ 16-bit storage
 
 @pytest.mark.parametrize("storage", [ag.STORE_BF16, ag.STORE_F16])
