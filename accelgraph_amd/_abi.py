@@ -29,6 +29,7 @@ BIN_NAMES = ["thread", "group16", "wave", "block", "hub"]
 STOP_ACTIVE, STOP_L1, STOP_NONE = 0, 1, 2
 INIT_REFERENCE, INIT_UNIFORM, INIT_USER = 0, 1, 2
 KIND_F32, KIND_FIXED64, KIND_FIXED32, KIND_FIXED16, KIND_FIXED8 = 0, 64, 32, 16, 8
+KIND_QUANT32, KIND_QUANT16, KIND_QUANT8 = 132, 116, 108
 STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
 GATHER_F32, GATHER_U64, GATHER_U32, GATHER_U16, GATHER_U8 = 0, 1, 2, 3, 4
 PPR_UNIFORM = 0xFFFFFFFF
@@ -218,6 +219,9 @@ SIGNATURES = {
     "pageRankPullFixedPoint32BitGraphCSR": (P(PageRankStats), [P(Arguments), P(GraphCSR)]),
     "pageRankPullFixedPoint16BitGraphCSR": (P(PageRankStats), [P(Arguments), P(GraphCSR)]),
     "pageRankPullFixedPoint8BitGraphCSR": (P(PageRankStats), [P(Arguments), P(GraphCSR)]),
+    "pageRankPullQuant32BitGraphCSR": (P(PageRankStats), [P(Arguments), P(GraphCSR)]),
+    "pageRankPullQuant16BitGraphCSR": (P(PageRankStats), [P(Arguments), P(GraphCSR)]),
+    "pageRankPullQuant8BitGraphCSR": (P(PageRankStats), [P(Arguments), P(GraphCSR)]),
     # SPMV.h
     "newSPMVStatsGraphCSR": (P(SPMVStats), [P(GraphCSR)]),
     "freeSPMVStats": (None, [P(SPMVStats)]),

@@ -103,6 +103,8 @@ struct PrState {
     double last_err;
     double best_err;        // smallest L1 delta of the 16-bit phase so far
     uint32_t stall;         // 16-bit iterations since best_err last improved by 1%
+    uint32_t qmax_bits;     // quantised kinds: bits of the largest next contribution
+    float qscale;           // quantised kinds: scale of the codes being gathered
     uint32_t pad;
 };
 
@@ -116,6 +118,7 @@ struct ag_pr_session {
     PrState *st_host;           // pinned
     double *part_err;
     uint32_t *part_act;
+    uint32_t *part_cmax;        // quantised kinds: per-workgroup largest contribution bits
     double *hist_err;
     uint32_t *hist_act;
     float *rank;

@@ -21,7 +21,9 @@
 // a host round trip per iteration.
 //
 // Fixed-point variants (pageRank.c:1276-1990) gather u64/u32/u16/u8 contributions into an
-// exact u64 sum (wraps modulo 2^64) and convert with the formats of fixedPoint.h.
+// exact u64 sum (wraps modulo 2^64) and convert with the formats of fixedPoint.h. Quantised
+// variants (pageRank.c:2119-2676) gather u32/u16/u8 codes of a per-iteration scale into an
+// exact u64 sum; a small pr_quant_kernel after each finish writes the next codes.
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -64,6 +66,7 @@ struct PrCtl {
     PrState *st;
     double *part_err;      // [nparts] per-workgroup partials of the main launch
     uint32_t *part_act;    // [nparts]
+    uint32_t *part_cmax;   // [nparts] quantised kinds only (else null): largest contribution
     double *hist_err;      // [max_iterations]
     uint32_t *hist_act;    // [max_iterations]
     uint32_t nparts;       // partials of all passes of one iteration
@@ -110,6 +113,11 @@ __device__ __forceinline__ void pr_store_partial(const PrCtl &C, Red red)
     __shared__ double s_e[AG_WG / AG_WAVE];
     __shared__ uint32_t s_a[AG_WG / AG_WAVE];
     Red t = block_reduce_red<AG_WG>(red, s_e, s_a);
+    if (C.part_cmax) {
+        const uint32_t m = block_max_u32<AG_WG>(red.cmax, s_a);
+        if (threadIdx.x == 0)
+            C.part_cmax[C.part_base + blockIdx.x] = m;
+    }
     if (threadIdx.x == 0) {
         C.part_err[C.part_base + blockIdx.x] = t.err;
         C.part_act[C.part_base + blockIdx.x] = t.active;
@@ -122,11 +130,20 @@ __device__ __forceinline__ void pr_close(const PrCtl &C, uint32_t iter, Red red)
 {
     __shared__ double s_e[AG_FIN_WG / AG_WAVE];
     __shared__ uint32_t s_a[AG_FIN_WG / AG_WAVE];
+    uint32_t mx = red.cmax;
     for (uint32_t i = threadIdx.x; i < C.nparts; i += AG_FIN_WG) {
         red.err += C.part_err[i];
         red.active += C.part_act[i];
+        if (C.part_cmax)
+            mx = max(mx, C.part_cmax[i]);
     }
     Red t = block_reduce_red<AG_FIN_WG>(red, s_e, s_a);
+    if (C.part_cmax) {
+        // the quantiser of the next iteration reads this (after the launch boundary)
+        mx = block_max_u32<AG_FIN_WG>(mx, s_a);
+        if (threadIdx.x == 0)
+            C.st->qmax_bits = mx;
+    }
     if (threadIdx.x == 0)
         pr_control(C, iter, t.err, t.active);
 }
@@ -386,8 +403,115 @@ __device__ __forceinline__ PrFixPol<BITS> fix_policy(uint32_t iter, const PrF32A
                           A.damp, A.dn, A.eps};
 }
 
+// ---------------------------------------------------------------- quantised policy
+//
+// pageRankPullQuant{32,16,8}BitGraphCSR (pageRank.c:2119-2676): per iteration the f32
+// contributions get a scale from their maximum (zero point 0), are quantised to B-bit codes,
+// the codes are summed exactly in u64 and the update is base + damp * (scale * (float)sum).
+// Here the update writes the f32 contributions and a per-workgroup maximum, the finish kernel
+// reduces the maximum, and pr_quant_kernel writes the codes the next gather reads; the u64
+// sums are exact, so any summation order gives the same ranks. The quantiser (the reference's
+// quantization.h is not in its tree) is defined as
+//   scale = mx > 0 ? (float)((double)mx / (2^B - 1)) : 1     mx = largest finite c >= 0
+//   code  = c > 0 ? min(2^B - 1, floor((double)c * (1 / (double)scale) + 0.5)) : 0
+
+// bits of a contribution if it takes part in the maximum (finite and not negative), else 0
+__device__ __forceinline__ uint32_t quant_max_bits(float c)
+{
+    const uint32_t b = __float_as_uint(c);
+    return b < 0x7F800000u ? b : 0u;
+}
+
 template <int BITS>
-__global__ __launch_bounds__(AG_WG) void pr_fixed_kernel(PullGeom G, PrCtl C, PrF32Args A)
+__device__ __forceinline__ float quant_scale(uint32_t mx_bits)
+{
+    const float mx = __uint_as_float(mx_bits);
+    const double maxcode = (double)((1ull << BITS) - 1ull);
+    return mx > 0.0f ? __double2float_rn(__ddiv_rn((double)mx, maxcode)) : 1.0f;
+}
+
+template <int BITS>
+__device__ __forceinline__ typename FixT<BITS>::T quant_code(float c, double inv)
+{
+    using CT = typename FixT<BITS>::T;
+    const double maxcode = (double)((1ull << BITS) - 1ull);
+    if (!(c > 0.0f))
+        return (CT)0;
+    const double t = floor(__dadd_rn(__dmul_rn((double)c, inv), 0.5));
+    return t >= maxcode ? (CT)((1ull << BITS) - 1ull) : (CT)(uint32_t)t;
+}
+
+template <int BITS>
+struct PrQuantPol {
+    using Acc = uint64_t;
+    using CT = typename FixT<BITS>::T;
+    const CT *codes;
+    float *cf;            // f32 contributions of the next iteration
+    float *rank;
+    const uint32_t *outdeg;
+    float base, damp, scale;
+    double dn, eps;
+
+    __device__ __forceinline__ uint64_t edge(uint32_t, uint32_t s) const { return (uint64_t)codes[s]; }
+    __device__ __forceinline__ void finish(uint32_t r, uint64_t q, Red &red) const
+    {
+        const float nw = __fadd_rn(base, __fmul_rn(damp, __fmul_rn(scale, __ull2float_rn(q))));
+        const float old = __builtin_nontemporal_load(&rank[r]);
+        __builtin_nontemporal_store(nw, &rank[r]);
+        const double e = (double)fabsf(__fsub_rn(nw, old));
+        red.err += __ddiv_rn(e, dn);
+        red.active += (e >= eps) ? 1u : 0u;
+        const uint32_t od = __builtin_nontemporal_load(&outdeg[r]);
+        const float c = od ? __fdiv_rn(nw, __uint2float_rn(od)) : 0.0f;
+        cf[r] = c;
+        red.cmax = max(red.cmax, quant_max_bits(c));
+    }
+};
+
+// Integer-sum policies: KC = 64/32/16/8 fixed point, 100 + B quantised B-bit codes.
+template <int KC>
+__device__ __forceinline__ auto int_policy(const PrState *st, uint32_t iter, const PrF32Args &A)
+{
+    if constexpr (KC > 100) {
+        using CT = typename FixT<KC - 100>::T;
+        const float scale = *(volatile const float *)&st->qscale;
+        return PrQuantPol<KC - 100>{(const CT *)A.c32[1], (float *)A.c32[0], A.rank, A.outdeg,
+                                    A.base, A.damp, scale, A.dn, A.eps};
+    } else {
+        return fix_policy<KC>(iter, A);
+    }
+}
+
+// Codes of the next iteration from its f32 contributions; records the scale they use.
+template <int BITS>
+__global__ __launch_bounds__(AG_WG) void pr_quant_kernel(uint32_t n, PrState *st, const float *cf,
+                                                         typename FixT<BITS>::T *codes)
+{
+    if (*(volatile const uint32_t *)&st->done)
+        return;
+    const float scale = quant_scale<BITS>(*(volatile const uint32_t *)&st->qmax_bits);
+    const double inv = __ddiv_rn(1.0, (double)scale);
+    for (uint32_t v = blockIdx.x * AG_WG + threadIdx.x; v < n; v += gridDim.x * AG_WG)
+        codes[v] = quant_code<BITS>(cf[v], inv);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        st->qscale = scale;
+}
+
+// Largest initial contribution (st->qmax_bits starts at 0).
+__global__ __launch_bounds__(AG_WG) void pr_quant_init_max(uint32_t n, const float *cf, PrState *st)
+{
+    __shared__ uint32_t s_w[AG_WG / AG_WAVE];
+    uint32_t m = 0u;
+    for (uint32_t v = blockIdx.x * AG_WG + threadIdx.x; v < n; v += gridDim.x * AG_WG)
+        m = max(m, quant_max_bits(cf[v]));
+    m = block_max_u32<AG_WG>(m, s_w);
+    if (threadIdx.x == 0)
+        atomicMax(&st->qmax_bits, m);
+}
+
+// Integer-sum iteration kernels (fixed point and quantised), KC as in int_policy.
+template <int KC>
+__global__ __launch_bounds__(AG_WG) void pr_int_kernel(PullGeom G, PrCtl C, PrF32Args A)
 {
     __shared__ uint64_t s_vals[AG_TILE_NNZ_MAX];
     __shared__ uint64_t s_wave[AG_WG / AG_WAVE];
@@ -396,12 +520,12 @@ __global__ __launch_bounds__(AG_WG) void pr_fixed_kernel(PullGeom G, PrCtl C, Pr
         return;
     const uint32_t iter = *(volatile const uint32_t *)&st->iter;
     Red red{0.0, 0u};
-    pull_items(G, fix_policy<BITS>(iter, A), red, s_vals, s_wave);
+    pull_items(G, int_policy<KC>(st, iter, A), red, s_vals, s_wave);
     pr_store_partial(C, red);
 }
 
-template <int BITS>
-__global__ __launch_bounds__(AG_WG) void pr_fixed_blk_kernel(PullGeom U, BlockGeom B, PrCtl C,
+template <int KC>
+__global__ __launch_bounds__(AG_WG) void pr_int_blk_kernel(PullGeom U, BlockGeom B, PrCtl C,
                                                              PrF32Args A)
 {
     __shared__ uint64_t s_vals[AG_TILE_NNZ_MAX];
@@ -412,13 +536,13 @@ __global__ __launch_bounds__(AG_WG) void pr_fixed_blk_kernel(PullGeom U, BlockGe
         return;
     const uint32_t iter = *(volatile const uint32_t *)&st->iter;
     Red red{0.0, 0u};
-    const PrFixPol<BITS> p = fix_policy<BITS>(iter, A);
+    const auto p = int_policy<KC>(st, iter, A);
     pull_blocked_items(U, B, p, p, red, s_vals, s_wave, L);
     pr_store_partial(C, red);
 }
 
-template <int BITS>
-__global__ __launch_bounds__(AG_WG) void pr_fixed_cmb_kernel(BlockGeom B, PrCtl C, PrF32Args A)
+template <int KC>
+__global__ __launch_bounds__(AG_WG) void pr_int_cmb_kernel(BlockGeom B, PrCtl C, PrF32Args A)
 {
     __shared__ uint64_t s_vals[AG_TILE_NNZ_MAX];
     __shared__ uint64_t s_wave[AG_WG / AG_WAVE];
@@ -428,25 +552,25 @@ __global__ __launch_bounds__(AG_WG) void pr_fixed_cmb_kernel(BlockGeom B, PrCtl 
         return;
     const uint32_t iter = *(volatile const uint32_t *)&st->iter;
     Red red{0.0, 0u};
-    pull_combine_items(B, fix_policy<BITS>(iter, A), red, s_vals, s_wave, L);
+    pull_combine_items(B, int_policy<KC>(st, iter, A), red, s_vals, s_wave, L);
     pr_store_partial(C, red);
 }
 
-template <int BITS>
-__global__ __launch_bounds__(AG_FIN_WG) void pr_fixed_finish(PullGeom G, PrCtl C, PrF32Args A)
+template <int KC>
+__global__ __launch_bounds__(AG_FIN_WG) void pr_int_finish(PullGeom G, PrCtl C, PrF32Args A)
 {
     const PrState *st = C.st;
     if (*(volatile const uint32_t *)&st->done)
         return;
     const uint32_t iter = *(volatile const uint32_t *)&st->iter;
     Red red{0.0, 0u};
-    pull_hub_finish(G, fix_policy<BITS>(iter, A), red);
+    pull_hub_finish(G, int_policy<KC>(st, iter, A), red);
     pr_close(C, iter, red);
 }
 
 // ---------------------------------------------------------------- init / finalise
 
-// kind: 0 = f32 contributions, 1 = bf16, 2 = f16, 64/32/16/8 = fixed point
+// kind: 0 = f32 contributions, 1 = bf16, 2 = f16, 64/32/16/8 = fixed point, 1xx = quantised
 __global__ void pr_init_kernel(uint32_t n, uint32_t kind, uint32_t init, float base,
                                const float *user_orig, const uint32_t *d2o,
                                const uint32_t *outdeg, float *rank, void *c0)
@@ -466,7 +590,8 @@ __global__ void pr_init_kernel(uint32_t n, uint32_t kind, uint32_t init, float b
     case 64: ((uint64_t *)c0)[r] = od ? ag_dev_double_to_fixed64((double)q) : 0ull; break;
     case 32: ((uint32_t *)c0)[r] = od ? ag_dev_float_to_fixed32(q) : 0u; break;
     case 16: ((uint16_t *)c0)[r] = od ? ag_dev_float_to_fixed16(q) : (uint16_t)0; break;
-    default: ((uint8_t *)c0)[r] = od ? ag_dev_float_to_fixed8(q) : (uint8_t)0; break;
+    case 8: ((uint8_t *)c0)[r] = od ? ag_dev_float_to_fixed8(q) : (uint8_t)0; break;
+    default: ((float *)c0)[r] = q; break;   // quantised kinds: f32, codes follow
     }
 }
 
@@ -551,8 +676,9 @@ static void session_free(ag_pr_session *s)
         return;
     hipSetDevice(s->g->dev->ordinal);
     hipStreamSynchronize(s->g->dev->stream);
-    void *ptrs[] = {s->st, s->part_err, s->part_act, s->hist_err, s->hist_act, s->rank, s->c[0],
-                    s->c[1], s->c16[0], s->c16[1], s->fin_part, s->fin_sum, s->user};
+    void *ptrs[] = {s->st, s->part_err, s->part_act, s->part_cmax, s->hist_err, s->hist_act,
+                    s->rank, s->c[0], s->c[1], s->c16[0], s->c16[1], s->fin_part, s->fin_sum,
+                    s->user};
     for (void *p : ptrs)
         if (p)
             hipFree(p);
@@ -586,6 +712,33 @@ static int salloc(ag_pr_session *s, void **p, size_t bytes)
         }                           \
     } while (0)
 
+static uint32_t ag_quant_grid(const ag_dgraph *g)
+{
+    const uint32_t need = (g->n + AG_WG - 1) / AG_WG;
+    const uint32_t cap = g->dev->num_cus * 8u;
+    return need < 1u ? 1u : need < cap ? need : cap;
+}
+
+// Codes of the next iteration (quantised kinds), after the iteration's finish kernel.
+static void ag_launch_quant(ag_pr_session *s)
+{
+    ag_dgraph *g = s->g;
+    hipStream_t stream = g->dev->stream;
+    const uint32_t grid = ag_quant_grid(g);
+    const float *cf = (const float *)s->c[0];
+    switch (s->kind_code) {
+    case 132:
+        pr_quant_kernel<32><<<grid, AG_WG, 0, stream>>>(g->n, s->st, cf, (uint32_t *)s->c[1]);
+        break;
+    case 116:
+        pr_quant_kernel<16><<<grid, AG_WG, 0, stream>>>(g->n, s->st, cf, (uint16_t *)s->c[1]);
+        break;
+    default:
+        pr_quant_kernel<8><<<grid, AG_WG, 0, stream>>>(g->n, s->st, cf, (uint8_t *)s->c[1]);
+        break;
+    }
+}
+
 extern "C" int ag_pr_begin(ag_dgraph *g, const ag_pr_options *opts, ag_pr_session **out)
 {
     if (!g || !out)
@@ -609,6 +762,10 @@ extern "C" int ag_pr_begin(ag_dgraph *g, const ag_pr_options *opts, ag_pr_sessio
     case AG_KIND_FIXED32: kc = 32; celem = 4; break;
     case AG_KIND_FIXED16: kc = 16; celem = 2; break;
     case AG_KIND_FIXED8: kc = 8; celem = 1; break;
+    // quantised: c[0] holds the f32 contributions, c[1] the codes
+    case AG_KIND_QUANT32: kc = 132; celem = 4; break;
+    case AG_KIND_QUANT16: kc = 116; celem = 4; break;
+    case AG_KIND_QUANT8: kc = 108; celem = 4; break;
     default: return ag_set_error(AG_ERR_INVALID, "unknown kind %u", o.kind);
     }
     if (o.kind != AG_KIND_F32 && o.storage != AG_STORE_F32)
@@ -645,6 +802,8 @@ extern "C" int ag_pr_begin(ag_dgraph *g, const ag_pr_options *opts, ag_pr_sessio
     }
     AG_TRY(salloc(s, (void **)&s->part_err, (size_t)nblk * sizeof(double)));
     AG_TRY(salloc(s, (void **)&s->part_act, (size_t)nblk * sizeof(uint32_t)));
+    if (kc > 100)
+        AG_TRY(salloc(s, (void **)&s->part_cmax, (size_t)nblk * sizeof(uint32_t)));
     AG_TRY(salloc(s, (void **)&s->hist_err, (size_t)maxit * sizeof(double)));
     AG_TRY(salloc(s, (void **)&s->hist_act, (size_t)maxit * sizeof(uint32_t)));
     AG_TRY(salloc(s, (void **)&s->rank, (size_t)n * sizeof(float)));
@@ -681,6 +840,11 @@ extern "C" int ag_pr_begin(ag_dgraph *g, const ag_pr_options *opts, ag_pr_sessio
         void *c0 = (kc == 1 || kc == 2) ? s->c16[0] : s->c[0];
         pr_init_kernel<<<(n + AG_WG - 1) / AG_WG, AG_WG, 0, stream>>>(
             n, kc, o.init, pr_base(), s->user, g->d2o, g->outdeg, s->rank, c0);
+        if (kc > 100) {
+            pr_quant_init_max<<<ag_quant_grid(g), AG_WG, 0, stream>>>(n, (const float *)s->c[0],
+                                                                     s->st);
+            ag_launch_quant(s);
+        }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) {
             session_free(s);
@@ -746,6 +910,7 @@ int ag_pr_launch_iteration(ag_pr_session *s)
     C.st = s->st;
     C.part_err = s->part_err;
     C.part_act = s->part_act;
+    C.part_cmax = s->kind_code > 100 ? s->part_cmax : nullptr;
     C.hist_err = s->hist_err;
     C.hist_act = s->hist_act;
     C.part_base = 0;
@@ -773,18 +938,29 @@ int ag_pr_launch_iteration(ag_pr_session *s)
         case 1: AG_PR_LAUNCH_BLOCKED(pr_bf16_blk_kernel, pr_bf16_cmb_kernel, pr_bf16_finish); break;
         case 2: AG_PR_LAUNCH_BLOCKED(pr_f16_blk_kernel, pr_f16_cmb_kernel, pr_f16_finish); break;
         case 64:
-            AG_PR_LAUNCH_BLOCKED(pr_fixed_blk_kernel<64>, pr_fixed_cmb_kernel<64>, pr_fixed_finish<64>);
+            AG_PR_LAUNCH_BLOCKED(pr_int_blk_kernel<64>, pr_int_cmb_kernel<64>, pr_int_finish<64>);
             break;
         case 32:
-            AG_PR_LAUNCH_BLOCKED(pr_fixed_blk_kernel<32>, pr_fixed_cmb_kernel<32>, pr_fixed_finish<32>);
+            AG_PR_LAUNCH_BLOCKED(pr_int_blk_kernel<32>, pr_int_cmb_kernel<32>, pr_int_finish<32>);
             break;
         case 16:
-            AG_PR_LAUNCH_BLOCKED(pr_fixed_blk_kernel<16>, pr_fixed_cmb_kernel<16>, pr_fixed_finish<16>);
+            AG_PR_LAUNCH_BLOCKED(pr_int_blk_kernel<16>, pr_int_cmb_kernel<16>, pr_int_finish<16>);
+            break;
+        case 8:
+            AG_PR_LAUNCH_BLOCKED(pr_int_blk_kernel<8>, pr_int_cmb_kernel<8>, pr_int_finish<8>);
+            break;
+        case 132:
+            AG_PR_LAUNCH_BLOCKED(pr_int_blk_kernel<132>, pr_int_cmb_kernel<132>, pr_int_finish<132>);
+            break;
+        case 116:
+            AG_PR_LAUNCH_BLOCKED(pr_int_blk_kernel<116>, pr_int_cmb_kernel<116>, pr_int_finish<116>);
             break;
         default:
-            AG_PR_LAUNCH_BLOCKED(pr_fixed_blk_kernel<8>, pr_fixed_cmb_kernel<8>, pr_fixed_finish<8>);
+            AG_PR_LAUNCH_BLOCKED(pr_int_blk_kernel<108>, pr_int_cmb_kernel<108>, pr_int_finish<108>);
             break;
         }
+        if (s->kind_code > 100)
+            ag_launch_quant(s);
         AG_HIP(hipGetLastError());
         return AG_OK;
     }
@@ -794,11 +970,16 @@ int ag_pr_launch_iteration(ag_pr_session *s)
     case 0: AG_PR_LAUNCH_REGULAR(pr_f32_kernel, pr_f32_finish); break;
     case 1: AG_PR_LAUNCH_REGULAR(pr_bf16_kernel, pr_bf16_finish); break;
     case 2: AG_PR_LAUNCH_REGULAR(pr_f16_kernel, pr_f16_finish); break;
-    case 64: AG_PR_LAUNCH_REGULAR(pr_fixed_kernel<64>, pr_fixed_finish<64>); break;
-    case 32: AG_PR_LAUNCH_REGULAR(pr_fixed_kernel<32>, pr_fixed_finish<32>); break;
-    case 16: AG_PR_LAUNCH_REGULAR(pr_fixed_kernel<16>, pr_fixed_finish<16>); break;
-    default: AG_PR_LAUNCH_REGULAR(pr_fixed_kernel<8>, pr_fixed_finish<8>); break;
+    case 64: AG_PR_LAUNCH_REGULAR(pr_int_kernel<64>, pr_int_finish<64>); break;
+    case 32: AG_PR_LAUNCH_REGULAR(pr_int_kernel<32>, pr_int_finish<32>); break;
+    case 16: AG_PR_LAUNCH_REGULAR(pr_int_kernel<16>, pr_int_finish<16>); break;
+    case 8: AG_PR_LAUNCH_REGULAR(pr_int_kernel<8>, pr_int_finish<8>); break;
+    case 132: AG_PR_LAUNCH_REGULAR(pr_int_kernel<132>, pr_int_finish<132>); break;
+    case 116: AG_PR_LAUNCH_REGULAR(pr_int_kernel<116>, pr_int_finish<116>); break;
+    default: AG_PR_LAUNCH_REGULAR(pr_int_kernel<108>, pr_int_finish<108>); break;
     }
+    if (s->kind_code > 100)
+        ag_launch_quant(s);
     AG_HIP(hipGetLastError());
     return AG_OK;
 }

@@ -38,6 +38,7 @@
 struct Red {
     double err;
     uint32_t active;
+    uint32_t cmax;      // quantised PageRank: bits of the largest contribution written
 };
 
 template <typename T>
@@ -81,7 +82,30 @@ __device__ __forceinline__ Red block_reduce_red(Red r, double *s_e, uint32_t *s_
     Red out;
     out.err = block_reduce<double, NT>(r.err, s_e);
     out.active = block_reduce<uint32_t, NT>(r.active, s_a);
+    out.cmax = 0u;
     return out;
+}
+
+// Largest value over an NT-lane workgroup; valid in thread 0. s_wave holds NT/64.
+template <uint32_t NT>
+__device__ __forceinline__ uint32_t block_max_u32(uint32_t v, uint32_t *s_wave)
+{
+    const uint32_t tid = threadIdx.x;
+#pragma unroll
+    for (int m = AG_WAVE / 2; m >= 1; m >>= 1)
+        v = max(v, (uint32_t)__shfl_xor(v, m, AG_WAVE));
+    if ((tid & (AG_WAVE - 1)) == 0)
+        s_wave[tid / AG_WAVE] = v;
+    __syncthreads();
+    uint32_t t = v;
+    if (tid == 0) {
+        t = s_wave[0];
+#pragma unroll
+        for (uint32_t k = 1; k < NT / AG_WAVE; k++)
+            t = max(t, s_wave[k]);
+    }
+    __syncthreads();
+    return t;
 }
 
 template <typename Acc>

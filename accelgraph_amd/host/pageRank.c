@@ -57,11 +57,15 @@ static const char *variant_title(uint32_t kind)
     case AG_KIND_FIXED32: return "Starting Page Rank Pull FP_32 (tolerance/epsilon)";
     case AG_KIND_FIXED16: return "Starting Page Rank Pull FP_16 (tolerance/epsilon)";
     case AG_KIND_FIXED8: return "Starting Page Rank Pull FP_8 (tolerance/epsilon)";
+    case AG_KIND_QUANT32: return "Starting Page Rank Pull Quant_32 (tolerance/epsilon)";
+    case AG_KIND_QUANT16: return "Starting Page Rank Pull Quant_16 (tolerance/epsilon)";
+    case AG_KIND_QUANT8: return "Starting Page Rank Pull Quant_8 (tolerance/epsilon)";
     default: return "Starting Page Rank Pull (tolerance/epsilon)";
     }
 }
 
-/* Shared body of the FP32 and fixed-point pulls (pageRank.c:942-1109, 1276-1990). */
+/* Shared body of the FP32, fixed-point and quantised pulls (pageRank.c:942-1109,
+ * 1276-1990, 2119-2676). */
 static struct PageRankStats *pagerank_pull_kind(struct Arguments *arguments,
                                                 struct GraphCSR *graph, uint32_t kind)
 {
@@ -174,8 +178,29 @@ struct PageRankStats *pageRankPullFixedPoint8BitGraphCSR(struct Arguments *argum
     return pagerank_pull_kind(arguments, graph, AG_KIND_FIXED8);
 }
 
-/* pageRank.c:874-939. Only the pull directions are provided; the push, quantised and
- * data-driven variants (pushpull 1,3-8,12,13) are outside this library. */
+/* pageRank.c:2119-2303 */
+struct PageRankStats *pageRankPullQuant32BitGraphCSR(struct Arguments *arguments,
+                                                     struct GraphCSR *graph)
+{
+    return pagerank_pull_kind(arguments, graph, AG_KIND_QUANT32);
+}
+
+/* pageRank.c:2305-2489 */
+struct PageRankStats *pageRankPullQuant16BitGraphCSR(struct Arguments *arguments,
+                                                     struct GraphCSR *graph)
+{
+    return pagerank_pull_kind(arguments, graph, AG_KIND_QUANT16);
+}
+
+/* pageRank.c:2491-2676 */
+struct PageRankStats *pageRankPullQuant8BitGraphCSR(struct Arguments *arguments,
+                                                    struct GraphCSR *graph)
+{
+    return pagerank_pull_kind(arguments, graph, AG_KIND_QUANT8);
+}
+
+/* pageRank.c:874-939. The pull directions are provided; the push and data-driven variants
+ * (pushpull 1,3,5-8) are outside this library. */
 struct PageRankStats *pageRankGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
 {
     switch (arguments->pushpull) {
@@ -189,7 +214,13 @@ struct PageRankStats *pageRankGraphCSR(struct Arguments *arguments, struct Graph
         return pageRankPullFixedPoint16BitGraphCSR(arguments, graph);
     case 11:
         return pageRankPullFixedPoint8BitGraphCSR(arguments, graph);
-    case 1: case 3: case 4: case 5: case 6: case 7: case 8: case 12: case 13:
+    case 4:
+        return pageRankPullQuant32BitGraphCSR(arguments, graph);
+    case 12:
+        return pageRankPullQuant16BitGraphCSR(arguments, graph);
+    case 13:
+        return pageRankPullQuant8BitGraphCSR(arguments, graph);
+    case 1: case 3: case 5: case 6: case 7: case 8:
         fprintf(stderr, "[accelgraph] pageRankGraphCSR: pushpull=%u is not a pull variant "
                         "provided by this library\n", arguments->pushpull);
         exit(EXIT_FAILURE);

@@ -161,7 +161,13 @@ enum { AG_INIT_REFERENCE = 0,  /* every scaled rank 1 - Damp (pageRank.c:57-58,7
        AG_INIT_USER = 2 };     /* init_ranks[], scaled units, original vertex order */
 
 enum { AG_KIND_F32 = 0, AG_KIND_FIXED64 = 64, AG_KIND_FIXED32 = 32, AG_KIND_FIXED16 = 16,
-       AG_KIND_FIXED8 = 8 };
+       AG_KIND_FIXED8 = 8,
+       /* quantised pulls (pageRankPullQuant{32,16,8}BitGraphCSR, pageRank.c:2119-2676): per
+        * iteration the f32 contributions are quantised with scale = max / (2^B - 1) and zero
+        * point 0 into B-bit codes, the codes are summed exactly in u64 and the update is
+        * base + damp * (scale * (float)sum). The quantiser is this library's definition
+        * (the reference's quantization.h is not in its tree). */
+       AG_KIND_QUANT32 = 132, AG_KIND_QUANT16 = 116, AG_KIND_QUANT8 = 108 };
 
 enum { AG_STORE_F32 = 0, AG_STORE_BF16 = 1, AG_STORE_F16 = 2 };
 

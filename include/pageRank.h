@@ -12,6 +12,9 @@
  *   pageRankPullFixedPoint32BitGraphCSR pageRank.h:74     (pageRank.c:1455-1633)
  *   pageRankPullFixedPoint16BitGraphCSR pageRank.h:75     (pageRank.c:1635-1811)
  *   pageRankPullFixedPoint8BitGraphCSR  pageRank.h:76     (pageRank.c:1813-1990)
+ *   pageRankPullQuant32BitGraphCSR      pageRank.h:79     (pageRank.c:2119-2303)
+ *   pageRankPullQuant16BitGraphCSR      pageRank.h:80     (pageRank.c:2305-2489)
+ *   pageRankPullQuant8BitGraphCSR       pageRank.h:81     (pageRank.c:2491-2676)
  *
  * The gather and the per-vertex update run on the GPU through the accelgraph C-ABI
  * (accelgraph.h). As in the reference, a device failure prints one diagnostic line to
@@ -54,6 +57,9 @@ struct PageRankStats *pageRankPullFixedPoint64BitGraphCSR(struct Arguments *argu
 struct PageRankStats *pageRankPullFixedPoint32BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
 struct PageRankStats *pageRankPullFixedPoint16BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
 struct PageRankStats *pageRankPullFixedPoint8BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+struct PageRankStats *pageRankPullQuant32BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+struct PageRankStats *pageRankPullQuant16BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+struct PageRankStats *pageRankPullQuant8BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
 
 #ifdef __cplusplus
 }

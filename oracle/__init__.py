@@ -53,6 +53,8 @@ def lib() -> C.CDLL:
         L.or_pagerank_pull.argtypes = [u32, P(u32), P(u32), P(u32), P(u32), u32, u32, f64, u32,
                                        f64, u32, P(f32), C.c_int, P(f32), P(f32), P(u32), P(f64),
                                        P(_PrOut)]
+        L.or_quantize.restype = f32
+        L.or_quantize.argtypes = [u32, P(f32), u32, P(u64)]
         L.or_pagerank_f64.restype = C.c_int
         L.or_pagerank_f64.argtypes = [u32, P(u32), P(u32), P(u32), P(u32), f64, u32, C.c_int,
                                       P(f64)]
@@ -157,6 +159,18 @@ def pagerank_pull(outdeg, idx, deg, src, kind: int = 0, max_iterations: int = 20
         raise RuntimeError(f"or_pagerank_pull failed ({st})")
     return PrOut(ranks, scaled, o.iterations, o.executed, o.active, o.error_total, o.rank_sum,
                  ha[:o.executed], he[:o.executed])
+
+
+# kind codes of the quantised pulls (pageRank.c:2119-2676): 100 + code width
+KIND_QUANT32, KIND_QUANT16, KIND_QUANT8 = 132, 116, 108
+
+
+def quantize(c, bits: int):
+    """One iteration's quantisation of f32 contributions: (scale, u64 codes)."""
+    c = np.ascontiguousarray(c, np.float32)
+    codes = np.zeros(max(c.size, 1), np.uint64)
+    scale = lib().or_quantize(c.size, _p(c, f32), bits, _p(codes, u64))
+    return np.float32(scale), codes[:c.size]
 
 
 def pagerank_f64(outdeg, idx, deg, src, tol: float = 1e-14, max_iterations: int = 2000,

@@ -22,6 +22,8 @@
  *   or_gather_*        the bare sum kernel (cu_sum_kernel_control.sv), float and integer.
  *   or_build_csr       an independent sort-based CSR + inverse-CSR builder (the reference's
  *                      builder lives in the absent OpenGraph submodule).
+ *   or_quantize        the quantised pulls' per-iteration scale and codes (pageRank.c:2119-2676;
+ *                      or_pagerank_pull kinds 132/116/108 run those pulls end to end).
  *   or_fixed_*         the fixed-point formats, computed with integer arithmetic on the float's
  *                      mantissa/exponent rather than with floating-point scaling.
  *
@@ -164,6 +166,52 @@ typedef struct or_pr_out {
     double rank_sum;
 } or_pr_out;
 
+/* ------------------------------------------------------------------ quantised contributions
+ *
+ * pageRankPullQuant{32,16,8}BitGraphCSR (pageRank.c:2119-2303, 2305-2489, 2491-2676), per
+ * iteration: f32 contributions rank/out_degree (:2203-2210), their min and max and a scale
+ * (getMinMax_32 / GetScale_32, zero point 0: :2213-2216), codes quantize_32(c, scale, 0)
+ * (:2221-2226), an integer gather of the codes into u64 sums on the FPGA (:2234-2239), and
+ * the update base + damp * (scale * sum) in float (:2250-2253). quantization.h lives in the
+ * absent OpenGraph submodule, so the three quantiser functions are this library's own:
+ *   scale(B)  = mx > 0 ? (float)((double)mx / (2^B - 1)) : 1, mx = largest finite c > 0
+ *               (with the zero point fixed at 0 the code range starts at 0, so the range
+ *               that must fit is [0, mx]; the minimum takes no part);
+ *   code(c)   = 0 unless c > 0, else min(2^B - 1, floor((double)c * inv + 0.5)) with
+ *               inv = 1.0 / (double)scale (one double product and one double sum, rounded
+ *               to nearest each).
+ * Parity of these three against upstream OpenGraph is unpinned (its source is absent). */
+#define OR_QUANT_BASE 100u
+
+static float or_quant_scale(float mx, uint32_t bits)
+{
+    const double maxcode = (double)((1ull << bits) - 1ull);
+    return mx > 0.0f ? (float)((double)mx / maxcode) : 1.0f;
+}
+
+static uint64_t or_quant_code(float c, double inv, uint32_t bits)
+{
+    const double maxcode = (double)((1ull << bits) - 1ull);
+    if (!(c > 0.0f))
+        return 0;
+    const double t = floor((double)c * inv + 0.5);
+    return t >= maxcode ? (uint64_t)maxcode : (uint64_t)t;
+}
+
+/* scale and codes of one iteration's f32 contributions; returns the scale */
+float or_quantize(uint32_t n, const float *c, uint32_t bits, uint64_t *codes)
+{
+    float mx = 0.0f;
+    for (uint32_t v = 0; v < n; v++)
+        if (c[v] > mx && !isinf(c[v]))
+            mx = c[v];
+    const float scale = or_quant_scale(mx, bits);
+    const double inv = 1.0 / (double)scale;
+    for (uint32_t v = 0; v < n; v++)
+        codes[v] = or_quant_code(c[v], inv, bits);
+    return scale;
+}
+
 /* contribution of one vertex in the representation of `kind` (0 = f32, else fixed width) */
 static inline void contrib_store(uint32_t kind, void *c, uint32_t v, float rank, uint32_t od)
 {
@@ -204,16 +252,22 @@ int or_pagerank_pull(uint32_t n, const uint32_t *outdeg, const uint32_t *idx,
                      float *scaled_out, uint32_t *hist_active, double *hist_err,
                      or_pr_out *out)
 {
-    if (kind != 0 && kind != 64 && kind != 32 && kind != 16 && kind != 8)
+    const int quant = kind == OR_QUANT_BASE + 32 || kind == OR_QUANT_BASE + 16 ||
+                      kind == OR_QUANT_BASE + 8;
+    if (kind != 0 && kind != 64 && kind != 32 && kind != 16 && kind != 8 && !quant)
         return -1;
+    const uint32_t qbits = quant ? kind - OR_QUANT_BASE : 0;
     const float damp = OR_DAMP;
     const float base = 1.0f - damp;
-    size_t csize = kind == 0 ? 4 : kind / 8;
+    /* quantised kinds: f32 contributions, then u64 codes (any width fits) */
+    size_t csize = (kind == 0 || quant) ? 4 : kind / 8;
     float *rank = (float *)malloc((size_t)(n ? n : 1) * sizeof(float));
     void *contrib = malloc((size_t)(n ? n : 1) * csize);
-    if (!rank || !contrib) {
+    uint64_t *codes = quant ? (uint64_t *)malloc((size_t)(n ? n : 1) * sizeof(uint64_t)) : NULL;
+    if (!rank || !contrib || (quant && !codes)) {
         free(rank);
         free(contrib);
+        free(codes);
         return -2;
     }
     for (uint32_t v = 0; v < n; v++)
@@ -229,7 +283,8 @@ int or_pagerank_pull(uint32_t n, const uint32_t *outdeg, const uint32_t *idx,
         active = 0;
         #pragma omp parallel for schedule(static) if (threads > 0) num_threads(nt)
         for (uint32_t v = 0; v < n; v++)
-            contrib_store(kind, contrib, v, rank[v], outdeg[v]);
+            contrib_store(quant ? 0 : kind, contrib, v, rank[v], outdeg[v]);
+        const float qscale = quant ? or_quantize(n, (const float *)contrib, qbits, codes) : 0.0f;
         #pragma omp parallel for schedule(dynamic, 1024) if (threads > 0) num_threads(nt) reduction(+ : error_total, active)
         for (uint32_t v = 0; v < n; v++) {
             float fsum = 0.0f;
@@ -241,12 +296,16 @@ int or_pagerank_pull(uint32_t n, const uint32_t *outdeg, const uint32_t *idx,
                     float x = c[src[b + k]];
                     fsum = (k == 0) ? x : fsum + x;
                 }
+            } else if (quant) {
+                for (uint32_t k = 0; k < d; k++)
+                    qsum += codes[src[b + k]];
             } else {
                 for (uint32_t k = 0; k < d; k++)
                     qsum += contrib_load_q(kind, contrib, src[b + k]);
             }
             float prev = rank[v];
-            float next = update_rank(kind, base, damp, fsum, qsum);
+            float next = quant ? base + damp * (qscale * (float)qsum)
+                               : update_rank(kind, base, damp, fsum, qsum);
             rank[v] = next;
             double error = fabs((double)(next - prev));
             error_total += error / n;
@@ -280,6 +339,7 @@ int or_pagerank_pull(uint32_t n, const uint32_t *outdeg, const uint32_t *idx,
     out->rank_sum = sum;
     free(rank);
     free(contrib);
+    free(codes);
     return 0;
 }
 

@@ -261,6 +261,60 @@ def test_pagerank_fixed_bit_exact(device, rmat16, bits, pname):
     dg.close()
 
 
+QUANT = [ag.KIND_QUANT32, ag.KIND_QUANT16, ag.KIND_QUANT8]
+
+
+@pytest.mark.parametrize("kind", QUANT)
+@pytest.mark.parametrize("pname", ["default", "small"] + BLOCKED)
+def test_pagerank_quant_bit_exact(device, rmat16, kind, pname):
+    """Quantised pulls: exact u64 code sums, so every plan gives the oracle's ranks bit for bit
+    (iteration count, per-iteration active counts and ranks)."""
+    say(f"pagerank quant{kind - 100} {pname}")
+    g = rmat16
+    dg = upload(device, g, pname)
+    run = dg.pagerank(kind=kind, max_iterations=60, epsilon=1e-4, scaled=True)
+    ref = ref_pr(g, kind=kind, max_iterations=60, epsilon=1e-4)
+    assert run.iterations == ref.iterations and run.executed == ref.executed
+    np.testing.assert_array_equal(run.hist_active, ref.hist_active)
+    np.testing.assert_array_equal(run.scaled.view(np.uint32), ref.scaled.view(np.uint32))
+    np.testing.assert_array_equal(run.ranks.view(np.uint32), ref.ranks.view(np.uint32))
+    np.testing.assert_allclose(run.hist_err, ref.hist_err, rtol=1e-9, atol=1e-15)
+    dg.close()
+
+
+@pytest.mark.parametrize("kind", QUANT)
+def test_pagerank_quant_inits_and_corners(device, rmat12, kind):
+    """User start vectors with zero, negative and huge entries (codes 0 and saturation), the
+    uniform start with the L1 stop rule, and graphs whose contributions are all zero (scale 1)."""
+    say(f"pagerank quant{kind - 100} inits")
+    g = rmat12
+    rng = np.random.default_rng(kind)
+    init = rng.random(g.n).astype(np.float32) * 3
+    init[::7] = 0.0
+    init[1::11] = -2.5
+    init[5] = 1e30
+    dg = device.upload(g)
+    for kw in (dict(init_ranks=init, max_iterations=12, stop_rule=ag.STOP_NONE),
+               dict(init=ag.INIT_UNIFORM, stop_rule=ag.STOP_L1, l1_tolerance=1e-6,
+                    max_iterations=200)):
+        run = dg.pagerank(kind=kind, scaled=True, **kw)
+        okw = dict(kw)
+        if "init_ranks" in okw:
+            okw["init"] = 2
+        ref = ref_pr(g, kind=kind, **okw)
+        assert run.iterations == ref.iterations
+        np.testing.assert_array_equal(run.scaled.view(np.uint32), ref.scaled.view(np.uint32))
+    dg.close()
+    for src, dst, n in [([], [], 5), ([0, 1], [1, 0], 2), ([1] * 3, [0] * 3, 4)]:
+        h = ag.Graph.from_edges(src, dst, n=n, simplify=False)
+        dh = device.upload(h)
+        run = dh.pagerank(kind=kind, max_iterations=50)
+        ref = ref_pr(h, kind=kind, max_iterations=50)
+        assert run.iterations == ref.iterations
+        np.testing.assert_array_equal(run.ranks.view(np.uint32), ref.ranks.view(np.uint32))
+        dh.close()
+
+
 def test_pagerank_blocked_auto_rmat22(device):
     """Above 2^21 vertices the default plan blocks sources (2^19 per block). RMAT-22: fixed32
     bit-exact and FP32 within tolerance of the oracle for 20 iterations, and the blocked and
