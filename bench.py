@@ -38,7 +38,28 @@ def algorithmic_bytes(n: int, m: int, kind: str) -> int:
     # rank read 4N, rank write 4N, contribution write 4N
     if kind in ("bf16", "f16"):
         return 6 * m + 18 * n + 4
+    if kind in ("q32", "q16", "q8"):
+        # B-bit codes gathered per edge, f32 contributions written by the update, and the
+        # code pass: f32 contributions read, codes written
+        c = int(kind[1:]) // 8
+        return (4 + c) * m + 20 * n + 4 + (4 + c) * n
     return 8 * m + 20 * n + 4
+
+
+def int_kind_iteration(ag, dev, dg, m: int, kind: int, nbytes: int) -> dict:
+    """Kernel time of one full iteration of an integer-sum kind (no early stop)."""
+    s = dg.session(kind=kind, max_iterations=3 + 10, stop_rule=ag.STOP_NONE,
+                   init=ag.INIT_UNIFORM)
+    s.step(3)
+    dev.sync()
+    s.step(10)
+    dev.sync()
+    ms, _ = s.last_step_kernel_ms()
+    s.end()
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    return {"ms_per_iteration": round(ms, 4), "gteps": round(m / (ms * 1e-3) / 1e9, 2),
+            "algorithmic_bytes": nbytes, "gb_per_s": round(gbs, 1),
+            "roofline_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
 def cpu_baseline(g, budget_s: float) -> dict:
@@ -350,18 +371,13 @@ def main() -> int:
 
     # Q16.16 fixed-point iteration (the reference's pushpull 9 path: u32 contributions, exact
     # u64 sums), same graph, same algorithmic bytes
-    fsess = dg.session(kind=ag.KIND_FIXED32, max_iterations=3 + 10, stop_rule=ag.STOP_NONE,
-                       init=ag.INIT_UNIFORM)
-    fsess.step(3)
-    dev.sync()
-    fsess.step(10)
-    dev.sync()
-    fix_ms, _ = fsess.last_step_kernel_ms()
-    fsess.end()
-    fix_gbs = algorithmic_bytes(n, m, "f32") / (fix_ms * 1e-3) / 1e9
-    fixed32 = {"ms_per_iteration": round(fix_ms, 4), "gteps": round(m / (fix_ms * 1e-3) / 1e9, 2),
-               "gb_per_s": round(fix_gbs, 1), "roofline_frac": round(fix_gbs / HBM_PEAK_GBS, 4)}
+    fixed32 = int_kind_iteration(ag, dev, dg, m, ag.KIND_FIXED32, algorithmic_bytes(n, m, "f32"))
     log(f"fixed-32 iteration: {fixed32}")
+    # quantised iterations (pushpull 4/12/13): B-bit codes gathered, plus the code pass
+    quant = {f"quant{b}": int_kind_iteration(ag, dev, dg, m, kind,
+                                             algorithmic_bytes(n, m, f"q{b}"))
+             for b, kind in ((32, ag.KIND_QUANT32), (16, ag.KIND_QUANT16), (8, ag.KIND_QUANT8))}
+    log(f"quantised iterations: {quant}")
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
@@ -394,6 +410,7 @@ def main() -> int:
     }
 
     out["fixed32_iteration"] = fixed32
+    out["quantized_iteration"] = quant
     if construction:
         out["graph_construction"] = construction
     if not args.no_ttl:
