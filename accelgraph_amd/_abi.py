@@ -30,6 +30,7 @@ STOP_ACTIVE, STOP_L1, STOP_NONE = 0, 1, 2
 INIT_REFERENCE, INIT_UNIFORM, INIT_USER = 0, 1, 2
 KIND_F32, KIND_FIXED64, KIND_FIXED32, KIND_FIXED16, KIND_FIXED8 = 0, 64, 32, 16, 8
 KIND_QUANT32, KIND_QUANT16, KIND_QUANT8 = 132, 116, 108
+DIR_PULL, DIR_PUSH, DIR_DATA_DRIVEN = 0, 1, 2
 STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
 GATHER_F32, GATHER_U64, GATHER_U32, GATHER_U16, GATHER_U8 = 0, 1, 2, 3, 4
 PPR_UNIFORM = 0xFFFFFFFF
@@ -54,7 +55,7 @@ class PrOptions(C.Structure):
     _fields_ = [("kind", u32), ("max_iterations", u32), ("epsilon", f64), ("stop_rule", u32),
                 ("l1_tolerance", f64), ("init", u32), ("init_ranks", P(f32)),
                 ("storage", u32), ("refine_l1", f64), ("chunk", u32),
-                ("record_history", u32)]
+                ("record_history", u32), ("direction", u32)]
 
 
 class PrResult(C.Structure):
@@ -222,6 +223,9 @@ SIGNATURES = {
     "pageRankPullQuant32BitGraphCSR": (P(PageRankStats), [P(Arguments), P(GraphCSR)]),
     "pageRankPullQuant16BitGraphCSR": (P(PageRankStats), [P(Arguments), P(GraphCSR)]),
     "pageRankPullQuant8BitGraphCSR": (P(PageRankStats), [P(Arguments), P(GraphCSR)]),
+    "pageRankPushGraphCSR": (P(PageRankStats), [P(Arguments), P(GraphCSR)]),
+    "pageRankPushFixedPointGraphCSR": (P(PageRankStats), [P(Arguments), P(GraphCSR)]),
+    "pageRankDataDrivenPullGraphCSR": (P(PageRankStats), [P(Arguments), P(GraphCSR)]),
     # SPMV.h
     "newSPMVStatsGraphCSR": (P(SPMVStats), [P(GraphCSR)]),
     "freeSPMVStats": (None, [P(SPMVStats)]),

@@ -538,7 +538,102 @@ int rmat_functor(uint32_t scale, double a, double b, double c, RmatEdge *f)
     return AG_OK;
 }
 
+// Row id of every edge of a CSR: the last row r with off[r] <= j.
+__global__ __launch_bounds__(BUILD_WG) void row_ids_kernel(uint32_t n, uint32_t m,
+                                                           const uint32_t *off, uint32_t *row)
+{
+    for (uint32_t j = blockIdx.x * BUILD_WG + threadIdx.x; j < m; j += gridDim.x * BUILD_WG) {
+        uint32_t lo = 0, hi = n;   // off[lo] <= j < off[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (off[mid] <= j)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        row[j] = lo;
+    }
+}
+
+__global__ void last_offset_kernel(uint32_t n, uint32_t m, uint32_t *off)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        off[n] = m;
+}
+
 } // namespace
+
+// Forward CSR of an uploaded graph in device row ids: the inverse CSR (off/src) transposed
+// by one stable radix sort of (source, row) pairs, so each out-row lists its destinations in
+// ascending order. Used by the push and data-driven PageRank directions.
+int ag_graph_forward(ag_dgraph *g)
+{
+    if (g->fwd_off)
+        return AG_OK;
+    ag_device *dev = g->dev;
+    AG_HIP(hipSetDevice(dev->ordinal));
+    const uint32_t n = g->n, m = g->m;
+    uint32_t *off = NULL, *fsrc = NULL, *fdst = NULL;
+    Pool T{};
+    T.status = AG_OK;
+    int st = AG_OK;
+    hipError_t e = hipMalloc((void **)&off, ((size_t)n + 1) * sizeof(uint32_t));
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&fsrc, (size_t)(m ? m : 1) * sizeof(uint32_t));
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&fdst, (size_t)(m ? m : 1) * sizeof(uint32_t));
+    if (e != hipSuccess)
+        st = ag_hip_fail(e, "forward CSR allocation", __FILE__, __LINE__);
+    uint32_t *row = T.get<uint32_t>(m);
+    uint32_t *deg = T.get<uint32_t>(n);
+    if (st == AG_OK)
+        st = T.status;
+    if (st == AG_OK && m) {
+        row_ids_kernel<<<grid_of(m, BUILD_WG) < 8192u ? grid_of(m, BUILD_WG) : 8192u, BUILD_WG, 0,
+                         dev->stream>>>(n, m, g->off, row);
+        e = hipGetLastError();
+        size_t tmp = 0;
+        if (e == hipSuccess)
+            e = rocprim::radix_sort_pairs(nullptr, tmp, g->src, fsrc, row, fdst, (size_t)m, 0u,
+                                          bits_for(n), dev->stream);
+        void *t = NULL;
+        if (e == hipSuccess)
+            e = hipMalloc(&t, tmp ? tmp : 8);
+        if (e == hipSuccess)
+            e = rocprim::radix_sort_pairs(t, tmp, g->src, fsrc, row, fdst, (size_t)m, 0u,
+                                          bits_for(n), dev->stream);
+        if (e == hipSuccess)
+            st = ag_wait_stream(dev);
+        else
+            st = ag_hip_fail(e, "forward CSR sort", __FILE__, __LINE__);
+        if (t)
+            hipFree(t);
+    }
+    if (st == AG_OK && n) {
+        offsets_kernel<<<grid_of(n, BUILD_WG), BUILD_WG, 0, dev->stream>>>(n, m, fsrc, off, deg);
+        e = hipGetLastError();
+        if (e != hipSuccess)
+            st = ag_hip_fail(e, "forward CSR offsets", __FILE__, __LINE__);
+    }
+    if (st == AG_OK) {
+        last_offset_kernel<<<1, 1, 0, dev->stream>>>(n, m, off);
+        e = hipGetLastError();
+        st = (e == hipSuccess) ? ag_wait_stream(dev)
+                               : ag_hip_fail(e, "forward CSR offsets", __FILE__, __LINE__);
+    }
+    T.release();
+    if (st != AG_OK) {
+        hipFree(off);
+        hipFree(fsrc);
+        hipFree(fdst);
+        return st;
+    }
+    g->fwd_off = off;
+    g->fwd_src = fsrc;
+    g->fwd_dst = fdst;
+    g->device_bytes += ((size_t)n + 1 + 2 * (size_t)m) * sizeof(uint32_t);
+    return AG_OK;
+}
 
 // ---------------------------------------------------------------- C-ABI
 

@@ -49,6 +49,10 @@ struct ag_dgraph {
     ag_block_plan blk;   // pointer members are device pointers here
     uint64_t *vpart;     // [blk.nv] partial sum of every virtual row (float or u64 bits)
     uint32_t *vwq;       // [blk.mb] FloatToFixed32(vw), built on the first fixed SpMV
+    // forward CSR in device row ids (out-edges), built on the first push / data-driven run
+    uint32_t *fwd_off;   // [n+1]
+    uint32_t *fwd_src;   // [m] source of every out-edge (row id of the forward CSR)
+    uint32_t *fwd_dst;   // [m] destination, ascending within a row
     // scratch for permuted vectors (x, y) of SpMV / gather, sized lazily
     void *vec_a;
     void *vec_b;
@@ -105,7 +109,7 @@ struct PrState {
     uint32_t stall;         // 16-bit iterations since best_err last improved by 1%
     uint32_t qmax_bits;     // quantised kinds: bits of the largest next contribution
     float qscale;           // quantised kinds: scale of the codes being gathered
-    uint32_t pad;
+    uint32_t dd_count[2];   // data-driven pull: worklist lengths by iteration parity
 };
 
 // A PageRank session (ag_pr_begin .. ag_pr_end).
@@ -124,6 +128,9 @@ struct ag_pr_session {
     float *rank;
     void *c[2];
     void *c16[2];
+    void *next;                 // push: scattered sums [n] (f32 or u64)
+    uint32_t *dd_queue[2];      // data-driven pull: worklists by iteration parity
+    uint8_t *dd_flag;           // data-driven pull: next-worklist marks [n rounded up to 4]
     double *fin_part;
     double *fin_sum;
     float *user;
@@ -137,6 +144,10 @@ struct ag_pr_session {
 
 // Enqueue one PageRank iteration of a session (main + finish kernel).
 int ag_pr_launch_iteration(ag_pr_session *s);
+
+// Builds g->fwd_off / fwd_src / fwd_dst (the forward CSR in device row ids) from the uploaded
+// inverse CSR, once; later calls return at once.
+int ag_graph_forward(ag_dgraph *g);
 
 int ag_set_error(int status, const char *fmt, ...);
 int ag_hip_fail(hipError_t e, const char *what, const char *file, int line);

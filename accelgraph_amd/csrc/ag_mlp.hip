@@ -879,8 +879,10 @@ extern "C" int ag_pr_warm_start(ag_pr_session *s, ag_mlp *m, ag_warm_result *res
         return ag_set_error(AG_ERR_INVALID, "null argument");
     memset(res, 0, sizeof(*res));
     ag_dgraph *g = s->g;
-    if (s->kind_code != 0 || s->o.init != AG_INIT_UNIFORM || s->enqueued != 0)
-        return ag_set_error(AG_ERR_INVALID, "warm start needs a fresh FP32 session with the uniform init");
+    if (s->kind_code != 0 || s->o.init != AG_INIT_UNIFORM || s->enqueued != 0 ||
+        s->o.direction != AG_DIR_PULL)
+        return ag_set_error(AG_ERR_INVALID,
+                            "warm start needs a fresh FP32 pull session with the uniform init");
     if (s->o.max_iterations < 4)
         return ag_set_error(AG_ERR_INVALID, "warm start needs max_iterations >= 4");
     if (g->n == 0)

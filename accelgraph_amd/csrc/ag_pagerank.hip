@@ -568,6 +568,250 @@ __global__ __launch_bounds__(AG_FIN_WG) void pr_int_finish(PullGeom G, PrCtl C, 
     pr_close(C, iter, red);
 }
 
+// ---------------------------------------------------------------- push direction
+//
+// pageRankPushGraphCSR (pageRank.c:1111-1272) and pageRankPushFixedPointGraphCSR
+// (:1992-2117) keep the pull's host loop and form next[u] by scattering every source's
+// contribution along its out-edges. Here: an edge-parallel scatter over the forward CSR with
+// hardware atomics into `next` (FP32: the order of the additions is the atomics' order;
+// fixed point: exact u64 sums, equal to the pull's), then one pass over every row that reads
+// and clears next[r] and applies the kind's update, then the one-workgroup finish.
+
+template <int KC>
+__global__ __launch_bounds__(AG_WG) void pr_push_scatter_kernel(PrCtl C, PrF32Args A, uint32_t m,
+                                                                const uint32_t *fsrc,
+                                                                const uint32_t *fdst, void *next)
+{
+    const PrState *st = C.st;
+    if (*(volatile const uint32_t *)&st->done)
+        return;
+    const uint32_t rd = (*(volatile const uint32_t *)&st->iter) & 1u;
+    for (uint32_t j = blockIdx.x * AG_WG + threadIdx.x; j < m; j += gridDim.x * AG_WG) {
+        const uint32_t s = __builtin_nontemporal_load(&fsrc[j]);
+        const uint32_t d = __builtin_nontemporal_load(&fdst[j]);
+        if constexpr (KC == 0) {
+            unsafeAtomicAdd(&((float *)next)[d], ((const float *)A.c32[rd])[s]);
+        } else {
+            using CT = typename FixT<KC>::T;
+            atomicAdd(&((unsigned long long *)next)[d],
+                      (unsigned long long)((const CT *)A.c32[rd])[s]);
+        }
+    }
+}
+
+template <int KC>
+__global__ __launch_bounds__(AG_WG) void pr_push_update_kernel(PrCtl C, PrF32Args A, uint32_t n,
+                                                               void *next)
+{
+    const PrState *st = C.st;
+    if (*(volatile const uint32_t *)&st->done)
+        return;
+    const uint32_t iter = *(volatile const uint32_t *)&st->iter;
+    Red red{0.0, 0u};
+    if constexpr (KC == 0) {
+        const uint32_t rd = iter & 1u, wr = rd ^ 1u;
+        const PrF32Pol<float, float> p{(const float *)A.c32[rd], (float *)A.c32[wr], A.rank,
+                                       A.outdeg, A.base, A.damp, A.dn, A.eps};
+        float *nx = (float *)next;
+        for (uint32_t r = blockIdx.x * AG_WG + threadIdx.x; r < n; r += gridDim.x * AG_WG) {
+            const float sum = nx[r];
+            nx[r] = 0.0f;
+            p.finish(r, sum, red);
+        }
+    } else {
+        const PrFixPol<KC> p = fix_policy<KC>(iter, A);
+        uint64_t *nx = (uint64_t *)next;
+        for (uint32_t r = blockIdx.x * AG_WG + threadIdx.x; r < n; r += gridDim.x * AG_WG) {
+            const uint64_t sum = nx[r];
+            nx[r] = 0ull;
+            p.finish(r, sum, red);
+        }
+    }
+    pr_store_partial(C, red);
+}
+
+// Closes an iteration whose every row was updated by the preceding launch.
+__global__ __launch_bounds__(AG_FIN_WG) void pr_plain_finish(PrCtl C)
+{
+    const PrState *st = C.st;
+    if (*(volatile const uint32_t *)&st->done)
+        return;
+    const uint32_t iter = *(volatile const uint32_t *)&st->iter;
+    pr_close(C, iter, Red{0.0, 0u});
+}
+
+// ---------------------------------------------------------------- data-driven pull
+//
+// pageRankDataDrivenPullGraphCSR (pageRank.c:2810-2969): every vertex starts on the worklist;
+// per iteration the contributions rank/out_degree are those of the iteration's start
+// (:2870-2876), each listed vertex sums its in-edges, adds |delta|/N to the error, and only
+// if |delta| >= epsilon takes the new rank, counts as active and lists its out-neighbours for
+// the next iteration (:2888-2921); unlisted vertices keep their rank. Per iteration:
+//   pr_dd_gather  one 16-lane group per listed vertex: gather, update, mark out-neighbours
+//   pr_dd_commit  contributions of the listed vertices from their (possibly new) ranks
+//   pr_dd_compact next worklist from the marks (one atomic per workgroup), marks cleared
+//   pr_dd_finish  iteration closed, the consumed worklist's length cleared
+// Ranks change only for listed vertices, so the contribution vector stays exact between
+// iterations without a full pass; the worklists' order varies, the per-vertex values do not.
+// Rows of in-degree <= AG_DD_SEQ_MAX are summed in the reference's sequential order, so on
+// an unblocked plan (rows keep their edge order) every value matches the sequential
+// restatement bit for bit.
+
+#define AG_DD_LANES 16u
+// rows up to this in-degree are summed in edge order; longer rows in a lane-strided tree
+#define AG_DD_SEQ_MAX 8192u
+
+__global__ __launch_bounds__(AG_WG) void pr_dd_gather_kernel(PrCtl C, PrF32Args A,
+                                                             const uint32_t *off,
+                                                             const uint32_t *src,
+                                                             const uint32_t *fwd_off,
+                                                             const uint32_t *fwd_dst,
+                                                             const uint32_t *queue,
+                                                             uint8_t *flag)
+{
+    const PrState *st = C.st;
+    if (*(volatile const uint32_t *)&st->done)
+        return;
+    const uint32_t iter = *(volatile const uint32_t *)&st->iter;
+    const uint32_t cnt = *(volatile const uint32_t *)&st->dd_count[iter & 1u];
+    const float *c = (const float *)A.c32[0];
+    const uint32_t lane = threadIdx.x & (AG_DD_LANES - 1u);
+    const uint32_t groups = gridDim.x * (AG_WG / AG_DD_LANES);
+    Red red{0.0, 0u};
+    for (uint32_t i = (blockIdx.x * AG_WG + threadIdx.x) / AG_DD_LANES; i < cnt; i += groups) {
+        const uint32_t v = queue[i];
+        const uint32_t b = off[v], e = off[v + 1];
+        float sum = 0.0f;
+        if (e - b <= AG_DD_SEQ_MAX) {
+            // the 16 lanes load a chunk, then every lane adds it left to right: the
+            // reference's sequential order (pageRank.c:2888-2894), the same sum in every lane
+            for (uint32_t j0 = b; j0 < e; j0 += AG_DD_LANES) {
+                const uint32_t j = j0 + lane;
+                const float x = j < e ? c[src[j]] : 0.0f;
+                const uint32_t k = e - j0 < AG_DD_LANES ? e - j0 : AG_DD_LANES;
+                for (uint32_t l = 0; l < k; l++)
+                    sum = __fadd_rn(sum, __shfl(x, (int)l, (int)AG_DD_LANES));
+            }
+        } else {
+            for (uint32_t j = b + lane; j < e; j += AG_DD_LANES)
+                sum = __fadd_rn(sum, c[src[j]]);
+            sum = group_reduce<float, AG_DD_LANES>(sum);
+        }
+        const float nw = __fadd_rn(A.base, __fmul_rn(A.damp, sum));
+        const float old = A.rank[v];
+        const double err = (double)fabsf(__fsub_rn(nw, old));
+        const bool upd = err >= A.eps;
+        if (lane == 0) {
+            red.err += __ddiv_rn(err, A.dn);
+            if (upd) {
+                A.rank[v] = nw;
+                red.active += 1u;
+            }
+        }
+        if (upd) {
+            for (uint32_t j = fwd_off[v] + lane; j < fwd_off[v + 1]; j += AG_DD_LANES)
+                flag[fwd_dst[j]] = 1;
+        }
+    }
+    pr_store_partial(C, red);
+}
+
+__global__ __launch_bounds__(AG_WG) void pr_dd_commit_kernel(PrCtl C, PrF32Args A,
+                                                             const uint32_t *queue)
+{
+    const PrState *st = C.st;
+    if (*(volatile const uint32_t *)&st->done)
+        return;
+    const uint32_t iter = *(volatile const uint32_t *)&st->iter;
+    const uint32_t cnt = *(volatile const uint32_t *)&st->dd_count[iter & 1u];
+    float *c = (float *)A.c32[0];
+    for (uint32_t i = blockIdx.x * AG_WG + threadIdx.x; i < cnt; i += gridDim.x * AG_WG) {
+        const uint32_t v = queue[i];
+        const uint32_t od = A.outdeg[v];
+        c[v] = od ? __fdiv_rn(A.rank[v], __uint2float_rn(od)) : 0.0f;
+    }
+}
+
+// Marked vertices -> next worklist; each thread owns 4 consecutive marks (one u32 word).
+__global__ __launch_bounds__(AG_WG) void pr_dd_compact_kernel(PrState *st, uint32_t n,
+                                                              uint8_t *flag, uint32_t *queue_next)
+{
+    __shared__ uint32_t s_wave[AG_WG / AG_WAVE];
+    __shared__ uint32_t s_base;
+    if (*(volatile const uint32_t *)&st->done)
+        return;
+    const uint32_t nxt = ((*(volatile const uint32_t *)&st->iter) & 1u) ^ 1u;
+    const uint32_t words = (n + 3u) / 4u;
+    const uint32_t lane = threadIdx.x & (AG_WAVE - 1u), wave = threadIdx.x / AG_WAVE;
+    uint32_t *fw = (uint32_t *)flag;
+    for (uint32_t w0 = blockIdx.x * AG_WG; w0 < words; w0 += gridDim.x * AG_WG) {
+        const uint32_t w = w0 + threadIdx.x;
+        const uint32_t bits = w < words ? fw[w] : 0u;
+        uint32_t k = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            k += (bits >> (8 * q)) & 1u;
+        // exclusive prefix of k over the workgroup
+        uint32_t inc = k;
+#pragma unroll
+        for (uint32_t d = 1; d < AG_WAVE; d <<= 1) {
+            const uint32_t t = __shfl_up(inc, d, AG_WAVE);
+            if (lane >= d)
+                inc += t;
+        }
+        if (lane == AG_WAVE - 1u)
+            s_wave[wave] = inc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t tot = 0;
+            for (uint32_t q = 0; q < AG_WG / AG_WAVE; q++) {
+                const uint32_t t = s_wave[q];
+                s_wave[q] = tot;
+                tot += t;
+            }
+            s_base = tot ? atomicAdd(&st->dd_count[nxt], tot) : 0u;
+        }
+        __syncthreads();
+        uint32_t pos = s_base + s_wave[wave] + inc - k;
+        if (bits) {
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if ((bits >> (8 * q)) & 1u)
+                    queue_next[pos++] = 4u * w + (uint32_t)q;
+            fw[w] = 0u;
+        }
+        __syncthreads();
+    }
+}
+
+// Finish of a data-driven iteration: closes it and clears the consumed worklist's length,
+// which the compaction of the next iteration fills again.
+__global__ __launch_bounds__(AG_FIN_WG) void pr_dd_finish(PrCtl C)
+{
+    PrState *st = C.st;
+    if (*(volatile const uint32_t *)&st->done)
+        return;
+    const uint32_t iter = *(volatile const uint32_t *)&st->iter;
+    pr_close(C, iter, Red{0.0, 0u});
+    if (threadIdx.x == 0)
+        st->dd_count[iter & 1u] = 0u;
+}
+
+__global__ __launch_bounds__(AG_WG) void pr_dd_init_kernel(uint32_t n, uint32_t *queue0,
+                                                           uint32_t words, uint32_t *flag_words)
+{
+    for (uint32_t v = blockIdx.x * AG_WG + threadIdx.x; v < n; v += gridDim.x * AG_WG)
+        queue0[v] = v;
+    for (uint32_t w = blockIdx.x * AG_WG + threadIdx.x; w < words; w += gridDim.x * AG_WG)
+        flag_words[w] = 0u;
+}
+
+__global__ __launch_bounds__(AG_WG) void pr_zero_kernel(uint64_t *p, uint32_t count)
+{
+    for (uint32_t i = blockIdx.x * AG_WG + threadIdx.x; i < count; i += gridDim.x * AG_WG)
+        p[i] = 0ull;
+}
+
 // ---------------------------------------------------------------- init / finalise
 
 // kind: 0 = f32 contributions, 1 = bf16, 2 = f16, 64/32/16/8 = fixed point, 1xx = quantised
@@ -677,8 +921,8 @@ static void session_free(ag_pr_session *s)
     hipSetDevice(s->g->dev->ordinal);
     hipStreamSynchronize(s->g->dev->stream);
     void *ptrs[] = {s->st, s->part_err, s->part_act, s->part_cmax, s->hist_err, s->hist_act,
-                    s->rank, s->c[0], s->c[1], s->c16[0], s->c16[1], s->fin_part, s->fin_sum,
-                    s->user};
+                    s->rank, s->c[0], s->c[1], s->c16[0], s->c16[1], s->next, s->dd_queue[0],
+                    s->dd_queue[1], s->dd_flag, s->fin_part, s->fin_sum, s->user};
     for (void *p : ptrs)
         if (p)
             hipFree(p);
@@ -770,6 +1014,19 @@ extern "C" int ag_pr_begin(ag_dgraph *g, const ag_pr_options *opts, ag_pr_sessio
     }
     if (o.kind != AG_KIND_F32 && o.storage != AG_STORE_F32)
         return ag_set_error(AG_ERR_UNSUPPORTED, "16-bit storage applies to the FP32 kind only");
+    if (o.direction > AG_DIR_DATA_DRIVEN)
+        return ag_set_error(AG_ERR_INVALID, "unknown direction %u", o.direction);
+    if (o.direction == AG_DIR_PUSH && !(kc == 0 || kc == 64 || kc == 32 || kc == 16 || kc == 8))
+        return ag_set_error(AG_ERR_UNSUPPORTED,
+                            "the push direction takes FP32 (f32 storage) or a fixed-point kind");
+    if (o.direction == AG_DIR_DATA_DRIVEN && kc != 0)
+        return ag_set_error(AG_ERR_UNSUPPORTED,
+                            "the data-driven pull takes FP32 with f32 storage only");
+    if (o.direction != AG_DIR_PULL) {
+        const int fst = ag_graph_forward(g);
+        if (fst != AG_OK)
+            return fst;
+    }
     if (o.stop_rule > AG_STOP_NONE)
         return ag_set_error(AG_ERR_INVALID, "unknown stop rule %u", o.stop_rule);
     if (o.init > AG_INIT_USER || (o.init == AG_INIT_USER && !o.init_ranks && g->n))
@@ -813,12 +1070,20 @@ extern "C" int ag_pr_begin(ag_dgraph *g, const ag_pr_options *opts, ag_pr_sessio
         AG_TRY(salloc(s, &s->c16[0], (size_t)n * 2));
         AG_TRY(salloc(s, &s->c16[1], (size_t)n * 2));
     }
+    if (o.direction == AG_DIR_PUSH)
+        AG_TRY(salloc(s, &s->next, (size_t)n * sizeof(uint64_t)));
+    if (o.direction == AG_DIR_DATA_DRIVEN) {
+        AG_TRY(salloc(s, (void **)&s->dd_queue[0], (size_t)n * sizeof(uint32_t)));
+        AG_TRY(salloc(s, (void **)&s->dd_queue[1], (size_t)n * sizeof(uint32_t)));
+        AG_TRY(salloc(s, (void **)&s->dd_flag, (size_t)(n + 3u) / 4u * 4u));
+    }
     AG_TRY(salloc(s, (void **)&s->fin_part, (size_t)fblk * sizeof(double)));
     AG_TRY(salloc(s, (void **)&s->fin_sum, sizeof(double)));
 
     PrState init_st;
     memset(&init_st, 0, sizeof(init_st));
     init_st.refine_iter = -1;
+    init_st.dd_count[0] = n;   // data-driven pull: every vertex starts listed
     *s->st_host = init_st;
     {
         hipError_t e = hipMemcpyAsync(s->st, s->st_host, sizeof(PrState), hipMemcpyHostToDevice, stream);
@@ -845,6 +1110,11 @@ extern "C" int ag_pr_begin(ag_dgraph *g, const ag_pr_options *opts, ag_pr_sessio
                                                                      s->st);
             ag_launch_quant(s);
         }
+        if (s->next)
+            pr_zero_kernel<<<ag_quant_grid(g), AG_WG, 0, stream>>>((uint64_t *)s->next, n);
+        if (s->dd_queue[0])
+            pr_dd_init_kernel<<<ag_quant_grid(g), AG_WG, 0, stream>>>(
+                n, s->dd_queue[0], (n + 3u) / 4u, (uint32_t *)s->dd_flag);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) {
             session_free(s);
@@ -931,6 +1201,40 @@ int ag_pr_launch_iteration(ag_pr_session *s)
     A.eps = s->o.epsilon;
     const uint32_t lds = (s->kind_code >= 8) ? AG_TILE_NNZ_MAX * 8u : AG_TILE_NNZ_MAX * 4u;
     dim3 block(AG_WG), fin(AG_FIN_WG);
+    if (s->o.direction == AG_DIR_PUSH) {
+        const uint32_t gs = g->dev->num_cus * 16u, gu = ag_quant_grid(g);
+        C.nparts = gu;
+        switch (s->kind_code) {
+#define AG_PUSH(KC)                                                                              \
+    pr_push_scatter_kernel<KC><<<gs, block, 0, stream>>>(C, A, g->m, g->fwd_src, g->fwd_dst,   \
+                                                         s->next);                              \
+    pr_push_update_kernel<KC><<<gu, block, 0, stream>>>(C, A, g->n, s->next);                  \
+    break
+        case 0: AG_PUSH(0);
+        case 64: AG_PUSH(64);
+        case 32: AG_PUSH(32);
+        case 16: AG_PUSH(16);
+        default: AG_PUSH(8);
+#undef AG_PUSH
+        }
+        pr_plain_finish<<<1, fin, 0, stream>>>(C);
+        AG_HIP(hipGetLastError());
+        return AG_OK;
+    }
+    if (s->o.direction == AG_DIR_DATA_DRIVEN) {
+        const uint32_t gq = ag_quant_grid(g);
+        const uint32_t it = s->enqueued;   // parity of the iteration being enqueued
+        C.nparts = gq;
+        pr_dd_gather_kernel<<<gq, block, 0, stream>>>(C, A, g->off, g->src, g->fwd_off,
+                                                      g->fwd_dst, s->dd_queue[it & 1u],
+                                                      s->dd_flag);
+        pr_dd_commit_kernel<<<gq, block, 0, stream>>>(C, A, s->dd_queue[it & 1u]);
+        pr_dd_compact_kernel<<<gq, block, 0, stream>>>(s->st, g->n, s->dd_flag,
+                                                       s->dd_queue[(it & 1u) ^ 1u]);
+        pr_dd_finish<<<1, fin, 0, stream>>>(C);
+        AG_HIP(hipGetLastError());
+        return AG_OK;
+    }
     if (blocked) {
         const BlockGeom B = ag_make_block_geom(g);
         switch (s->kind_code) {

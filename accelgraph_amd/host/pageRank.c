@@ -50,8 +50,13 @@ void freePageRankStats(struct PageRankStats *stats)
     }
 }
 
-static const char *variant_title(uint32_t kind)
+static const char *variant_title(uint32_t kind, uint32_t direction)
 {
+    if (direction == AG_DIR_PUSH)
+        return kind == AG_KIND_F32 ? "Starting Page Rank Push (tolerance/epsilon)"
+                                   : "Starting Page Rank Push FP (tolerance/epsilon)";
+    if (direction == AG_DIR_DATA_DRIVEN)
+        return "Starting Page Rank Pull DD (tolerance/epsilon)";
     switch (kind) {
     case AG_KIND_FIXED64: return "Starting Page Rank Pull FP_64 (tolerance/epsilon)";
     case AG_KIND_FIXED32: return "Starting Page Rank Pull FP_32 (tolerance/epsilon)";
@@ -64,17 +69,17 @@ static const char *variant_title(uint32_t kind)
     }
 }
 
-/* Shared body of the FP32, fixed-point and quantised pulls (pageRank.c:942-1109,
- * 1276-1990, 2119-2676). */
-static struct PageRankStats *pagerank_pull_kind(struct Arguments *arguments,
-                                                struct GraphCSR *graph, uint32_t kind)
+/* Shared body of the FP32, fixed-point and quantised pulls, the pushes and the data-driven
+ * pull (pageRank.c:942-1109, 1111-1272, 1276-2117, 2119-2676, 2810-2969). */
+static struct PageRankStats *pagerank_run(struct Arguments *arguments, struct GraphCSR *graph,
+                                          uint32_t kind, uint32_t direction)
 {
     struct PageRankStats *stats = newPageRankStatsGraphCSR(graph);
 
     printf(" -----------------------------------------------------\n");
     printf("| %-51s | \n", "               ---->>> GPU <<<----");
     printf(" -----------------------------------------------------\n");
-    printf("| %-51s | \n", variant_title(kind));
+    printf("| %-51s | \n", variant_title(kind, direction));
     printf(" -----------------------------------------------------\n");
     printf("| %-51.13lf | \n", arguments->epsilon);
     printf(" -----------------------------------------------------\n");
@@ -85,6 +90,7 @@ static struct PageRankStats *pagerank_pull_kind(struct Arguments *arguments,
     ag_pr_options o;
     ag_pr_options_default(&o);
     o.kind = kind;
+    o.direction = direction;
     o.max_iterations = arguments->iterations;
     o.epsilon = arguments->epsilon;
     o.stop_rule = AG_STOP_ACTIVE;
@@ -147,60 +153,82 @@ static struct PageRankStats *pagerank_pull_kind(struct Arguments *arguments,
 /* pageRank.c:942-1109 */
 struct PageRankStats *pageRankPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
 {
-    return pagerank_pull_kind(arguments, graph, AG_KIND_F32);
+    return pagerank_run(arguments, graph, AG_KIND_F32, AG_DIR_PULL);
 }
 
 /* pageRank.c:1276-1453 */
 struct PageRankStats *pageRankPullFixedPoint64BitGraphCSR(struct Arguments *arguments,
                                                           struct GraphCSR *graph)
 {
-    return pagerank_pull_kind(arguments, graph, AG_KIND_FIXED64);
+    return pagerank_run(arguments, graph, AG_KIND_FIXED64, AG_DIR_PULL);
 }
 
 /* pageRank.c:1455-1633 */
 struct PageRankStats *pageRankPullFixedPoint32BitGraphCSR(struct Arguments *arguments,
                                                           struct GraphCSR *graph)
 {
-    return pagerank_pull_kind(arguments, graph, AG_KIND_FIXED32);
+    return pagerank_run(arguments, graph, AG_KIND_FIXED32, AG_DIR_PULL);
 }
 
 /* pageRank.c:1635-1811 */
 struct PageRankStats *pageRankPullFixedPoint16BitGraphCSR(struct Arguments *arguments,
                                                           struct GraphCSR *graph)
 {
-    return pagerank_pull_kind(arguments, graph, AG_KIND_FIXED16);
+    return pagerank_run(arguments, graph, AG_KIND_FIXED16, AG_DIR_PULL);
 }
 
 /* pageRank.c:1813-1990 */
 struct PageRankStats *pageRankPullFixedPoint8BitGraphCSR(struct Arguments *arguments,
                                                          struct GraphCSR *graph)
 {
-    return pagerank_pull_kind(arguments, graph, AG_KIND_FIXED8);
+    return pagerank_run(arguments, graph, AG_KIND_FIXED8, AG_DIR_PULL);
 }
 
 /* pageRank.c:2119-2303 */
 struct PageRankStats *pageRankPullQuant32BitGraphCSR(struct Arguments *arguments,
                                                      struct GraphCSR *graph)
 {
-    return pagerank_pull_kind(arguments, graph, AG_KIND_QUANT32);
+    return pagerank_run(arguments, graph, AG_KIND_QUANT32, AG_DIR_PULL);
 }
 
 /* pageRank.c:2305-2489 */
 struct PageRankStats *pageRankPullQuant16BitGraphCSR(struct Arguments *arguments,
                                                      struct GraphCSR *graph)
 {
-    return pagerank_pull_kind(arguments, graph, AG_KIND_QUANT16);
+    return pagerank_run(arguments, graph, AG_KIND_QUANT16, AG_DIR_PULL);
 }
 
 /* pageRank.c:2491-2676 */
 struct PageRankStats *pageRankPullQuant8BitGraphCSR(struct Arguments *arguments,
                                                     struct GraphCSR *graph)
 {
-    return pagerank_pull_kind(arguments, graph, AG_KIND_QUANT8);
+    return pagerank_run(arguments, graph, AG_KIND_QUANT8, AG_DIR_PULL);
 }
 
-/* pageRank.c:874-939. The pull directions are provided; the push and data-driven variants
- * (pushpull 1,3,5-8) are outside this library. */
+/* pageRank.c:1111-1272 */
+struct PageRankStats *pageRankPushGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    return pagerank_run(arguments, graph, AG_KIND_F32, AG_DIR_PUSH);
+}
+
+/* pageRank.c:1992-2117 (64-bit fixed-point contributions and sums, as the pull at :1276) */
+struct PageRankStats *pageRankPushFixedPointGraphCSR(struct Arguments *arguments,
+                                                     struct GraphCSR *graph)
+{
+    return pagerank_run(arguments, graph, AG_KIND_FIXED64, AG_DIR_PUSH);
+}
+
+/* pageRank.c:2810-2969 */
+struct PageRankStats *pageRankDataDrivenPullGraphCSR(struct Arguments *arguments,
+                                                     struct GraphCSR *graph)
+{
+    return pagerank_run(arguments, graph, AG_KIND_F32, AG_DIR_DATA_DRIVEN);
+}
+
+/* pageRank.c:874-939. Not provided: the quantised push (5: a zero-point quantiser with float
+ * atomic sums), the data-driven push and pull-push (7, 8). Those three update shared
+ * residuals without ordering in the reference, so their results depend on the thread
+ * schedule and have no fixed answer to reproduce. */
 struct PageRankStats *pageRankGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
 {
     switch (arguments->pushpull) {
@@ -214,13 +242,19 @@ struct PageRankStats *pageRankGraphCSR(struct Arguments *arguments, struct Graph
         return pageRankPullFixedPoint16BitGraphCSR(arguments, graph);
     case 11:
         return pageRankPullFixedPoint8BitGraphCSR(arguments, graph);
+    case 1:
+        return pageRankPushGraphCSR(arguments, graph);
+    case 3:
+        return pageRankPushFixedPointGraphCSR(arguments, graph);
+    case 6:
+        return pageRankDataDrivenPullGraphCSR(arguments, graph);
     case 4:
         return pageRankPullQuant32BitGraphCSR(arguments, graph);
     case 12:
         return pageRankPullQuant16BitGraphCSR(arguments, graph);
     case 13:
         return pageRankPullQuant8BitGraphCSR(arguments, graph);
-    case 1: case 3: case 5: case 6: case 7: case 8:
+    case 5: case 7: case 8:
         fprintf(stderr, "[accelgraph] pageRankGraphCSR: pushpull=%u is not a pull variant "
                         "provided by this library\n", arguments->pushpull);
         exit(EXIT_FAILURE);

@@ -183,7 +183,21 @@ typedef struct ag_pr_options {
     double refine_l1;          /* 16-bit storage: switch to f32 once error_total <= this  */
     uint32_t chunk;            /* iterations enqueued between convergence polls (0=auto)  */
     uint32_t record_history;   /* keep per-iteration active/error/time                    */
+    uint32_t direction;        /* AG_DIR_*                                                */
 } ag_pr_options;
+
+/* Direction of the sum (pageRankGraphCSR's pushpull codes, pageRank.c:874-939):
+ *   PULL         gather over in-edges (codes 0, 2, 4, 9-13)
+ *   PUSH         scatter every contribution along the out-edges with atomics, then update
+ *                every vertex (codes 1 and 3, pageRank.c:1111-1272, 1992-2117); AG_KIND_F32
+ *                with f32 storage or a fixed-point kind. FP32 sums take the atomics' order;
+ *                fixed-point sums are exact and equal the pull's
+ *   DATA_DRIVEN  pull over a worklist (code 6, pageRank.c:2810-2969): only listed vertices
+ *                gather; a vertex whose |delta| >= epsilon takes its new rank and lists its
+ *                out-neighbours for the next iteration, others keep their rank; AG_KIND_F32
+ *                with f32 storage only
+ * PUSH and DATA_DRIVEN build the graph's forward CSR on the device on first use. */
+enum { AG_DIR_PULL = 0, AG_DIR_PUSH = 1, AG_DIR_DATA_DRIVEN = 2 };
 
 void ag_pr_options_default(ag_pr_options *o);
 

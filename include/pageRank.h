@@ -15,6 +15,9 @@
  *   pageRankPullQuant32BitGraphCSR      pageRank.h:79     (pageRank.c:2119-2303)
  *   pageRankPullQuant16BitGraphCSR      pageRank.h:80     (pageRank.c:2305-2489)
  *   pageRankPullQuant8BitGraphCSR       pageRank.h:81     (pageRank.c:2491-2676)
+ *   pageRankPushGraphCSR                pageRank.h:71     (pageRank.c:1111-1272)
+ *   pageRankPushFixedPointGraphCSR      pageRank.h:77     (pageRank.c:1992-2117)
+ *   pageRankDataDrivenPullGraphCSR      pageRank.h:84     (pageRank.c:2810-2969)
  *
  * The gather and the per-vertex update run on the GPU through the accelgraph C-ABI
  * (accelgraph.h). As in the reference, a device failure prints one diagnostic line to
@@ -60,6 +63,9 @@ struct PageRankStats *pageRankPullFixedPoint8BitGraphCSR(struct Arguments *argum
 struct PageRankStats *pageRankPullQuant32BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
 struct PageRankStats *pageRankPullQuant16BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
 struct PageRankStats *pageRankPullQuant8BitGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+struct PageRankStats *pageRankPushGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+struct PageRankStats *pageRankPushFixedPointGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+struct PageRankStats *pageRankDataDrivenPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
 
 #ifdef __cplusplus
 }

@@ -14,6 +14,7 @@
  *                      update base + damp*next, err += |delta|/N, active += |delta| >= eps
  *                      (:1055-1069), stop on active == 0 (:1072-1074), ranks /= N and an FP64
  *                      sum (:1079-1085).
+ *   or_pagerank_dd_pull  the data-driven (worklist) pull, pageRank.c:2810-2969.
  *   or_pagerank_f64    the same recurrence in FP64, iterated to a tight tolerance: the
  *                      double-precision reference every FP32 result is measured against.
  *   or_spmv_f32/fixed  SPMV.c:622-751 / :849-1016 with the FPGA datapath arithmetic
@@ -340,6 +341,99 @@ int or_pagerank_pull(uint32_t n, const uint32_t *outdeg, const uint32_t *idx,
     free(rank);
     free(contrib);
     free(codes);
+    return 0;
+}
+
+/* pageRankDataDrivenPullGraphCSR (pageRank.c:2810-2969), sequentially. Every vertex starts
+ * on the worklist (:2849-2856). Per iteration the contributions rank/out_degree are taken from
+ * the ranks at the iteration's start (:2870-2876); each listed vertex sums its in-edges in
+ * inverse-CSR order (:2888-2894), adds |new - old| / N to the error (:2896-2899) and, only when
+ * |new - old| >= eps, takes the new rank, counts as active and lists its out-neighbours for
+ * the next iteration (:2901-2912). The loop stops when no vertex was active (:2922-2923).
+ * The contributions are fixed for the whole iteration and each vertex writes only its own
+ * rank, so the per-vertex values do not depend on the order vertices are visited in. */
+int or_pagerank_dd_pull(uint32_t n, const uint32_t *outdeg, const uint32_t *idx,
+                        const uint32_t *deg, const uint32_t *src, const uint32_t *fwd_idx,
+                        const uint32_t *fwd_deg, const uint32_t *fwd_dst, uint32_t max_iter,
+                        double eps, uint32_t init, const float *init_ranks, float *ranks_out,
+                        float *scaled_out, uint32_t *hist_active, double *hist_err,
+                        or_pr_out *out)
+{
+    const float damp = OR_DAMP;
+    const float base = 1.0f - damp;
+    const size_t nn = n ? n : 1;
+    float *rank = (float *)malloc(nn * sizeof(float));
+    float *contrib = (float *)malloc(nn * sizeof(float));
+    uint8_t *work = (uint8_t *)malloc(nn);
+    uint8_t *next_work = (uint8_t *)malloc(nn);
+    if (!rank || !contrib || !work || !next_work) {
+        free(rank);
+        free(contrib);
+        free(work);
+        free(next_work);
+        return -2;
+    }
+    for (uint32_t v = 0; v < n; v++) {
+        rank[v] = init == OR_INIT_REFERENCE ? base : init == OR_INIT_UNIFORM ? 1.0f : init_ranks[v];
+        work[v] = 1;
+        next_work[v] = 0;
+    }
+    memset(out, 0, sizeof(*out));
+    uint32_t it, active = 0;
+    double error_total = 0.0;
+    int stopped = 0;
+    for (it = 0; it < max_iter; it++) {
+        error_total = 0.0;
+        active = 0;
+        for (uint32_t v = 0; v < n; v++)
+            contrib[v] = outdeg[v] ? rank[v] / (float)outdeg[v] : 0.0f;
+        for (uint32_t v = 0; v < n; v++) {
+            if (!work[v])
+                continue;
+            float sum = 0.0f;
+            for (uint32_t k = 0; k < deg[v]; k++)
+                sum += contrib[src[idx[v] + k]];
+            const float next = base + damp * sum;
+            const double error = fabs((double)(next - rank[v]));
+            error_total += error / n;
+            if (error >= eps) {
+                rank[v] = next;
+                for (uint32_t k = 0; k < fwd_deg[v]; k++)
+                    next_work[fwd_dst[fwd_idx[v] + k]] = 1;
+                active++;
+            }
+        }
+        uint8_t *t = work;
+        work = next_work;
+        next_work = t;
+        memset(next_work, 0, nn);
+        if (hist_active)
+            hist_active[it] = active;
+        if (hist_err)
+            hist_err[it] = error_total;
+        if (active == 0) {
+            stopped = 1;
+            break;
+        }
+    }
+    out->iterations = it;
+    out->executed = stopped ? it + 1 : it;
+    out->active = active;
+    out->error_total = error_total;
+    double sum = 0.0;
+    for (uint32_t v = 0; v < n; v++) {
+        if (scaled_out)
+            scaled_out[v] = rank[v];
+        const float r = rank[v] / n;
+        if (ranks_out)
+            ranks_out[v] = r;
+        sum += r;
+    }
+    out->rank_sum = sum;
+    free(rank);
+    free(contrib);
+    free(work);
+    free(next_work);
     return 0;
 }
 

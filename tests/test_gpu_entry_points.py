@@ -1,7 +1,9 @@
 """The drop-in entry points themselves (pageRank.h, SPMV.h), called as the reference's callers
 call them: a struct Arguments and a struct GraphCSR in, a stats record out. Every dispatch
 code of pageRankGraphCSR (pageRank.c:874-939) and SPMVGraphCSR (SPMV.c:594-621) that this
-library provides is checked against the oracle: integer kinds bit-exact, FP32 within 1e-5."""
+library provides is checked against the oracle: integer kinds bit-exact, FP32 within 1e-5.
+The codes this library does not provide (5, 7, 8) print a message and exit the process, so
+they are not called here."""
 import ctypes as C
 import sys
 
@@ -46,7 +48,10 @@ PR_CODES = {0: (0, "pageRankPullGraphCSR"),
             11: (8, "pageRankPullFixedPoint8BitGraphCSR"),
             4: (oracle.KIND_QUANT32, "pageRankPullQuant32BitGraphCSR"),
             12: (oracle.KIND_QUANT16, "pageRankPullQuant16BitGraphCSR"),
-            13: (oracle.KIND_QUANT8, "pageRankPullQuant8BitGraphCSR")}
+            13: (oracle.KIND_QUANT8, "pageRankPullQuant8BitGraphCSR"),
+            1: (0, "pageRankPushGraphCSR"),
+            3: (64, "pageRankPushFixedPointGraphCSR"),
+            6: ("dd", "pageRankDataDrivenPullGraphCSR")}
 
 
 @pytest.mark.parametrize("pushpull", list(PR_CODES))
@@ -55,21 +60,29 @@ def test_pagerank_entry_points(graph, pushpull):
     say(f"pageRankGraphCSR pushpull={pushpull} ({direct})")
     L = ag.lib()
     g = graph
-    ref = oracle.pagerank_pull(g.out_degree, g.inv_idx, g.in_degree, g.inv_src, kind=kind,
-                               max_iterations=100, epsilon=1e-4)
+    if kind == "dd":
+        # summed in the reference's edge order on this unblocked graph: exact
+        ref = oracle.pagerank_dd_pull(g.out_degree, g.inv_idx, g.in_degree, g.inv_src,
+                                      g.edges_idx, g.out_degree, g.fwd_dest,
+                                      max_iterations=100, epsilon=1e-4)
+    else:
+        ref = oracle.pagerank_pull(g.out_degree, g.inv_idx, g.in_degree, g.inv_src, kind=kind,
+                                   max_iterations=100, epsilon=1e-4)
     for fn in (L.pageRankGraphCSR, getattr(L, direct)):
         a = arguments(pushpull)
         st = fn(C.byref(a), g.ptr)
         s = st.contents
         ranks = np.ctypeslib.as_array(s.pageRanks, shape=(g.n,)).copy()
         assert s.num_vertices == g.n
-        if kind == 0:
-            # the FP32 bar of the parity suite: rows summed in a tree order may move the
-            # active-rule stop by an iteration or two, ranks agree to 1e-5
-            assert abs(int(s.iterations) - int(ref.iterations)) <= 3
+        if kind == 0 and pushpull != 6:
+            # the FP32 bar of the parity suite: rows summed in a tree order (pull) or in the
+            # atomics' order (push) may move the active-rule stop by a few iterations, ranks
+            # agree to 1e-5
+            assert abs(int(s.iterations) - int(ref.iterations)) <= (3 if pushpull == 0 else 8)
             assert np.abs(ranks - ref.ranks).max() <= FP32_TOL
             np.testing.assert_allclose(ranks, ref.ranks, rtol=5e-5)
-            assert abs(s.error_total - ref.error_total) <= 1e-3 * ref.error_total + 1e-7
+            if s.iterations == ref.iterations:   # same final iteration: same L1 delta
+                assert abs(s.error_total - ref.error_total) <= 1e-3 * ref.error_total + 1e-7
         else:
             assert s.iterations == ref.iterations and s.activeVertices == ref.active
             np.testing.assert_array_equal(ranks.view(np.uint32), ref.ranks.view(np.uint32))

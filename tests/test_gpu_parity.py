@@ -315,6 +315,123 @@ def test_pagerank_quant_inits_and_corners(device, rmat12, kind):
         dh.close()
 
 
+# ------------------------------------------------------------------ push / data-driven
+
+@pytest.mark.parametrize("kind", [64, 32, 16, 8])
+@pytest.mark.parametrize("pname", ["default", "blocked-all"])
+def test_pagerank_push_fixed_bit_exact(device, rmat16, kind, pname):
+    """Push direction, fixed point: atomically scattered u64 sums are exact, so the ranks equal
+    the pull oracle's bit for bit (the plan only relabels rows: the forward CSR is built in
+    device row ids for each)."""
+    say(f"pagerank push fixed{kind} {pname}")
+    g = rmat16
+    dg = upload(device, g, pname)
+    run = dg.pagerank(kind=kind, direction=ag.DIR_PUSH, max_iterations=60, epsilon=1e-4,
+                      scaled=True)
+    ref = ref_pr(g, kind=kind, max_iterations=60, epsilon=1e-4)
+    assert run.iterations == ref.iterations and run.executed == ref.executed
+    np.testing.assert_array_equal(run.hist_active, ref.hist_active)
+    np.testing.assert_array_equal(run.ranks.view(np.uint32), ref.ranks.view(np.uint32))
+    dg.close()
+
+
+def _fp32_bar(run, ref, iteration_slack=3):
+    assert abs(run.iterations - ref.iterations) <= iteration_slack
+    assert np.abs(run.ranks - ref.ranks).max() <= FP32_TOL
+    np.testing.assert_allclose(run.ranks, ref.ranks, rtol=5e-5)
+
+
+@pytest.mark.parametrize("pname", ["default", "small", "blocked"])
+def test_pagerank_push_f32(device, rmat16, pname):
+    """Push direction, FP32: the atomics add in an order of their own, different on every
+    run, so the bar is the FP32 one. With a fixed iteration count the ranks are compared
+    directly; under the reference's stop rule the last few vertices at the epsilon threshold
+    (ulp-sized deltas) can keep the run going a few iterations longer or shorter."""
+    say(f"pagerank push f32 {pname}")
+    g = rmat16
+    dg = upload(device, g, pname)
+    run = dg.pagerank(direction=ag.DIR_PUSH, max_iterations=40, stop_rule=ag.STOP_NONE)
+    ref = ref_pr(g, max_iterations=40, stop_rule=2)
+    _fp32_bar(run, ref, iteration_slack=0)
+    run = dg.pagerank(direction=ag.DIR_PUSH, max_iterations=100, epsilon=1e-4)
+    ref = ref_pr(g, max_iterations=100, epsilon=1e-4)
+    _fp32_bar(run, ref, iteration_slack=8)
+    f64 = f64_ref(g)
+    assert np.abs(run.ranks - f64).max() <= 2 * np.abs(ref.ranks - f64).max() + 1e-9
+    dg.close()
+
+
+def dd_ref(g, **kw):
+    return oracle.pagerank_dd_pull(g.out_degree, g.inv_idx, g.in_degree, g.inv_src,
+                                   g.edges_idx, g.out_degree, g.fwd_dest, **kw)
+
+
+@pytest.mark.parametrize("pname", ["default", "small", "blocked-all"])
+def test_pagerank_data_driven(device, rmat16, pname):
+    """Data-driven pull against the oracle's. Unblocked plans keep every row's edge order and
+    rows up to in-degree 8192 (all of RMAT-16's) are summed in that order, so the run is the
+    oracle's bit for bit; a blocked plan reorders row edges by source block, so it gets the
+    FP32 bar. Either way the stopping state satisfies the invariant: one more evaluation of
+    any vertex moves it by less than epsilon."""
+    say(f"pagerank data-driven {pname}")
+    g = rmat16
+    assert g.in_degree.max() <= 8192
+    dg = upload(device, g, pname)
+    eps = 1e-4
+    run = dg.pagerank(direction=ag.DIR_DATA_DRIVEN, max_iterations=300, epsilon=eps,
+                      scaled=True)
+    ref = dd_ref(g, max_iterations=300, epsilon=eps)
+    if pname in BLOCKED:
+        assert abs(run.iterations - ref.iterations) <= 10
+        assert np.abs(run.ranks - ref.ranks).max() <= FP32_TOL
+    else:
+        assert run.iterations == ref.iterations and run.executed == ref.executed
+        np.testing.assert_array_equal(run.hist_active, ref.hist_active)
+        np.testing.assert_array_equal(run.scaled.view(np.uint32), ref.scaled.view(np.uint32))
+        np.testing.assert_allclose(run.hist_err, ref.hist_err, rtol=1e-9, atol=1e-15)
+    assert run.executed == run.iterations + 1 and run.hist_active[-1] == 0
+    one = oracle.pagerank_pull(g.out_degree, g.inv_idx, g.in_degree, g.inv_src,
+                               max_iterations=1, stop_rule=2, init=2, init_ranks=run.scaled)
+    slack = 4e-7 * max(1.0, float(np.abs(run.scaled).max()))
+    assert np.abs(one.scaled.astype(np.float64) - run.scaled).max() < eps + slack
+    # epsilon 0: every listed vertex updates, and the ranks are the plain pull's
+    run = dg.pagerank(direction=ag.DIR_DATA_DRIVEN, max_iterations=12, epsilon=0.0)
+    ref = ref_pr(g, max_iterations=12, stop_rule=2)
+    assert np.abs(run.ranks - ref.ranks).max() <= FP32_TOL
+    np.testing.assert_allclose(run.ranks, ref.ranks, rtol=5e-5)
+    dg.close()
+
+
+def test_pagerank_directions_corners(device):
+    say("push / data-driven corner cases")
+    for src, dst, n in [([], [], 1), ([], [], 6), ([0], [1], 2), ([0, 1, 2], [1, 2, 0], 3),
+                        ([1] * 5, [0] * 5, 7), ([0, 0, 0, 0], [1, 2, 3, 4], 5)]:
+        g = ag.Graph.from_edges(src, dst, n=n, simplify=False)
+        dg = device.upload(g)
+        run = dg.pagerank(direction=ag.DIR_PUSH, kind=32, max_iterations=100)
+        ref = ref_pr(g, kind=32, max_iterations=100)
+        assert run.iterations == ref.iterations
+        np.testing.assert_array_equal(run.ranks.view(np.uint32), ref.ranks.view(np.uint32))
+        run = dg.pagerank(direction=ag.DIR_PUSH, max_iterations=100)
+        ref = ref_pr(g, max_iterations=100)
+        assert run.iterations == ref.iterations
+        np.testing.assert_allclose(run.ranks, ref.ranks, rtol=1e-6)
+        run = dg.pagerank(direction=ag.DIR_DATA_DRIVEN, max_iterations=100)
+        ref = dd_ref(g, max_iterations=100)
+        assert run.iterations == ref.iterations
+        np.testing.assert_allclose(run.ranks, ref.ranks, rtol=1e-6)
+        dg.close()
+    g = ag.Graph.rmat(8)
+    dg = device.upload(g)
+    for kw in (dict(direction=ag.DIR_PUSH, kind=ag.KIND_QUANT16),
+               dict(direction=ag.DIR_PUSH, storage=ag.STORE_BF16),
+               dict(direction=ag.DIR_DATA_DRIVEN, kind=32),
+               dict(direction=7)):
+        with pytest.raises(ag.AGError):
+            dg.pagerank(max_iterations=3, **kw)
+    dg.close()
+
+
 def test_pagerank_blocked_auto_rmat22(device):
     """Above 2^21 vertices the default plan blocks sources (2^19 per block). RMAT-22: fixed32
     bit-exact and FP32 within tolerance of the oracle for 20 iterations, and the blocked and

@@ -261,3 +261,63 @@ def test_fixed_decode_exact(bits, frac):
             # round the integer to binary32 (nearest even), then scale exactly
             f32 = np.uint32(f32_round(Fraction(int(q)))).view(np.float32)
             assert g == float(f32) / (1 << frac)
+
+
+# ------------------------------------------------------------------ data-driven pull
+
+def _dd(g, **kw):
+    return oracle.pagerank_dd_pull(g.out_degree, g.inv_idx, g.in_degree, g.inv_src,
+                                   g.edges_idx, g.out_degree, g.fwd_dest, **kw)
+
+
+def test_dd_pull_with_zero_epsilon_is_the_pull():
+    """With epsilon 0 every listed vertex takes its new rank and lists its out-neighbours. A
+    vertex left off the list has in-neighbours that did not change since it was last
+    evaluated, so re-evaluating it would give its current rank: the ranks equal the plain
+    pull's bit for bit after every number of iterations."""
+    import accelgraph_amd as ag
+    g = ag.Graph.rmat(11)
+    for k in (1, 2, 5, 17):
+        d = _dd(g, max_iterations=k, epsilon=0.0)
+        p = oracle.pagerank_pull(g.out_degree, g.inv_idx, g.in_degree, g.inv_src,
+                                 max_iterations=k, stop_rule=2)
+        np.testing.assert_array_equal(d.scaled.view(np.uint32), p.scaled.view(np.uint32))
+
+
+def test_dd_pull_star_and_cycle():
+    import accelgraph_amd as ag
+    # star: the centre (no in-edges) never changes, the leaves change once, then nothing is
+    # listed: stop at iteration 1 with the pull's ranks
+    g = ag.Graph.from_edges([0, 0, 0, 0], [1, 2, 3, 4])
+    d = _dd(g, max_iterations=100)
+    assert d.iterations == 1 and d.active == 0
+    np.testing.assert_allclose(d.scaled[1:], 0.15 + 0.85 * 0.15 / 4, rtol=1e-6)
+    assert d.scaled[0] == np.float32(1) - np.float32(0.85)
+    # cycle: every vertex sees the same delta, so all update until the delta drops below eps;
+    # the last (sub-epsilon) update is not applied
+    g = ag.Graph.from_edges([0, 1, 2], [1, 2, 0])
+    d = _dd(g, max_iterations=200, epsilon=1e-3)
+    p = oracle.pagerank_pull(g.out_degree, g.inv_idx, g.in_degree, g.inv_src,
+                             max_iterations=200, epsilon=1e-3)
+    assert d.iterations == p.iterations == 30
+    np.testing.assert_allclose(d.scaled, 1 - 0.85 ** (d.iterations + 1), rtol=1e-6)
+
+
+def test_dd_pull_final_ranks_are_epsilon_fixed_points():
+    """When the worklist empties, re-evaluating any vertex moves it by less than epsilon: a
+    listed vertex was re-evaluated in the last iteration and did not move by epsilon, and an
+    unlisted one has unchanged in-neighbours since its last evaluation."""
+    import accelgraph_amd as ag
+    g = ag.Graph.rmat(12)
+    for eps in (1e-3, 1e-5):
+        d = _dd(g, max_iterations=500, epsilon=eps)
+        assert d.executed == d.iterations + 1 and d.active == 0
+        one = oracle.pagerank_pull(g.out_degree, g.inv_idx, g.in_degree, g.inv_src,
+                                   max_iterations=1, stop_rule=2, init=2,
+                                   init_ranks=d.scaled)
+        assert np.abs(one.scaled.astype(np.float64) - d.scaled).max() < eps
+        f64 = oracle.pagerank_f64(g.out_degree, g.inv_idx, g.in_degree, g.inv_src,
+                                  tol=1e-15, max_iterations=3000)
+        # e = r + 0.85 P e with |r_v| < eps and ||P||_1 <= 1 (dangling mass is dropped):
+        # sum |e| <= N eps / 0.15 in scaled units, i.e. eps / 0.15 normalised
+        assert np.abs(d.ranks.astype(np.float64) - f64).sum() <= eps / 0.15 * 1.001
