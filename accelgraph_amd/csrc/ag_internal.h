@@ -110,6 +110,7 @@ struct PrState {
     uint32_t qmax_bits;     // quantised kinds: bits of the largest next contribution
     float qscale;           // quantised kinds: scale of the codes being gathered
     uint32_t dd_count[2];   // data-driven pull: worklist lengths by iteration parity
+    uint32_t dd_hubs[2];    // data-driven pull: hub rows listed at the worklist's far end
 };
 
 // A PageRank session (ag_pr_begin .. ag_pr_end).

@@ -647,21 +647,22 @@ __global__ __launch_bounds__(AG_FIN_WG) void pr_plain_finish(PrCtl C)
 // (:2870-2876), each listed vertex sums its in-edges, adds |delta|/N to the error, and only
 // if |delta| >= epsilon takes the new rank, counts as active and lists its out-neighbours for
 // the next iteration (:2888-2921); unlisted vertices keep their rank. Per iteration:
-//   pr_dd_gather  one 16-lane group per listed vertex: gather, update, mark out-neighbours
+//   pr_dd_gather  one workgroup per listed hub row (in-degree > AG_DD_SEQ_MAX), one 16-lane
+//                 group per other listed vertex: gather, update, mark out-neighbours
 //   pr_dd_commit  contributions of the listed vertices from their (possibly new) ranks
 //   pr_dd_compact next worklist from the marks (one atomic per workgroup), marks cleared
 //   pr_dd_finish  iteration closed, the consumed worklist's length cleared
 // Ranks change only for listed vertices, so the contribution vector stays exact between
 // iterations without a full pass; the worklists' order varies, the per-vertex values do not.
 // Rows of in-degree <= AG_DD_SEQ_MAX are summed in the reference's sequential order, so on
-// an unblocked plan (rows keep their edge order) every value matches the sequential
-// restatement bit for bit.
+// an unblocked plan (rows keep their edge order) with no longer row every value matches the
+// sequential restatement bit for bit; longer rows take a fixed lane-strided tree order.
 
 #define AG_DD_LANES 16u
 // rows up to this in-degree are summed in edge order; longer rows in a lane-strided tree
 #define AG_DD_SEQ_MAX 8192u
 
-__global__ __launch_bounds__(AG_WG) void pr_dd_gather_kernel(PrCtl C, PrF32Args A,
+__global__ __launch_bounds__(AG_WG) void pr_dd_gather_kernel(PrCtl C, PrF32Args A, uint32_t n,
                                                              const uint32_t *off,
                                                              const uint32_t *src,
                                                              const uint32_t *fwd_off,
@@ -669,33 +670,64 @@ __global__ __launch_bounds__(AG_WG) void pr_dd_gather_kernel(PrCtl C, PrF32Args 
                                                              const uint32_t *queue,
                                                              uint8_t *flag)
 {
+    __shared__ float s_red[AG_WG / AG_WAVE];
+    __shared__ uint32_t s_upd;
     const PrState *st = C.st;
     if (*(volatile const uint32_t *)&st->done)
         return;
     const uint32_t iter = *(volatile const uint32_t *)&st->iter;
     const uint32_t cnt = *(volatile const uint32_t *)&st->dd_count[iter & 1u];
+    const uint32_t hubs = *(volatile const uint32_t *)&st->dd_hubs[iter & 1u];
     const float *c = (const float *)A.c32[0];
+    Red red{0.0, 0u};
+    // hub rows (in-degree > AG_DD_SEQ_MAX), listed from the far end: one workgroup each, four
+    // independent lane accumulators (a fixed order, so the sum is deterministic)
+    for (uint32_t h = blockIdx.x; h < hubs; h += gridDim.x) {
+        const uint32_t v = queue[n - 1u - h];
+        const uint32_t b = off[v], e = off[v + 1];
+        float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+        uint32_t j = b + threadIdx.x;
+        for (; j + 3u * AG_WG < e; j += 4u * AG_WG) {
+            a0 = __fadd_rn(a0, c[src[j]]);
+            a1 = __fadd_rn(a1, c[src[j + AG_WG]]);
+            a2 = __fadd_rn(a2, c[src[j + 2u * AG_WG]]);
+            a3 = __fadd_rn(a3, c[src[j + 3u * AG_WG]]);
+        }
+        for (; j < e; j += AG_WG)
+            a0 = __fadd_rn(a0, c[src[j]]);
+        const float sum = block_reduce<float, AG_WG>(__fadd_rn(__fadd_rn(a0, a1),
+                                                               __fadd_rn(a2, a3)), s_red);
+        if (threadIdx.x == 0) {
+            const float nw = __fadd_rn(A.base, __fmul_rn(A.damp, sum));
+            const double err = (double)fabsf(__fsub_rn(nw, A.rank[v]));
+            red.err += __ddiv_rn(err, A.dn);
+            s_upd = err >= A.eps ? 1u : 0u;
+            if (s_upd) {
+                A.rank[v] = nw;
+                red.active += 1u;
+            }
+        }
+        __syncthreads();
+        if (s_upd) {
+            for (uint32_t k = fwd_off[v] + threadIdx.x; k < fwd_off[v + 1]; k += AG_WG)
+                flag[fwd_dst[k]] = 1;
+        }
+        __syncthreads();
+    }
     const uint32_t lane = threadIdx.x & (AG_DD_LANES - 1u);
     const uint32_t groups = gridDim.x * (AG_WG / AG_DD_LANES);
-    Red red{0.0, 0u};
     for (uint32_t i = (blockIdx.x * AG_WG + threadIdx.x) / AG_DD_LANES; i < cnt; i += groups) {
         const uint32_t v = queue[i];
-        const uint32_t b = off[v], e = off[v + 1];
+        const uint32_t b = off[v], e = off[v + 1];   // e - b <= AG_DD_SEQ_MAX
+        // the 16 lanes load a chunk, then every lane adds it left to right: the reference's
+        // sequential order (pageRank.c:2888-2894), the same sum in every lane
         float sum = 0.0f;
-        if (e - b <= AG_DD_SEQ_MAX) {
-            // the 16 lanes load a chunk, then every lane adds it left to right: the
-            // reference's sequential order (pageRank.c:2888-2894), the same sum in every lane
-            for (uint32_t j0 = b; j0 < e; j0 += AG_DD_LANES) {
-                const uint32_t j = j0 + lane;
-                const float x = j < e ? c[src[j]] : 0.0f;
-                const uint32_t k = e - j0 < AG_DD_LANES ? e - j0 : AG_DD_LANES;
-                for (uint32_t l = 0; l < k; l++)
-                    sum = __fadd_rn(sum, __shfl(x, (int)l, (int)AG_DD_LANES));
-            }
-        } else {
-            for (uint32_t j = b + lane; j < e; j += AG_DD_LANES)
-                sum = __fadd_rn(sum, c[src[j]]);
-            sum = group_reduce<float, AG_DD_LANES>(sum);
+        for (uint32_t j0 = b; j0 < e; j0 += AG_DD_LANES) {
+            const uint32_t j = j0 + lane;
+            const float x = j < e ? c[src[j]] : 0.0f;
+            const uint32_t k = e - j0 < AG_DD_LANES ? e - j0 : AG_DD_LANES;
+            for (uint32_t l = 0; l < k; l++)
+                sum = __fadd_rn(sum, __shfl(x, (int)l, (int)AG_DD_LANES));
         }
         const float nw = __fadd_rn(A.base, __fmul_rn(A.damp, sum));
         const float old = A.rank[v];
@@ -716,7 +748,7 @@ __global__ __launch_bounds__(AG_WG) void pr_dd_gather_kernel(PrCtl C, PrF32Args 
     pr_store_partial(C, red);
 }
 
-__global__ __launch_bounds__(AG_WG) void pr_dd_commit_kernel(PrCtl C, PrF32Args A,
+__global__ __launch_bounds__(AG_WG) void pr_dd_commit_kernel(PrCtl C, PrF32Args A, uint32_t n,
                                                              const uint32_t *queue)
 {
     const PrState *st = C.st;
@@ -724,60 +756,83 @@ __global__ __launch_bounds__(AG_WG) void pr_dd_commit_kernel(PrCtl C, PrF32Args 
         return;
     const uint32_t iter = *(volatile const uint32_t *)&st->iter;
     const uint32_t cnt = *(volatile const uint32_t *)&st->dd_count[iter & 1u];
+    const uint32_t tot = cnt + *(volatile const uint32_t *)&st->dd_hubs[iter & 1u];
     float *c = (float *)A.c32[0];
-    for (uint32_t i = blockIdx.x * AG_WG + threadIdx.x; i < cnt; i += gridDim.x * AG_WG) {
-        const uint32_t v = queue[i];
+    for (uint32_t i = blockIdx.x * AG_WG + threadIdx.x; i < tot; i += gridDim.x * AG_WG) {
+        const uint32_t v = queue[i < cnt ? i : n - 1u - (i - cnt)];
         const uint32_t od = A.outdeg[v];
         c[v] = od ? __fdiv_rn(A.rank[v], __uint2float_rn(od)) : 0.0f;
     }
 }
 
-// Marked vertices -> next worklist; each thread owns 4 consecutive marks (one u32 word).
+// Marked vertices -> worklist `nxt`: rows of in-degree <= AG_DD_SEQ_MAX from the front, hub
+// rows from the far end. Each thread owns 4 consecutive marks (one u32 word); one atomic per
+// list and workgroup.
 __global__ __launch_bounds__(AG_WG) void pr_dd_compact_kernel(PrState *st, uint32_t n,
-                                                              uint8_t *flag, uint32_t *queue_next)
+                                                              const uint32_t *off, uint8_t *flag,
+                                                              uint32_t *queue_next, uint32_t nxt)
 {
-    __shared__ uint32_t s_wave[AG_WG / AG_WAVE];
-    __shared__ uint32_t s_base;
+    __shared__ uint32_t s_wave[2][AG_WG / AG_WAVE];
+    __shared__ uint32_t s_base[2];
     if (*(volatile const uint32_t *)&st->done)
         return;
-    const uint32_t nxt = ((*(volatile const uint32_t *)&st->iter) & 1u) ^ 1u;
     const uint32_t words = (n + 3u) / 4u;
     const uint32_t lane = threadIdx.x & (AG_WAVE - 1u), wave = threadIdx.x / AG_WAVE;
     uint32_t *fw = (uint32_t *)flag;
     for (uint32_t w0 = blockIdx.x * AG_WG; w0 < words; w0 += gridDim.x * AG_WG) {
         const uint32_t w = w0 + threadIdx.x;
         const uint32_t bits = w < words ? fw[w] : 0u;
-        uint32_t k = 0;
+        uint32_t hub = 0, k[2] = {0u, 0u};
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-            k += (bits >> (8 * q)) & 1u;
-        // exclusive prefix of k over the workgroup
-        uint32_t inc = k;
+        for (int q = 0; q < 4; q++) {
+            if ((bits >> (8 * q)) & 1u) {
+                const uint32_t v = 4u * w + (uint32_t)q;
+                const uint32_t h = off[v + 1] - off[v] > AG_DD_SEQ_MAX ? 1u : 0u;
+                hub |= h << q;
+                k[h]++;
+            }
+        }
+        // exclusive prefixes of k[0], k[1] over the workgroup
+        uint32_t inc[2] = {k[0], k[1]};
 #pragma unroll
         for (uint32_t d = 1; d < AG_WAVE; d <<= 1) {
-            const uint32_t t = __shfl_up(inc, d, AG_WAVE);
-            if (lane >= d)
-                inc += t;
+            const uint32_t t0 = __shfl_up(inc[0], d, AG_WAVE);
+            const uint32_t t1 = __shfl_up(inc[1], d, AG_WAVE);
+            if (lane >= d) {
+                inc[0] += t0;
+                inc[1] += t1;
+            }
         }
-        if (lane == AG_WAVE - 1u)
-            s_wave[wave] = inc;
+        if (lane == AG_WAVE - 1u) {
+            s_wave[0][wave] = inc[0];
+            s_wave[1][wave] = inc[1];
+        }
         __syncthreads();
-        if (threadIdx.x == 0) {
+        if (threadIdx.x < 2) {
+            const uint32_t c = threadIdx.x;
             uint32_t tot = 0;
             for (uint32_t q = 0; q < AG_WG / AG_WAVE; q++) {
-                const uint32_t t = s_wave[q];
-                s_wave[q] = tot;
+                const uint32_t t = s_wave[c][q];
+                s_wave[c][q] = tot;
                 tot += t;
             }
-            s_base = tot ? atomicAdd(&st->dd_count[nxt], tot) : 0u;
+            uint32_t *ctr = c ? &st->dd_hubs[nxt] : &st->dd_count[nxt];
+            s_base[c] = tot ? atomicAdd(ctr, tot) : 0u;
         }
         __syncthreads();
-        uint32_t pos = s_base + s_wave[wave] + inc - k;
+        uint32_t pos0 = s_base[0] + s_wave[0][wave] + inc[0] - k[0];
+        uint32_t pos1 = s_base[1] + s_wave[1][wave] + inc[1] - k[1];
         if (bits) {
 #pragma unroll
-            for (int q = 0; q < 4; q++)
-                if ((bits >> (8 * q)) & 1u)
-                    queue_next[pos++] = 4u * w + (uint32_t)q;
+            for (int q = 0; q < 4; q++) {
+                if ((bits >> (8 * q)) & 1u) {
+                    const uint32_t v = 4u * w + (uint32_t)q;
+                    if ((hub >> q) & 1u)
+                        queue_next[n - 1u - pos1++] = v;
+                    else
+                        queue_next[pos0++] = v;
+                }
+            }
             fw[w] = 0u;
         }
         __syncthreads();
@@ -793,17 +848,18 @@ __global__ __launch_bounds__(AG_FIN_WG) void pr_dd_finish(PrCtl C)
         return;
     const uint32_t iter = *(volatile const uint32_t *)&st->iter;
     pr_close(C, iter, Red{0.0, 0u});
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
         st->dd_count[iter & 1u] = 0u;
+        st->dd_hubs[iter & 1u] = 0u;
+    }
 }
 
-__global__ __launch_bounds__(AG_WG) void pr_dd_init_kernel(uint32_t n, uint32_t *queue0,
-                                                           uint32_t words, uint32_t *flag_words)
+// Every vertex marked (the compaction then builds the first worklist); pad bytes cleared.
+__global__ __launch_bounds__(AG_WG) void pr_dd_init_kernel(uint32_t n, uint8_t *flag,
+                                                           uint32_t bytes)
 {
-    for (uint32_t v = blockIdx.x * AG_WG + threadIdx.x; v < n; v += gridDim.x * AG_WG)
-        queue0[v] = v;
-    for (uint32_t w = blockIdx.x * AG_WG + threadIdx.x; w < words; w += gridDim.x * AG_WG)
-        flag_words[w] = 0u;
+    for (uint32_t v = blockIdx.x * AG_WG + threadIdx.x; v < bytes; v += gridDim.x * AG_WG)
+        flag[v] = v < n ? 1 : 0;
 }
 
 __global__ __launch_bounds__(AG_WG) void pr_zero_kernel(uint64_t *p, uint32_t count)
@@ -1083,7 +1139,6 @@ extern "C" int ag_pr_begin(ag_dgraph *g, const ag_pr_options *opts, ag_pr_sessio
     PrState init_st;
     memset(&init_st, 0, sizeof(init_st));
     init_st.refine_iter = -1;
-    init_st.dd_count[0] = n;   // data-driven pull: every vertex starts listed
     *s->st_host = init_st;
     {
         hipError_t e = hipMemcpyAsync(s->st, s->st_host, sizeof(PrState), hipMemcpyHostToDevice, stream);
@@ -1112,9 +1167,13 @@ extern "C" int ag_pr_begin(ag_dgraph *g, const ag_pr_options *opts, ag_pr_sessio
         }
         if (s->next)
             pr_zero_kernel<<<ag_quant_grid(g), AG_WG, 0, stream>>>((uint64_t *)s->next, n);
-        if (s->dd_queue[0])
-            pr_dd_init_kernel<<<ag_quant_grid(g), AG_WG, 0, stream>>>(
-                n, s->dd_queue[0], (n + 3u) / 4u, (uint32_t *)s->dd_flag);
+        if (s->dd_queue[0]) {
+            // data-driven pull: every vertex starts listed
+            pr_dd_init_kernel<<<ag_quant_grid(g), AG_WG, 0, stream>>>(n, s->dd_flag,
+                                                                      (n + 3u) / 4u * 4u);
+            pr_dd_compact_kernel<<<ag_quant_grid(g), AG_WG, 0, stream>>>(
+                s->st, n, g->off, s->dd_flag, s->dd_queue[0], 0u);
+        }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) {
             session_free(s);
@@ -1225,12 +1284,13 @@ int ag_pr_launch_iteration(ag_pr_session *s)
         const uint32_t gq = ag_quant_grid(g);
         const uint32_t it = s->enqueued;   // parity of the iteration being enqueued
         C.nparts = gq;
-        pr_dd_gather_kernel<<<gq, block, 0, stream>>>(C, A, g->off, g->src, g->fwd_off,
+        pr_dd_gather_kernel<<<gq, block, 0, stream>>>(C, A, g->n, g->off, g->src, g->fwd_off,
                                                       g->fwd_dst, s->dd_queue[it & 1u],
                                                       s->dd_flag);
-        pr_dd_commit_kernel<<<gq, block, 0, stream>>>(C, A, s->dd_queue[it & 1u]);
-        pr_dd_compact_kernel<<<gq, block, 0, stream>>>(s->st, g->n, s->dd_flag,
-                                                       s->dd_queue[(it & 1u) ^ 1u]);
+        pr_dd_commit_kernel<<<gq, block, 0, stream>>>(C, A, g->n, s->dd_queue[it & 1u]);
+        pr_dd_compact_kernel<<<gq, block, 0, stream>>>(s->st, g->n, g->off, s->dd_flag,
+                                                       s->dd_queue[(it & 1u) ^ 1u],
+                                                       (it & 1u) ^ 1u);
         pr_dd_finish<<<1, fin, 0, stream>>>(C);
         AG_HIP(hipGetLastError());
         return AG_OK;

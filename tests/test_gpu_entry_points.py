@@ -78,7 +78,11 @@ def test_pagerank_entry_points(graph, pushpull):
             # the FP32 bar of the parity suite: rows summed in a tree order (pull) or in the
             # atomics' order (push) may move the active-rule stop by a few iterations, ranks
             # agree to 1e-5
-            assert abs(int(s.iterations) - int(ref.iterations)) <= (3 if pushpull == 0 else 8)
+            if pushpull == 0:
+                assert abs(int(s.iterations) - int(ref.iterations)) <= 3
+            else:   # push: stop inside the oracle's tail (ulp-level deltas)
+                k = min(int(s.iterations), ref.executed - 1)
+                assert ref.hist_active[k] <= max(1, g.n // 1000)
             assert np.abs(ranks - ref.ranks).max() <= FP32_TOL
             np.testing.assert_allclose(ranks, ref.ranks, rtol=5e-5)
             if s.iterations == ref.iterations:   # same final iteration: same L1 delta

@@ -336,7 +336,13 @@ def test_pagerank_push_fixed_bit_exact(device, rmat16, kind, pname):
 
 
 def _fp32_bar(run, ref, iteration_slack=3):
-    assert abs(run.iterations - ref.iterations) <= iteration_slack
+    """iteration_slack None: the stopping iteration is only required to fall in the oracle's
+    tail (at most 0.1% of the vertices still active there)."""
+    if iteration_slack is None:
+        k = min(run.iterations, ref.executed - 1)
+        assert ref.hist_active[k] <= max(1, ref.ranks.size // 1000)
+    else:
+        assert abs(run.iterations - ref.iterations) <= iteration_slack
     assert np.abs(run.ranks - ref.ranks).max() <= FP32_TOL
     np.testing.assert_allclose(run.ranks, ref.ranks, rtol=5e-5)
 
@@ -345,8 +351,9 @@ def _fp32_bar(run, ref, iteration_slack=3):
 def test_pagerank_push_f32(device, rmat16, pname):
     """Push direction, FP32: the atomics add in an order of their own, different on every
     run, so the bar is the FP32 one. With a fixed iteration count the ranks are compared
-    directly; under the reference's stop rule the last few vertices at the epsilon threshold
-    (ulp-sized deltas) can keep the run going a few iterations longer or shorter."""
+    directly. Under the reference's stop rule the last vertices at the epsilon threshold have
+    ulp-sized deltas, so how long that tail lasts is rounding noise: the run must stop inside
+    the oracle's tail, with ranks within the FP32 bar."""
     say(f"pagerank push f32 {pname}")
     g = rmat16
     dg = upload(device, g, pname)
@@ -355,10 +362,20 @@ def test_pagerank_push_f32(device, rmat16, pname):
     _fp32_bar(run, ref, iteration_slack=0)
     run = dg.pagerank(direction=ag.DIR_PUSH, max_iterations=100, epsilon=1e-4)
     ref = ref_pr(g, max_iterations=100, epsilon=1e-4)
-    _fp32_bar(run, ref, iteration_slack=8)
+    _fp32_bar(run, ref, iteration_slack=None)
     f64 = f64_ref(g)
     assert np.abs(run.ranks - f64).max() <= 2 * np.abs(ref.ranks - f64).max() + 1e-9
     dg.close()
+
+
+def assert_eps_fixed_point(g, scaled, eps):
+    """One more (sequential) evaluation of every vertex moves it by less than epsilon, up to
+    the rounding of a sum taken in another order: a d-term sum of positive terms differs
+    between orders by at most about d * 2^-24 * sum, and sum < rank / 0.85."""
+    one = oracle.pagerank_pull(g.out_degree, g.inv_idx, g.in_degree, g.inv_src,
+                               max_iterations=1, stop_rule=2, init=2, init_ranks=scaled)
+    order = (g.in_degree.astype(np.float64) + 2) * 2.0 ** -24 * np.abs(scaled) / 0.85 * 2
+    assert np.all(np.abs(one.scaled.astype(np.float64) - scaled) < eps + order)
 
 
 def dd_ref(g, **kw):
@@ -390,16 +407,38 @@ def test_pagerank_data_driven(device, rmat16, pname):
         np.testing.assert_array_equal(run.scaled.view(np.uint32), ref.scaled.view(np.uint32))
         np.testing.assert_allclose(run.hist_err, ref.hist_err, rtol=1e-9, atol=1e-15)
     assert run.executed == run.iterations + 1 and run.hist_active[-1] == 0
-    one = oracle.pagerank_pull(g.out_degree, g.inv_idx, g.in_degree, g.inv_src,
-                               max_iterations=1, stop_rule=2, init=2, init_ranks=run.scaled)
-    slack = 4e-7 * max(1.0, float(np.abs(run.scaled).max()))
-    assert np.abs(one.scaled.astype(np.float64) - run.scaled).max() < eps + slack
+    assert_eps_fixed_point(g, run.scaled, eps)
     # epsilon 0: every listed vertex updates, and the ranks are the plain pull's
     run = dg.pagerank(direction=ag.DIR_DATA_DRIVEN, max_iterations=12, epsilon=0.0)
     ref = ref_pr(g, max_iterations=12, stop_rule=2)
     assert np.abs(run.ranks - ref.ranks).max() <= FP32_TOL
     np.testing.assert_allclose(run.ranks, ref.ranks, rtol=5e-5)
     dg.close()
+
+
+def test_pagerank_data_driven_hub_rows(device):
+    """Rows of in-degree above 8192 are listed apart and gathered by a whole workgroup in a
+    fixed tree order: the FP32 bar and the stopping invariant, on a graph with a row of
+    in-degree 12000 and a vertex whose 9000 out-edges all get marked when it changes."""
+    say("pagerank data-driven hub rows")
+    rng = np.random.default_rng(5)
+    n = 30000
+    src = [np.arange(1, 12001), np.full(9000, 7), rng.integers(0, n, 60000)]
+    dst = [np.zeros(12000, np.int64), np.arange(20000, 29000), rng.integers(0, n, 60000)]
+    src.append(np.arange(100, 200))
+    dst.append(np.arange(101, 201))
+    g = ag.Graph.from_edges(np.concatenate(src), np.concatenate(dst), n=n)
+    assert g.in_degree.max() > 8192 and g.out_degree.max() > 8192
+    eps = 1e-5
+    for pname in ("default", "blocked"):
+        dg = upload(device, g, pname)
+        run = dg.pagerank(direction=ag.DIR_DATA_DRIVEN, max_iterations=300, epsilon=eps,
+                          scaled=True)
+        ref = dd_ref(g, max_iterations=300, epsilon=eps)
+        assert abs(run.iterations - ref.iterations) <= 10
+        assert np.abs(run.ranks - ref.ranks).max() <= FP32_TOL
+        assert_eps_fixed_point(g, run.scaled, eps)
+        dg.close()
 
 
 def test_pagerank_directions_corners(device):
