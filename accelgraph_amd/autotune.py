@@ -20,6 +20,9 @@ import numpy as np
 
 from . import _abi, _check, lib, plan_params
 
+# a plan holds at most this many source blocks (AG_MAX_SRC_BLOCKS of the planner)
+MAX_SRC_BLOCKS = 1024
+
 GRID = {
     "src_block_bits": [255, 18, 19, 20],   # 255: no source blocking
     "blocked_min_bin": [0, 1],
@@ -105,7 +108,8 @@ def autotune(dev, graph, iterations: int = 5, repeats: int = 3, passes: int = 2,
                 if not (d["thread_max_deg"] <= d["group16_max_deg"] <= d["wave_max_deg"]
                         <= d["block_max_deg"]):
                     continue
-                if knob == "src_block_bits" and v != 255 and (graph.n + (1 << v) - 1) >> v > 4096:
+                if (knob == "src_block_bits" and v != 255
+                        and (graph.n + (1 << v) - 1) >> v > MAX_SRC_BLOCKS):
                     continue   # more source blocks than a plan allows
                 dicts.append(d)
                 cands.append(plan_params(**d))

@@ -335,11 +335,13 @@ def test_block_plan_rejects_bad_bits():
     g = ag.Graph.rmat(6)
     st, p, err = build(g, ag.plan_params(src_block_bits=7))
     assert st == 1 and b"src_block_bits" in err
-    n = (1 << 18) + 1   # 1025 blocks of 256 sources
+    from accelgraph_amd import autotune
+    cap = autotune.MAX_SRC_BLOCKS   # the autotuner's copy of the planner's limit
+    n = cap * 256 + 1   # one block more than the limit, 256 sources each
     st, p, err = build(None, n=n, outdeg=np.zeros(n, np.uint32), idx=np.zeros(n, np.uint32),
                        deg=np.zeros(n, np.uint32), src=np.zeros(0, np.uint32),
                        params=ag.plan_params(src_block_bits=8))
-    assert st == 1 and b"1024" in err
+    assert st == 1 and str(cap).encode() in err
 
 
 def test_block_plan_weights_follow_sources():
