@@ -62,6 +62,74 @@ def int_kind_iteration(ag, dev, dg, m: int, kind: int, nbytes: int) -> dict:
             "roofline_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
+def cpu_rows(g, quant: dict, directions: dict) -> None:
+    """The oracle's OpenMP restatements of the same rows on the host cores, on bounded samples
+    (2 iterations per quantised kind, 1 per push kind, the first 5 data-driven iterations),
+    written beside each GPU figure."""
+    import oracle
+    threads = len(os.sched_getaffinity(0))
+    inv = (g.out_degree, g.inv_idx, g.in_degree, g.inv_src)
+    fwd = (g.edges_idx, g.out_degree, g.fwd_dest)
+    for b, kind in ((32, oracle.KIND_QUANT32), (16, oracle.KIND_QUANT16), (8, oracle.KIND_QUANT8)):
+        t0 = time.perf_counter()
+        oracle.pagerank_pull(*inv, kind=kind, max_iterations=2, stop_rule=2, threads=threads)
+        ms = (time.perf_counter() - t0) * 1e3 / 2
+        row = quant[f"quant{b}"]
+        row["cpu_ms_per_iteration"] = round(ms, 2)
+        row["gpu_speedup"] = round(ms / row["ms_per_iteration"], 1)
+    for name, kind in (("push_f32", 0), ("push_fixed64", 64)):
+        t0 = time.perf_counter()
+        oracle.pagerank_push(g.out_degree, *fwd, kind=kind, max_iterations=1, stop_rule=2,
+                             threads=threads)
+        ms = (time.perf_counter() - t0) * 1e3
+        row = directions[name]
+        row["cpu_ms_per_iteration"] = round(ms, 2)
+        row["gpu_speedup"] = round(ms / row["ms_per_iteration"], 1)
+    t0 = time.perf_counter()
+    oracle.pagerank_dd_pull(*inv, *fwd, max_iterations=5, epsilon=1e-4, threads=threads)
+    ms = (time.perf_counter() - t0) * 1e3
+    row = directions["data_driven_pull_first_5_iterations"]
+    row["cpu_ms"] = round(ms, 1)
+    row["gpu_speedup"] = round(ms / row["loop_ms"], 1)
+    directions["cpu_cores"] = threads
+
+
+def directions_section(ag, dev, dg, m: int, n: int) -> dict:
+    """Push iterations (FP32 atomics, fixed-64 u64 atomics) and the data-driven pull against
+    the plain pull under the reference's stop rule (epsilon 1e-4, scaled units)."""
+    out = {}
+    t0 = time.perf_counter()
+    s = dg.session(direction=ag.DIR_PUSH, max_iterations=1, stop_rule=ag.STOP_NONE)
+    dev.sync()
+    s.end()
+    out["forward_csr_build_s"] = round(time.perf_counter() - t0, 3)
+    for name, kind in (("push_f32", ag.KIND_F32), ("push_fixed64", ag.KIND_FIXED64)):
+        s = dg.session(kind=kind, direction=ag.DIR_PUSH, max_iterations=13,
+                       stop_rule=ag.STOP_NONE, init=ag.INIT_UNIFORM)
+        s.step(3)
+        dev.sync()
+        s.step(10)
+        dev.sync()
+        ms, _ = s.last_step_kernel_ms()
+        s.end()
+        # forward src + dst streamed, one contribution read and one atomic per edge, then the
+        # update pass: next read + cleared, rank read + written, out-degree, contribution
+        cb = 4 if kind == ag.KIND_F32 else 8
+        nbytes = 8 * m + 4 * m + cb * m + (2 * cb + 16) * n
+        out[name] = {"ms_per_iteration": round(ms, 4), "gteps": round(m / (ms * 1e-3) / 1e9, 2),
+                     "algorithmic_bytes": nbytes,
+                     "gb_per_s": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+    for name, d in (("pull", ag.DIR_PULL), ("data_driven_pull", ag.DIR_DATA_DRIVEN)):
+        run = dg.pagerank(direction=d, max_iterations=200, epsilon=1e-4)
+        out[f"{name}_to_reference_stop"] = {"iterations": run.iterations,
+                                            "loop_ms": round(run.loop_ms, 2),
+                                            "error_total": run.error_total}
+    run = dg.pagerank(direction=ag.DIR_DATA_DRIVEN, max_iterations=5, epsilon=1e-4,
+                      stop_rule=ag.STOP_NONE)
+    out["data_driven_pull_first_5_iterations"] = {"loop_ms": round(run.loop_ms, 2)}
+    return out
+
+
 def cpu_baseline(g, budget_s: float) -> dict:
     import oracle
     threads = len(os.sched_getaffinity(0))
@@ -378,6 +446,8 @@ def main() -> int:
                                              algorithmic_bytes(n, m, f"q{b}"))
              for b, kind in ((32, ag.KIND_QUANT32), (16, ag.KIND_QUANT16), (8, ag.KIND_QUANT8))}
     log(f"quantised iterations: {quant}")
+    directions = directions_section(ag, dev, dg, m, n)
+    log(f"push / data-driven: {directions}")
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
@@ -411,6 +481,11 @@ def main() -> int:
 
     out["fixed32_iteration"] = fixed32
     out["quantized_iteration"] = quant
+    out["directions"] = directions
+    if not args.no_cpu:
+        t0 = time.perf_counter()
+        cpu_rows(g, quant, directions)
+        log(f"CPU legs of the quantised / push / data-driven rows ({time.perf_counter() - t0:.1f} s)")
     if construction:
         out["graph_construction"] = construction
     if not args.no_ttl:

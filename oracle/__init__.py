@@ -55,8 +55,12 @@ def lib() -> C.CDLL:
                                        P(_PrOut)]
         L.or_pagerank_dd_pull.restype = C.c_int
         L.or_pagerank_dd_pull.argtypes = [u32, P(u32), P(u32), P(u32), P(u32), P(u32), P(u32),
-                                          P(u32), u32, f64, u32, P(f32), P(f32), P(f32), P(u32),
-                                          P(f64), P(_PrOut)]
+                                          P(u32), u32, f64, u32, P(f32), C.c_int, P(f32),
+                                          P(f32), P(u32), P(f64), P(_PrOut)]
+        L.or_pagerank_push.restype = C.c_int
+        L.or_pagerank_push.argtypes = [u32, P(u32), P(u32), P(u32), P(u32), u32, u32, f64, u32,
+                                       f64, u32, P(f32), C.c_int, P(f32), P(f32), P(u32),
+                                       P(f64), P(_PrOut)]
         L.or_quantize.restype = f32
         L.or_quantize.argtypes = [u32, P(f32), u32, P(u64)]
         L.or_pagerank_f64.restype = C.c_int
@@ -165,9 +169,31 @@ def pagerank_pull(outdeg, idx, deg, src, kind: int = 0, max_iterations: int = 20
                  ha[:o.executed], he[:o.executed])
 
 
+def pagerank_push(outdeg, fwd_idx, fwd_deg, fwd_dst, kind: int = 0, max_iterations: int = 20,
+                  epsilon: float = 1e-4, stop_rule: int = 0, l1_tolerance: float = 1e-6,
+                  init: int = 0, init_ranks=None, threads: int = 0) -> PrOut:
+    """Push directions (pageRank.c:1111-1272, 1992-2117) over the forward CSR."""
+    outdeg, fwd_idx, fwd_deg, fwd_dst = _u32(outdeg), _u32(fwd_idx), _u32(fwd_deg), _u32(fwd_dst)
+    n = outdeg.size
+    ir = None if init_ranks is None else np.ascontiguousarray(init_ranks, np.float32)
+    ranks = np.zeros(n, np.float32)
+    scaled = np.zeros(n, np.float32)
+    ha = np.zeros(max(max_iterations, 1), np.uint32)
+    he = np.zeros(max(max_iterations, 1), np.float64)
+    o = _PrOut()
+    st = lib().or_pagerank_push(n, _p(outdeg, u32), _p(fwd_idx, u32), _p(fwd_deg, u32),
+                                _p(fwd_dst, u32), kind, max_iterations, epsilon, stop_rule,
+                                l1_tolerance, init, _p(ir, f32), threads, _p(ranks, f32),
+                                _p(scaled, f32), _p(ha, u32), _p(he, f64), C.byref(o))
+    if st != 0:
+        raise RuntimeError(f"or_pagerank_push failed ({st})")
+    return PrOut(ranks, scaled, o.iterations, o.executed, o.active, o.error_total, o.rank_sum,
+                 ha[:o.executed], he[:o.executed])
+
+
 def pagerank_dd_pull(outdeg, idx, deg, src, fwd_idx, fwd_deg, fwd_dst,
                      max_iterations: int = 20, epsilon: float = 1e-4, init: int = 0,
-                     init_ranks=None) -> PrOut:
+                     init_ranks=None, threads: int = 0) -> PrOut:
     """Data-driven (worklist) pull, pageRank.c:2810-2969; forward CSR for the worklist."""
     outdeg, idx, deg, src = _u32(outdeg), _u32(idx), _u32(deg), _u32(src)
     fwd_idx, fwd_deg, fwd_dst = _u32(fwd_idx), _u32(fwd_deg), _u32(fwd_dst)
@@ -181,7 +207,7 @@ def pagerank_dd_pull(outdeg, idx, deg, src, fwd_idx, fwd_deg, fwd_dst,
     st = lib().or_pagerank_dd_pull(n, _p(outdeg, u32), _p(idx, u32), _p(deg, u32),
                                    _p(src, u32), _p(fwd_idx, u32), _p(fwd_deg, u32),
                                    _p(fwd_dst, u32), max_iterations, epsilon, init,
-                                   _p(ir, f32), _p(ranks, f32), _p(scaled, f32), _p(ha, u32),
+                                   _p(ir, f32), threads, _p(ranks, f32), _p(scaled, f32), _p(ha, u32),
                                    _p(he, f64), C.byref(o))
     if st != 0:
         raise RuntimeError(f"or_pagerank_dd_pull failed ({st})")

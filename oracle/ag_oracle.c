@@ -14,6 +14,7 @@
  *                      update base + damp*next, err += |delta|/N, active += |delta| >= eps
  *                      (:1055-1069), stop on active == 0 (:1072-1074), ranks /= N and an FP64
  *                      sum (:1079-1085).
+ *   or_pagerank_push   the push directions, pageRank.c:1111-1272 and :1992-2117.
  *   or_pagerank_dd_pull  the data-driven (worklist) pull, pageRank.c:2810-2969.
  *   or_pagerank_f64    the same recurrence in FP64, iterated to a tight tolerance: the
  *                      double-precision reference every FP32 result is measured against.
@@ -344,7 +345,111 @@ int or_pagerank_pull(uint32_t n, const uint32_t *outdeg, const uint32_t *idx,
     return 0;
 }
 
-/* pageRankDataDrivenPullGraphCSR (pageRank.c:2810-2969), sequentially. Every vertex starts
+/* pageRankPushGraphCSR / pageRankPushFixedPointGraphCSR (pageRank.c:1111-1272, 1992-2117):
+ * the pull's loop with next[u] formed by scattering every source's contribution along its
+ * out-edges (forward CSR fwd_idx/fwd_deg/fwd_dst), `#pragma omp atomic` adds as at :2062-2063,
+ * then the update loop of the pull. kind 0 (FP32) or a fixed-point width. Sequentially the
+ * scatter adds each vertex's in-edges in ascending source order, which is the inverse CSR's
+ * order, so the result equals or_pagerank_pull's; with threads the FP32 additions take the
+ * threads' order and fixed-point sums stay exact. */
+int or_pagerank_push(uint32_t n, const uint32_t *outdeg, const uint32_t *fwd_idx,
+                     const uint32_t *fwd_deg, const uint32_t *fwd_dst, uint32_t kind,
+                     uint32_t max_iter, double eps, uint32_t stop_rule, double l1_tol,
+                     uint32_t init, const float *init_ranks, int threads, float *ranks_out,
+                     float *scaled_out, uint32_t *hist_active, double *hist_err,
+                     or_pr_out *out)
+{
+    if (kind != 0 && kind != 64 && kind != 32 && kind != 16 && kind != 8)
+        return -1;
+    const float damp = OR_DAMP;
+    const float base = 1.0f - damp;
+    const size_t nn = n ? n : 1;
+    const size_t csize = kind == 0 ? 4 : kind / 8;
+    float *rank = (float *)malloc(nn * sizeof(float));
+    void *contrib = malloc(nn * csize);
+    float *next_f = (float *)calloc(nn, sizeof(float));
+    uint64_t *next_q = (uint64_t *)calloc(nn, sizeof(uint64_t));
+    if (!rank || !contrib || !next_f || !next_q) {
+        free(rank);
+        free(contrib);
+        free(next_f);
+        free(next_q);
+        return -2;
+    }
+    for (uint32_t v = 0; v < n; v++)
+        rank[v] = init == OR_INIT_REFERENCE ? base : init == OR_INIT_UNIFORM ? 1.0f : init_ranks[v];
+    memset(out, 0, sizeof(*out));
+    uint32_t it, active = 0;
+    double error_total = 0.0;
+    int stopped = 0;
+    const int nt = threads > 0 ? threads : 1;
+    for (it = 0; it < max_iter; it++) {
+        error_total = 0.0;
+        active = 0;
+        #pragma omp parallel for schedule(static) if (threads > 0) num_threads(nt)
+        for (uint32_t v = 0; v < n; v++)
+            contrib_store(kind, contrib, v, rank[v], outdeg[v]);
+        #pragma omp parallel for schedule(dynamic, 1024) if (threads > 0) num_threads(nt)
+        for (uint32_t v = 0; v < n; v++) {
+            const uint32_t b = fwd_idx[v], d = fwd_deg[v];
+            if (kind == 0) {
+                const float c = ((const float *)contrib)[v];
+                for (uint32_t k = 0; k < d; k++) {
+                    #pragma omp atomic
+                    next_f[fwd_dst[b + k]] += c;
+                }
+            } else {
+                const uint64_t q = contrib_load_q(kind, contrib, v);
+                for (uint32_t k = 0; k < d; k++) {
+                    #pragma omp atomic
+                    next_q[fwd_dst[b + k]] += q;
+                }
+            }
+        }
+        #pragma omp parallel for schedule(static) if (threads > 0) num_threads(nt) reduction(+ : error_total, active)
+        for (uint32_t v = 0; v < n; v++) {
+            const float prev = rank[v];
+            const float next = update_rank(kind, base, damp, next_f[v], next_q[v]);
+            next_f[v] = 0.0f;
+            next_q[v] = 0;
+            rank[v] = next;
+            const double error = fabs((double)(next - prev));
+            error_total += error / n;
+            if (error >= eps)
+                active++;
+        }
+        if (hist_active)
+            hist_active[it] = active;
+        if (hist_err)
+            hist_err[it] = error_total;
+        if ((stop_rule == OR_STOP_ACTIVE && active == 0) ||
+            (stop_rule == OR_STOP_L1 && error_total <= l1_tol)) {
+            stopped = 1;
+            break;
+        }
+    }
+    out->iterations = it;
+    out->executed = stopped ? it + 1 : it;
+    out->active = active;
+    out->error_total = error_total;
+    double sum = 0.0;
+    for (uint32_t v = 0; v < n; v++) {
+        if (scaled_out)
+            scaled_out[v] = rank[v];
+        const float r = rank[v] / n;
+        if (ranks_out)
+            ranks_out[v] = r;
+        sum += r;
+    }
+    out->rank_sum = sum;
+    free(rank);
+    free(contrib);
+    free(next_f);
+    free(next_q);
+    return 0;
+}
+
+/* pageRankDataDrivenPullGraphCSR (pageRank.c:2810-2969). Every vertex starts
  * on the worklist (:2849-2856). Per iteration the contributions rank/out_degree are taken from
  * the ranks at the iteration's start (:2870-2876); each listed vertex sums its in-edges in
  * inverse-CSR order (:2888-2894), adds |new - old| / N to the error (:2896-2899) and, only when
@@ -355,9 +460,9 @@ int or_pagerank_pull(uint32_t n, const uint32_t *outdeg, const uint32_t *idx,
 int or_pagerank_dd_pull(uint32_t n, const uint32_t *outdeg, const uint32_t *idx,
                         const uint32_t *deg, const uint32_t *src, const uint32_t *fwd_idx,
                         const uint32_t *fwd_deg, const uint32_t *fwd_dst, uint32_t max_iter,
-                        double eps, uint32_t init, const float *init_ranks, float *ranks_out,
-                        float *scaled_out, uint32_t *hist_active, double *hist_err,
-                        or_pr_out *out)
+                        double eps, uint32_t init, const float *init_ranks, int threads,
+                        float *ranks_out, float *scaled_out, uint32_t *hist_active,
+                        double *hist_err, or_pr_out *out)
 {
     const float damp = OR_DAMP;
     const float base = 1.0f - damp;
@@ -382,11 +487,16 @@ int or_pagerank_dd_pull(uint32_t n, const uint32_t *outdeg, const uint32_t *idx,
     uint32_t it, active = 0;
     double error_total = 0.0;
     int stopped = 0;
+    const int nt = threads > 0 ? threads : 1;
     for (it = 0; it < max_iter; it++) {
         error_total = 0.0;
         active = 0;
+        #pragma omp parallel for schedule(static) if (threads > 0) num_threads(nt)
         for (uint32_t v = 0; v < n; v++)
             contrib[v] = outdeg[v] ? rank[v] / (float)outdeg[v] : 0.0f;
+        /* threads > 0: the reference's OpenMP loop (schedule(dynamic,1024), :2890); the marks
+         * are idempotent stores, so the per-vertex values equal the sequential run's */
+        #pragma omp parallel for schedule(dynamic, 1024) if (threads > 0) num_threads(nt) reduction(+ : error_total, active)
         for (uint32_t v = 0; v < n; v++) {
             if (!work[v])
                 continue;
@@ -398,8 +508,10 @@ int or_pagerank_dd_pull(uint32_t n, const uint32_t *outdeg, const uint32_t *idx,
             error_total += error / n;
             if (error >= eps) {
                 rank[v] = next;
-                for (uint32_t k = 0; k < fwd_deg[v]; k++)
+                for (uint32_t k = 0; k < fwd_deg[v]; k++) {
+                    #pragma omp atomic write
                     next_work[fwd_dst[fwd_idx[v] + k]] = 1;
+                }
                 active++;
             }
         }

@@ -321,3 +321,49 @@ def test_dd_pull_final_ranks_are_epsilon_fixed_points():
         # e = r + 0.85 P e with |r_v| < eps and ||P||_1 <= 1 (dangling mass is dropped):
         # sum |e| <= N eps / 0.15 in scaled units, i.e. eps / 0.15 normalised
         assert np.abs(d.ranks.astype(np.float64) - f64).sum() <= eps / 0.15 * 1.001
+
+
+# ------------------------------------------------------------------ push, parallel forms
+
+def _push(g, **kw):
+    return oracle.pagerank_push(g.out_degree, g.edges_idx, g.out_degree, g.fwd_dest, **kw)
+
+
+@pytest.mark.parametrize("kind", [0, 64, 32, 16, 8])
+def test_push_sequential_equals_pull(kind):
+    """Scattering sources in ascending order adds each vertex's in-edges in the inverse CSR's
+    order (rows sorted by source), so the sequential push is the pull bit for bit."""
+    import accelgraph_amd as ag
+    g = ag.Graph.rmat(11)
+    p = oracle.pagerank_pull(g.out_degree, g.inv_idx, g.in_degree, g.inv_src, kind=kind,
+                             max_iterations=80, epsilon=1e-4)
+    q = _push(g, kind=kind, max_iterations=80, epsilon=1e-4)
+    assert (q.iterations, q.executed) == (p.iterations, p.executed)
+    np.testing.assert_array_equal(q.scaled.view(np.uint32), p.scaled.view(np.uint32))
+    np.testing.assert_array_equal(q.hist_active, p.hist_active)
+
+
+def test_push_threads():
+    """With threads the fixed-point sums stay exact (the pull's ranks bit for bit); FP32 adds
+    in the threads' order and stays within rounding."""
+    import accelgraph_amd as ag
+    g = ag.Graph.rmat(12)
+    p = oracle.pagerank_pull(g.out_degree, g.inv_idx, g.in_degree, g.inv_src, kind=32,
+                             max_iterations=40, stop_rule=2)
+    q = _push(g, kind=32, max_iterations=40, stop_rule=2, threads=4)
+    np.testing.assert_array_equal(q.scaled.view(np.uint32), p.scaled.view(np.uint32))
+    p = oracle.pagerank_pull(g.out_degree, g.inv_idx, g.in_degree, g.inv_src,
+                             max_iterations=40, stop_rule=2)
+    q = _push(g, max_iterations=40, stop_rule=2, threads=4)
+    np.testing.assert_allclose(q.scaled, p.scaled, rtol=2e-6)
+
+
+def test_dd_pull_threads_equal_sequential():
+    import accelgraph_amd as ag
+    g = ag.Graph.rmat(12)
+    a = _dd(g, max_iterations=300, epsilon=1e-5)
+    b = _dd(g, max_iterations=300, epsilon=1e-5, threads=4)
+    assert (a.iterations, a.executed) == (b.iterations, b.executed)
+    np.testing.assert_array_equal(a.scaled.view(np.uint32), b.scaled.view(np.uint32))
+    np.testing.assert_array_equal(a.hist_active, b.hist_active)
+    np.testing.assert_allclose(a.hist_err, b.hist_err, rtol=1e-12)
