@@ -33,6 +33,36 @@
 #include "ag_fixed_dev.h"
 #include "ag_pull_kernel.h"
 
+// Accumulator of the data-driven pull on a source-blocked plan: the FP32 sum of the in-edges
+// and whether any in-neighbour took a new rank in the previous iteration (0 or 1, reduced
+// with max). Sums are added exactly as the FP32 pull adds them.
+struct DdAcc {
+    float s;
+    float f;
+    DdAcc() = default;
+    __device__ __forceinline__ DdAcc(int) : s(0.0f), f(0.0f) {}
+    __device__ __forceinline__ DdAcc(float s_, float f_) : s(s_), f(f_) {}
+};
+__device__ __forceinline__ DdAcc operator+(DdAcc a, DdAcc b)
+{
+    return DdAcc(__fadd_rn(a.s, b.s), fmaxf(a.f, b.f));
+}
+template <>
+__device__ __forceinline__ DdAcc ag_shfl_xor<DdAcc>(DdAcc v, int mask, int width)
+{
+    return DdAcc(__shfl_xor(v.s, mask, width), __shfl_xor(v.f, mask, width));
+}
+template <>
+__device__ __forceinline__ uint64_t acc_bits<DdAcc>(DdAcc v)
+{
+    return (uint64_t)__float_as_uint(v.s) | ((uint64_t)__float_as_uint(v.f) << 32);
+}
+template <>
+__device__ __forceinline__ DdAcc acc_from_bits<DdAcc>(uint64_t b)
+{
+    return DdAcc(__uint_as_float((uint32_t)b), __uint_as_float((uint32_t)(b >> 32)));
+}
+
 namespace {
 
 // ---------------------------------------------------------------- 16-bit storage
@@ -568,6 +598,98 @@ __global__ __launch_bounds__(AG_FIN_WG) void pr_int_finish(PullGeom G, PrCtl C, 
     pr_close(C, iter, red);
 }
 
+// ---------------------------------------------------------------- data-driven, full pull
+//
+// The data-driven pull (pageRank.c:2810-2969) on a source-blocked plan, with the FP32 pull's
+// kernels: every row is evaluated every iteration, and a row counts as listed when one of its
+// in-neighbours took a new rank in the previous iteration. The contribution of such a
+// neighbour carries the fact in its sign bit (contributions are never negative otherwise), so
+// the gather reads no extra byte. This equals the worklist form exactly: an unlisted vertex's
+// in-neighbours kept their contributions since its previous evaluation, so its sum and |delta|
+// repeat -- 0 if it took the new rank then, below epsilon if it did not -- and it neither
+// moves nor counts towards the error. Sums are ordered as in the FP32 pull.
+struct PrDdPol {
+    using Acc = DdAcc;
+    const float *cin;     // signed contributions of this iteration
+    float *cout;          // of the next one
+    float *rank;
+    const uint32_t *outdeg;
+    float base, damp;
+    double dn, eps;
+    bool all_listed;      // the first iteration lists every vertex
+
+    __device__ __forceinline__ DdAcc edge(uint32_t, uint32_t s) const
+    {
+        const float x = cin[s];
+        return DdAcc(fabsf(x), signbit(x) ? 1.0f : 0.0f);
+    }
+    __device__ __forceinline__ void finish(uint32_t r, DdAcc acc, Red &red) const
+    {
+        const float nw = __fadd_rn(base, __fmul_rn(damp, acc.s));
+        float rk = __builtin_nontemporal_load(&rank[r]);
+        const double e = (double)fabsf(__fsub_rn(nw, rk));
+        if (all_listed || acc.f > 0.0f)
+            red.err += __ddiv_rn(e, dn);
+        const bool upd = e >= eps;
+        if (upd) {
+            rk = nw;
+            __builtin_nontemporal_store(nw, &rank[r]);
+            red.active += 1u;
+        }
+        const uint32_t od = __builtin_nontemporal_load(&outdeg[r]);
+        const float c = od ? __fdiv_rn(rk, __uint2float_rn(od)) : 0.0f;
+        cout[r] = upd ? -c : c;
+    }
+};
+
+__device__ __forceinline__ PrDdPol dd_policy(uint32_t iter, const PrF32Args &A)
+{
+    const uint32_t rd = iter & 1u, wr = rd ^ 1u;
+    return PrDdPol{(const float *)A.c32[rd], (float *)A.c32[wr], A.rank, A.outdeg, A.base,
+                   A.damp, A.dn, A.eps, iter == 0};
+}
+
+__global__ __launch_bounds__(AG_WG) void pr_ddf_blk_kernel(PullGeom U, BlockGeom B, PrCtl C,
+                                                           PrF32Args A)
+{
+    __shared__ DdAcc s_vals[AG_TILE_NNZ_MAX];
+    __shared__ DdAcc s_wave[AG_WG / AG_WAVE];
+    __shared__ BlkLds L;
+    const PrState *st = C.st;
+    if (*(volatile const uint32_t *)&st->done)
+        return;
+    const uint32_t iter = *(volatile const uint32_t *)&st->iter;
+    Red red{0.0, 0u};
+    const PrDdPol p = dd_policy(iter, A);
+    pull_blocked_items(U, B, p, p, red, s_vals, s_wave, L);
+    pr_store_partial(C, red);
+}
+
+__global__ __launch_bounds__(AG_WG) void pr_ddf_cmb_kernel(BlockGeom B, PrCtl C, PrF32Args A)
+{
+    __shared__ DdAcc s_vals[AG_TILE_NNZ_MAX];
+    __shared__ DdAcc s_wave[AG_WG / AG_WAVE];
+    __shared__ BlkLds L;
+    const PrState *st = C.st;
+    if (*(volatile const uint32_t *)&st->done)
+        return;
+    const uint32_t iter = *(volatile const uint32_t *)&st->iter;
+    Red red{0.0, 0u};
+    pull_combine_items(B, dd_policy(iter, A), red, s_vals, s_wave, L);
+    pr_store_partial(C, red);
+}
+
+__global__ __launch_bounds__(AG_FIN_WG) void pr_ddf_finish(PullGeom G, PrCtl C, PrF32Args A)
+{
+    const PrState *st = C.st;
+    if (*(volatile const uint32_t *)&st->done)
+        return;
+    const uint32_t iter = *(volatile const uint32_t *)&st->iter;
+    Red red{0.0, 0u};
+    pull_hub_finish(G, dd_policy(iter, A), red);
+    pr_close(C, iter, red);
+}
+
 // ---------------------------------------------------------------- push direction
 //
 // pageRankPushGraphCSR (pageRank.c:1111-1272) and pageRankPushFixedPointGraphCSR
@@ -1078,7 +1200,10 @@ extern "C" int ag_pr_begin(ag_dgraph *g, const ag_pr_options *opts, ag_pr_sessio
     if (o.direction == AG_DIR_DATA_DRIVEN && kc != 0)
         return ag_set_error(AG_ERR_UNSUPPORTED,
                             "the data-driven pull takes FP32 with f32 storage only");
-    if (o.direction != AG_DIR_PULL) {
+    // the data-driven pull keeps explicit worklists on unblocked plans (edge-order sums) and
+    // runs as a full pull with sign-bit lists on source-blocked plans
+    const bool dd_lists = o.direction == AG_DIR_DATA_DRIVEN && g->blk.bits == 0;
+    if (o.direction == AG_DIR_PUSH || dd_lists) {
         const int fst = ag_graph_forward(g);
         if (fst != AG_OK)
             return fst;
@@ -1128,7 +1253,7 @@ extern "C" int ag_pr_begin(ag_dgraph *g, const ag_pr_options *opts, ag_pr_sessio
     }
     if (o.direction == AG_DIR_PUSH)
         AG_TRY(salloc(s, &s->next, (size_t)n * sizeof(uint64_t)));
-    if (o.direction == AG_DIR_DATA_DRIVEN) {
+    if (dd_lists) {
         AG_TRY(salloc(s, (void **)&s->dd_queue[0], (size_t)n * sizeof(uint32_t)));
         AG_TRY(salloc(s, (void **)&s->dd_queue[1], (size_t)n * sizeof(uint32_t)));
         AG_TRY(salloc(s, (void **)&s->dd_flag, (size_t)(n + 3u) / 4u * 4u));
@@ -1277,6 +1402,12 @@ int ag_pr_launch_iteration(ag_pr_session *s)
 #undef AG_PUSH
         }
         pr_plain_finish<<<1, fin, 0, stream>>>(C);
+        AG_HIP(hipGetLastError());
+        return AG_OK;
+    }
+    if (s->o.direction == AG_DIR_DATA_DRIVEN && blocked) {
+        const BlockGeom B = ag_make_block_geom(g);
+        AG_PR_LAUNCH_BLOCKED(pr_ddf_blk_kernel, pr_ddf_cmb_kernel, pr_ddf_finish);
         AG_HIP(hipGetLastError());
         return AG_OK;
     }

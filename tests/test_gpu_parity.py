@@ -383,13 +383,16 @@ def dd_ref(g, **kw):
                                    g.edges_idx, g.out_degree, g.fwd_dest, **kw)
 
 
-@pytest.mark.parametrize("pname", ["default", "small", "blocked-all"])
+@pytest.mark.parametrize("pname", ["default", "small", "blocked", "blocked-all",
+                                   "blocked-small"])
 def test_pagerank_data_driven(device, rmat16, pname):
-    """Data-driven pull against the oracle's. Unblocked plans keep every row's edge order and
-    rows up to in-degree 8192 (all of RMAT-16's) are summed in that order, so the run is the
-    oracle's bit for bit; a blocked plan reorders row edges by source block, so it gets the
-    FP32 bar. Either way the stopping state satisfies the invariant: one more evaluation of
-    any vertex moves it by less than epsilon."""
+    """Data-driven pull against the oracle's. On unblocked plans it keeps worklists, rows keep
+    their edge order and rows up to in-degree 8192 (all of RMAT-16's) are summed in that order,
+    so the run is the oracle's bit for bit. On a source-blocked plan it runs as a full pull with
+    sign-bit lists, summed in the FP32 pull's order, so it gets the FP32 bar there, and with
+    epsilon 0 it equals the FP32 pull on the same plan bit for bit. Either way the stopping
+    state satisfies the invariant: one more evaluation of any vertex moves it by less than
+    epsilon."""
     say(f"pagerank data-driven {pname}")
     g = rmat16
     assert g.in_degree.max() <= 8192
@@ -413,6 +416,10 @@ def test_pagerank_data_driven(device, rmat16, pname):
     ref = ref_pr(g, max_iterations=12, stop_rule=2)
     assert np.abs(run.ranks - ref.ranks).max() <= FP32_TOL
     np.testing.assert_allclose(run.ranks, ref.ranks, rtol=5e-5)
+    if pname in BLOCKED:
+        pull = dg.pagerank(max_iterations=12, stop_rule=ag.STOP_NONE)
+        assert dg.info().src_block_bits > 0
+        np.testing.assert_array_equal(run.ranks.view(np.uint32), pull.ranks.view(np.uint32))
     dg.close()
 
 
