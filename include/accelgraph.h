@@ -195,8 +195,11 @@ typedef struct ag_pr_options {
  *   DATA_DRIVEN  pull over a worklist (code 6, pageRank.c:2810-2969): only listed vertices
  *                gather; a vertex whose |delta| >= epsilon takes its new rank and lists its
  *                out-neighbours for the next iteration, others keep their rank; AG_KIND_F32
- *                with f32 storage only
- * PUSH and DATA_DRIVEN build the graph's forward CSR on the device on first use. */
+ *                with f32 storage only. Unblocked plans keep explicit worklists (sums in the
+ *                reference's edge order); source-blocked plans run it as a full pull that
+ *                carries the lists in the contributions' sign bits (same result, pull order)
+ * PUSH, and DATA_DRIVEN on an unblocked plan, build the graph's forward CSR on the device on
+ * first use. */
 enum { AG_DIR_PULL = 0, AG_DIR_PUSH = 1, AG_DIR_DATA_DRIVEN = 2 };
 
 void ag_pr_options_default(ag_pr_options *o);
