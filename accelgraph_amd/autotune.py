@@ -21,7 +21,7 @@ import numpy as np
 from . import _abi, _check, lib, plan_params
 
 # a plan holds at most this many source blocks (AG_MAX_SRC_BLOCKS of the planner)
-MAX_SRC_BLOCKS = 1024
+MAX_SRC_BLOCKS = 256
 
 GRID = {
     "src_block_bits": [255, 18, 19, 20],   # 255: no source blocking

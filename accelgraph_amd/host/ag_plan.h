@@ -40,7 +40,9 @@ extern const uint32_t ag_slot_bin[AG_NSLOTS];
  * row-major position lpos, and adds every row's partials in block order. */
 #define AG_NXCD 8u
 #define AG_PIECE_NNZ 4096u
-#define AG_MAX_SRC_BLOCKS 1024u
+/* source blocks of one plan: the combine keeps two per-block words of its tile in LDS, so this
+ * bounds every blocked kernel's LDS (2 KB here); the automatic block size grows to fit it */
+#define AG_MAX_SRC_BLOCKS 256u
 #define AG_CTILE_ROWS 4096u     /* rows of one combine tile */
 #define AG_AUTO_BLOCK_BITS 19u
 #define AG_AUTO_BLOCK_MIN_N (1u << 21)

@@ -169,7 +169,7 @@ static uint32_t block_bits_for(const ag_plan *p, const char **err)
         return b;
     }
     if ((((uint64_t)p->n + (1ull << b) - 1) >> b) > AG_MAX_SRC_BLOCKS)
-        *err = "src_block_bits too small for this graph: more than 1024 source blocks";
+        *err = "src_block_bits too small for this graph: more than 256 source blocks";
     return b;
 }
 

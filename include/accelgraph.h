@@ -89,8 +89,9 @@ typedef struct ag_plan_params {
                                   a blocked bin are regrouped block by block, so the gathers of
                                   one pass stay inside one block (an L2-sized window of the
                                   contribution vector). AG_SRC_BLOCK_AUTO: 19 bits when the
-                                  contribution vector exceeds 8 MB (n > 2^21), else off;
-                                  AG_SRC_BLOCK_OFF: off; else 8..30                      */
+                                  contribution vector exceeds 8 MB (n > 2^21), more if
+                                  needed to stay within 256 blocks, else off;
+                                  AG_SRC_BLOCK_OFF: off; else 8..30 (at most 256 blocks)   */
     uint32_t wg_per_cu;        /* resident workgroups per CU of the persistent grid (0 = auto:
                                   4, fewer if LDS limits; more in flight only queues at L2) */
     uint32_t blocked_min_bin;  /* with source blocking, rows of bins >= this are blocked.
