@@ -1340,12 +1340,20 @@ extern "C" int ag_pr_begin(ag_dgraph *g, const ag_pr_options *opts, ag_pr_sessio
         FIN<<<1, fin, 0, stream>>>(G, C, A);                 \
     } while (0)
 
+// Default resident workgroups per CU of the two blocked passes (plan wg_per_cu overrides
+// both): pass 1 is L2-request bound and saturates at 4; the combine streams partials and
+// likes 8 (measured on RMAT-25).
+#ifndef AG_PASS1_PER_CU
+#define AG_PASS1_PER_CU 4u
+#endif
+#define AG_COMBINE_PER_CU 8u
+
 #define AG_PR_LAUNCH_BLOCKED(BLK, CMB, FIN)                                  \
     do {                                                                     \
         static const uint32_t res1 = ag_resident_per_cu(BLK);               \
         static const uint32_t res2 = ag_resident_per_cu(CMB);               \
-        const uint32_t g1 = ag_blocked_grid(g, res1, 4u);                    \
-        const uint32_t g2 = ag_blocked_grid(g, res2, 8u);                    \
+        const uint32_t g1 = ag_blocked_grid(g, res1, AG_PASS1_PER_CU);                    \
+        const uint32_t g2 = ag_blocked_grid(g, res2, AG_COMBINE_PER_CU);                    \
         C.nparts = g1 + g2;                                                  \
         C.part_base = 0;                                                     \
         BLK<<<g1, block, 0, stream>>>(G, B, C, A);                           \
