@@ -1201,12 +1201,17 @@ extern "C" int ag_pr_begin(ag_dgraph *g, const ag_pr_options *opts, ag_pr_sessio
         return ag_set_error(AG_ERR_UNSUPPORTED,
                             "the data-driven pull takes FP32 with f32 storage only");
     // the data-driven pull keeps explicit worklists on unblocked plans (edge-order sums) and
-    // runs as a full pull with sign-bit lists on source-blocked plans
-    const bool dd_lists = o.direction == AG_DIR_DATA_DRIVEN && g->blk.bits == 0;
-    if (o.direction == AG_DIR_PUSH || dd_lists) {
-        const int fst = ag_graph_forward(g);
-        if (fst != AG_OK)
-            return fst;
+    // runs as a full pull with sign-bit lists on source-blocked plans. The sign bit is free
+    // only while every contribution is >= +0, i.e. from a start vector without negative
+    // entries, -0 or NaN (ranks stay positive from there); other user starts keep worklists.
+    bool dd_lists = o.direction == AG_DIR_DATA_DRIVEN && g->blk.bits == 0;
+    if (o.direction == AG_DIR_DATA_DRIVEN && !dd_lists && o.init == AG_INIT_USER &&
+        o.init_ranks) {
+        for (uint32_t v = 0; v < g->n && !dd_lists; v++) {
+            uint32_t bits;
+            memcpy(&bits, &o.init_ranks[v], sizeof(bits));
+            dd_lists = (bits >> 31) != 0u || !(o.init_ranks[v] >= 0.0f);
+        }
     }
     if (o.stop_rule > AG_STOP_NONE)
         return ag_set_error(AG_ERR_INVALID, "unknown stop rule %u", o.stop_rule);
@@ -1214,6 +1219,11 @@ extern "C" int ag_pr_begin(ag_dgraph *g, const ag_pr_options *opts, ag_pr_sessio
         return ag_set_error(AG_ERR_INVALID, "bad init mode or missing init_ranks");
     if (!(o.epsilon >= 0.0))
         return ag_set_error(AG_ERR_INVALID, "epsilon must be >= 0");
+    if (o.direction == AG_DIR_PUSH || dd_lists) {
+        const int fst = ag_graph_forward(g);
+        if (fst != AG_OK)
+            return fst;
+    }
 
     AG_HIP(hipSetDevice(g->dev->ordinal));
     ag_pr_session *s = (ag_pr_session *)calloc(1, sizeof(ag_pr_session));
@@ -1413,7 +1423,7 @@ int ag_pr_launch_iteration(ag_pr_session *s)
         AG_HIP(hipGetLastError());
         return AG_OK;
     }
-    if (s->o.direction == AG_DIR_DATA_DRIVEN && blocked) {
+    if (s->o.direction == AG_DIR_DATA_DRIVEN && blocked && !s->dd_queue[0]) {
         const BlockGeom B = ag_make_block_geom(g);
         AG_PR_LAUNCH_BLOCKED(pr_ddf_blk_kernel, pr_ddf_cmb_kernel, pr_ddf_finish);
         AG_HIP(hipGetLastError());

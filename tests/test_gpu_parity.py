@@ -448,6 +448,36 @@ def test_pagerank_data_driven_hub_rows(device):
         dg.close()
 
 
+def test_pagerank_data_driven_user_starts(device, rmat16):
+    """On a source-blocked plan a non-negative start runs as the full pull with sign-bit lists;
+    a start with negative entries or -0 keeps explicit worklists (the sign bit is not free).
+    Both against the oracle's data-driven pull (FP32 bar) and the stopping invariant."""
+    say("pagerank data-driven user starts")
+    g = rmat16
+    dg = upload(device, g, "blocked")
+    rng = np.random.default_rng(9)
+    pos = (rng.random(g.n) * 2).astype(np.float32)
+    signed = pos.copy()
+    signed[::5] *= -1
+    signed[7] = -0.0
+    for init in (pos, signed):
+        # three iterations still depend on the start: a sign taken for a mark would move
+        # ranks by O(1); block-ordered sums of mixed-sign terms differ by far less than 1e-3
+        run = dg.pagerank(direction=ag.DIR_DATA_DRIVEN, init_ranks=init, max_iterations=3,
+                          epsilon=1e-4, stop_rule=ag.STOP_NONE, scaled=True)
+        ref = dd_ref(g, init=2, init_ranks=init, max_iterations=3, epsilon=1e-4)
+        np.testing.assert_allclose(run.scaled, ref.scaled, rtol=1e-3, atol=1e-4)
+        assert np.all(np.abs(run.hist_active.astype(np.int64) -
+                             ref.hist_active.astype(np.int64)) <= 10 + g.n // 1000)
+        run = dg.pagerank(direction=ag.DIR_DATA_DRIVEN, init_ranks=init, max_iterations=300,
+                          epsilon=1e-4, scaled=True)
+        ref = dd_ref(g, init=2, init_ranks=init, max_iterations=300, epsilon=1e-4)
+        assert abs(run.iterations - ref.iterations) <= 10
+        assert np.abs(run.ranks - ref.ranks).max() <= FP32_TOL
+        assert_eps_fixed_point(g, run.scaled, 1e-4)
+    dg.close()
+
+
 def test_pagerank_directions_corners(device):
     say("push / data-driven corner cases")
     for src, dst, n in [([], [], 1), ([], [], 6), ([0], [1], 2), ([0, 1, 2], [1, 2, 0], 3),
