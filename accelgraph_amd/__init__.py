@@ -490,6 +490,24 @@ class DeviceGraph:
                                         reps, C.byref(ms)), "ag_spmv_pull_fixed")
         return y, ms.value
 
+    def spmv_push(self, x, reps: int = 1) -> tuple[np.ndarray, float]:
+        """Push form: `reps` products accumulated onto y = 0 (the reference's push)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros(self.n, np.float32)
+        ms = C.c_double(0.0)
+        _check(lib().ag_spmv_push(self._h, _ptr(x, C.c_float), _ptr(y, C.c_float), reps,
+                                  C.byref(ms)), "ag_spmv_push")
+        return y, ms.value
+
+    def spmv_push_fixed(self, xq, reps: int = 1) -> tuple[np.ndarray, float]:
+        """Fixed-point push: Q32.32 x and y."""
+        xq = np.ascontiguousarray(xq, dtype=np.uint64)
+        y = np.zeros(self.n, np.uint64)
+        ms = C.c_double(0.0)
+        _check(lib().ag_spmv_push_fixed(self._h, _ptr(xq, C.c_uint64), _ptr(y, C.c_uint64),
+                                        reps, C.byref(ms)), "ag_spmv_push_fixed")
+        return y, ms.value
+
     def gather(self, values, gtype: int) -> np.ndarray:
         vt = {GATHER_F32: np.float32, GATHER_U64: np.uint64, GATHER_U32: np.uint32,
               GATHER_U16: np.uint16, GATHER_U8: np.uint8}[gtype]

@@ -162,6 +162,8 @@ SIGNATURES = {
     "ag_pr_last_step_kernel_ms": (C.c_int, [VP, P(f64), P(u32)]),
     "ag_spmv_pull": (C.c_int, [VP, P(f32), P(f32), u32, P(f64)]),
     "ag_spmv_pull_fixed": (C.c_int, [VP, P(u32), P(u64), u32, P(f64)]),
+    "ag_spmv_push": (C.c_int, [VP, P(f32), P(f32), u32, P(f64)]),
+    "ag_spmv_push_fixed": (C.c_int, [VP, P(u64), P(u64), u32, P(f64)]),
     "ag_pull_gather": (C.c_int, [VP, u32, VP, VP]),
     "ag_ppr_options_default": (None, [P(PprOptions)]),
     "ag_ppr_pull": (C.c_int, [VP, u32, P(u32), P(PprOptions), P(f32), P(f64), P(PprResult)]),
@@ -224,6 +226,8 @@ SIGNATURES = {
     "pageRankPullQuant16BitGraphCSR": (P(PageRankStats), [P(Arguments), P(GraphCSR)]),
     "pageRankPullQuant8BitGraphCSR": (P(PageRankStats), [P(Arguments), P(GraphCSR)]),
     "pageRankPushGraphCSR": (P(PageRankStats), [P(Arguments), P(GraphCSR)]),
+    "SPMVPushGraphCSR": (P(SPMVStats), [P(Arguments), P(GraphCSR)]),
+    "SPMVPushFixedPointGraphCSR": (P(SPMVStats), [P(Arguments), P(GraphCSR)]),
     "pageRankPushFixedPointGraphCSR": (P(PageRankStats), [P(Arguments), P(GraphCSR)]),
     "pageRankDataDrivenPullGraphCSR": (P(PageRankStats), [P(Arguments), P(GraphCSR)]),
     # SPMV.h

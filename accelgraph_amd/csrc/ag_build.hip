@@ -555,6 +555,27 @@ __global__ __launch_bounds__(BUILD_WG) void row_ids_kernel(uint32_t n, uint32_t 
     }
 }
 
+__global__ __launch_bounds__(BUILD_WG) void iota_kernel(uint32_t m, uint32_t *a)
+{
+    for (uint32_t j = blockIdx.x * BUILD_WG + threadIdx.x; j < m; j += gridDim.x * BUILD_WG)
+        a[j] = j;
+}
+
+// Out-edge k came from inverse edge idx[k]: its destination is that edge's row, its weight
+// that edge's weight.
+__global__ __launch_bounds__(BUILD_WG) void forward_gather_kernel(uint32_t m, const uint32_t *idx,
+                                                                  const uint32_t *row,
+                                                                  const float *w, uint32_t *dst,
+                                                                  float *fw)
+{
+    for (uint32_t k = blockIdx.x * BUILD_WG + threadIdx.x; k < m; k += gridDim.x * BUILD_WG) {
+        const uint32_t j = idx[k];
+        dst[k] = row[j];
+        if (fw)
+            fw[k] = w[j];
+    }
+}
+
 __global__ void last_offset_kernel(uint32_t n, uint32_t m, uint32_t *off)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0)
@@ -564,8 +585,9 @@ __global__ void last_offset_kernel(uint32_t n, uint32_t m, uint32_t *off)
 } // namespace
 
 // Forward CSR of an uploaded graph in device row ids: the inverse CSR (off/src) transposed
-// by one stable radix sort of (source, row) pairs, so each out-row lists its destinations in
-// ascending order. Used by the push and data-driven PageRank directions.
+// by one stable radix sort of (source, inverse edge index) pairs, so each out-row lists its
+// destinations in ascending order; weights follow their edges. Used by the push directions
+// (PageRank and SpMV) and the worklist form of the data-driven pull.
 int ag_graph_forward(ag_dgraph *g)
 {
     if (g->fwd_off)
@@ -574,6 +596,7 @@ int ag_graph_forward(ag_dgraph *g)
     AG_HIP(hipSetDevice(dev->ordinal));
     const uint32_t n = g->n, m = g->m;
     uint32_t *off = NULL, *fsrc = NULL, *fdst = NULL;
+    float *fw = NULL;
     Pool T{};
     T.status = AG_OK;
     int st = AG_OK;
@@ -582,26 +605,35 @@ int ag_graph_forward(ag_dgraph *g)
         e = hipMalloc((void **)&fsrc, (size_t)(m ? m : 1) * sizeof(uint32_t));
     if (e == hipSuccess)
         e = hipMalloc((void **)&fdst, (size_t)(m ? m : 1) * sizeof(uint32_t));
+    if (e == hipSuccess && g->w)
+        e = hipMalloc((void **)&fw, (size_t)(m ? m : 1) * sizeof(float));
     if (e != hipSuccess)
         st = ag_hip_fail(e, "forward CSR allocation", __FILE__, __LINE__);
     uint32_t *row = T.get<uint32_t>(m);
+    uint32_t *idx = T.get<uint32_t>(m);
+    uint32_t *sidx = T.get<uint32_t>(m);
     uint32_t *deg = T.get<uint32_t>(n);
     if (st == AG_OK)
         st = T.status;
     if (st == AG_OK && m) {
-        row_ids_kernel<<<grid_of(m, BUILD_WG) < 8192u ? grid_of(m, BUILD_WG) : 8192u, BUILD_WG, 0,
-                         dev->stream>>>(n, m, g->off, row);
+        const uint32_t gm = grid_of(m, BUILD_WG) < 8192u ? grid_of(m, BUILD_WG) : 8192u;
+        row_ids_kernel<<<gm, BUILD_WG, 0, dev->stream>>>(n, m, g->off, row);
+        iota_kernel<<<gm, BUILD_WG, 0, dev->stream>>>(m, idx);
         e = hipGetLastError();
         size_t tmp = 0;
         if (e == hipSuccess)
-            e = rocprim::radix_sort_pairs(nullptr, tmp, g->src, fsrc, row, fdst, (size_t)m, 0u,
+            e = rocprim::radix_sort_pairs(nullptr, tmp, g->src, fsrc, idx, sidx, (size_t)m, 0u,
                                           bits_for(n), dev->stream);
         void *t = NULL;
         if (e == hipSuccess)
             e = hipMalloc(&t, tmp ? tmp : 8);
         if (e == hipSuccess)
-            e = rocprim::radix_sort_pairs(t, tmp, g->src, fsrc, row, fdst, (size_t)m, 0u,
+            e = rocprim::radix_sort_pairs(t, tmp, g->src, fsrc, idx, sidx, (size_t)m, 0u,
                                           bits_for(n), dev->stream);
+        if (e == hipSuccess) {
+            forward_gather_kernel<<<gm, BUILD_WG, 0, dev->stream>>>(m, sidx, row, g->w, fdst, fw);
+            e = hipGetLastError();
+        }
         if (e == hipSuccess)
             st = ag_wait_stream(dev);
         else
@@ -626,12 +658,14 @@ int ag_graph_forward(ag_dgraph *g)
         hipFree(off);
         hipFree(fsrc);
         hipFree(fdst);
+        hipFree(fw);
         return st;
     }
     g->fwd_off = off;
     g->fwd_src = fsrc;
     g->fwd_dst = fdst;
-    g->device_bytes += ((size_t)n + 1 + 2 * (size_t)m) * sizeof(uint32_t);
+    g->fwd_w = fw;
+    g->device_bytes += ((size_t)n + 1 + (fw ? 3 : 2) * (size_t)m) * sizeof(uint32_t);
     return AG_OK;
 }
 

@@ -30,6 +30,12 @@ __device__ __forceinline__ double ag_dev_fixed64_to_double(uint64_t q)
     return __dmul_rn(__ull2double_rn(q), 1.0 / 4294967296.0);
 }
 
+// MULFixed64V1 of fixedPoint.h: bits [32, 96) of the exact 128-bit product
+__device__ __forceinline__ uint64_t ag_dev_mul_fixed64(uint64_t a, uint64_t b)
+{
+    return (__umul64hi(a, b) << 32) | ((a * b) >> 32);
+}
+
 __device__ __forceinline__ uint32_t ag_dev_float_to_fixed32(float x)
 {
     double s = __dmul_rn((double)x, 65536.0);

@@ -53,6 +53,7 @@ struct ag_dgraph {
     uint32_t *fwd_off;   // [n+1]
     uint32_t *fwd_src;   // [m] source of every out-edge (row id of the forward CSR)
     uint32_t *fwd_dst;   // [m] destination, ascending within a row
+    float *fwd_w;        // [m] weight of every out-edge (null for an unweighted graph)
     // scratch for permuted vectors (x, y) of SpMV / gather, sized lazily
     void *vec_a;
     void *vec_b;

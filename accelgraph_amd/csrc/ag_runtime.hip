@@ -208,7 +208,8 @@ extern "C" int ag_graph_free(ag_dgraph *g)
                     g->seg_lo, g->seg_hi, g->hub_seg0, g->hub_part, g->vec_a, g->vec_b,
                     g->blk.voff, g->blk.vsrc, g->blk.vw, g->blk.item, g->blk.coff, g->blk.cidx,
                     g->blk.lpos, g->blk.ct_row, g->blk.ct_start, g->blk.ct_pre, g->blk.c_item,
-                    g->vpart, g->vwq, g->fwd_off, g->fwd_src, g->fwd_dst};
+                    g->vpart, g->vwq, g->fwd_off, g->fwd_src, g->fwd_dst,
+                    g->fwd_w};
     for (void *p : ptrs)
         if (p)
             hipFree(p);

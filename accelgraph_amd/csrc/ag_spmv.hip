@@ -6,13 +6,16 @@
 // 345-367); the FixedPoint datapath adds (u32)(((u64)x * w) >> SCALEF) into a u64 sum with
 // SCALEF = 16 (FixedPoint/cu/cu_sum_kernel_control.sv:155-164, FixedPoint/pkg/
 // globals_cu_pkg.sv:37). Each call computes y = A x once per repetition (the RTL overwrites
-// y every run). Unweighted graphs use the constant weight 0.0001f (SPMV.c:1206).
+// y every run). Unweighted graphs use the constant weight 0.0001f (SPMV.c:1206). The push
+// forms (SPMV.c:752-848, 1017-1136) scatter along the forward CSR and accumulate across
+// repetitions, as the reference's output is never cleared there.
 //
 // The raw gather is the sum kernel itself (cu_PageRank/CSR/PULL/*/cu/cu_sum_kernel_control.sv)
 // on caller-supplied per-vertex values; it exists so the gather arithmetic can be checked
 // against the reference's kernel-level known-answer fixtures.
 #include <string.h>
 
+#include "fixedPoint.h"
 #include "ag_internal.h"
 #include "ag_fixed_dev.h"
 #include "ag_pull_kernel.h"
@@ -62,6 +65,44 @@ __global__ void permute_out_kernel(uint32_t n, const T *dev, const uint32_t *d2o
     uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r < n)
         orig[d2o[r]] = dev[r];
+}
+
+// SpMV push (SPMV.c:752-848, 1017-1136): every source scatters w * x[src] to its out-edges'
+// destinations with atomics. The reference never clears the output between iterations, so
+// repetitions accumulate (y = reps * A x, added edge by edge). FP32: binary32 product, the
+// atomics' order of additions. Fixed point: Q32.32 x and weights, MULFixed64V1 products,
+// exact u64 sums.
+template <bool WEIGHTED>
+__global__ __launch_bounds__(AG_WG) void spmv_push_kernel(uint32_t m, const uint32_t *fsrc,
+                                                          const uint32_t *fdst, const float *fw,
+                                                          const float *x, float *y)
+{
+    for (uint32_t j = blockIdx.x * AG_WG + threadIdx.x; j < m; j += gridDim.x * AG_WG) {
+        const float w = WEIGHTED ? __builtin_nontemporal_load(&fw[j]) : 0.0001f;
+        unsafeAtomicAdd(&y[__builtin_nontemporal_load(&fdst[j])],
+                        __fmul_rn(w, x[__builtin_nontemporal_load(&fsrc[j])]));
+    }
+}
+
+template <bool WEIGHTED>
+__global__ __launch_bounds__(AG_WG) void spmv_push_fixed_kernel(uint32_t m, const uint32_t *fsrc,
+                                                                const uint32_t *fdst,
+                                                                const float *fw,
+                                                                const uint64_t *xq, uint64_t wc,
+                                                                uint64_t *y)
+{
+    for (uint32_t j = blockIdx.x * AG_WG + threadIdx.x; j < m; j += gridDim.x * AG_WG) {
+        const uint64_t w =
+            WEIGHTED ? ag_dev_double_to_fixed64((double)__builtin_nontemporal_load(&fw[j])) : wc;
+        atomicAdd((unsigned long long *)&y[__builtin_nontemporal_load(&fdst[j])],
+                  (unsigned long long)ag_dev_mul_fixed64(w, xq[__builtin_nontemporal_load(&fsrc[j])]));
+    }
+}
+
+__global__ __launch_bounds__(AG_WG) void zero_u32_kernel(uint32_t *p, uint32_t count)
+{
+    for (uint32_t i = blockIdx.x * AG_WG + threadIdx.x; i < count; i += gridDim.x * AG_WG)
+        p[i] = 0u;
 }
 
 __global__ void quantize_weights_kernel(uint32_t m, const float *w, uint32_t *wq)
@@ -268,6 +309,126 @@ extern "C" int ag_spmv_pull_fixed(ag_dgraph *g, const uint32_t *xq, uint64_t *yq
     permute_out_kernel<uint64_t><<<blk, AG_WG, 0, stream>>>(n, yd, g->d2o, yo);
     AG_HIP(hipGetLastError());
     AG_HIP(hipMemcpyAsync(yq, yo, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+    return ag_wait_stream(g->dev);
+}
+
+// reps scatter passes of the push into yd (ywords u32 words, zeroed first), timed with HIP
+// events.
+template <class L>
+static int run_push(ag_dgraph *g, uint32_t reps, double *avg_ms, void *yd, uint32_t ywords,
+                    const L &launch)
+{
+    hipStream_t stream = g->dev->stream;
+    const uint32_t zb = (ywords + AG_WG - 1) / AG_WG;
+    zero_u32_kernel<<<zb < 4096u ? (zb ? zb : 1u) : 4096u, AG_WG, 0, stream>>>((uint32_t *)yd,
+                                                                               ywords);
+    hipEvent_t e0 = NULL, e1 = NULL;
+    if (avg_ms) {
+        AG_HIP(hipEventCreate(&e0));
+        AG_HIP(hipEventCreate(&e1));
+        AG_HIP(hipEventRecord(e0, stream));
+    }
+    const uint32_t grid = g->dev->num_cus * 16u;
+    for (uint32_t i = 0; i < reps && g->m; i++)
+        launch(grid);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && avg_ms)
+        e = hipEventRecord(e1, stream);
+    int st = (e == hipSuccess) ? ag_wait_stream(g->dev) : ag_hip_fail(e, "push launch", __FILE__, __LINE__);
+    if (st == AG_OK && avg_ms) {
+        float ms = 0.0f;
+        e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess)
+            st = ag_hip_fail(e, "hipEventElapsedTime", __FILE__, __LINE__);
+        *avg_ms = reps ? (double)ms / reps : 0.0;
+    }
+    if (e0)
+        hipEventDestroy(e0);
+    if (e1)
+        hipEventDestroy(e1);
+    return st;
+}
+
+extern "C" int ag_spmv_push(ag_dgraph *g, const float *x, float *y, uint32_t reps, double *avg_ms)
+{
+    if (!g || (g->n && (!x || !y)))
+        return ag_set_error(AG_ERR_INVALID, "null argument");
+    if (avg_ms)
+        *avg_ms = 0.0;
+    const uint32_t n = g->n;
+    if (n == 0)
+        return AG_OK;
+    AG_HIP(hipSetDevice(g->dev->ordinal));
+    int st = ag_graph_forward(g);
+    if (st == AG_OK)
+        st = ag_ensure_vec(g, (size_t)n * sizeof(uint64_t));
+    if (st != AG_OK)
+        return st;
+    hipStream_t stream = g->dev->stream;
+    float *xo = (float *)g->vec_a;
+    float *xd = (float *)g->vec_b;
+    float *yd = (float *)((char *)g->vec_b + (size_t)n * sizeof(float));
+    const uint32_t blk = (n + AG_WG - 1) / AG_WG;
+    AG_HIP(hipMemcpyAsync(xo, x, (size_t)n * sizeof(float), hipMemcpyHostToDevice, stream));
+    permute_in_kernel<float><<<blk, AG_WG, 0, stream>>>(n, xo, g->d2o, xd);
+    AG_HIP(hipGetLastError());
+    const uint32_t m = g->m;
+    const float *fw = g->fwd_w;
+    st = run_push(g, reps, avg_ms, yd, n, [&](uint32_t grid) {
+        if (fw)
+            spmv_push_kernel<true><<<grid, AG_WG, 0, stream>>>(m, g->fwd_src, g->fwd_dst, fw, xd, yd);
+        else
+            spmv_push_kernel<false><<<grid, AG_WG, 0, stream>>>(m, g->fwd_src, g->fwd_dst, NULL, xd,
+                                                                yd);
+    });
+    if (st != AG_OK)
+        return st;
+    permute_out_kernel<float><<<blk, AG_WG, 0, stream>>>(n, yd, g->d2o, xo);
+    AG_HIP(hipGetLastError());
+    AG_HIP(hipMemcpyAsync(y, xo, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, stream));
+    return ag_wait_stream(g->dev);
+}
+
+extern "C" int ag_spmv_push_fixed(ag_dgraph *g, const uint64_t *xq, uint64_t *yq, uint32_t reps,
+                                  double *avg_ms)
+{
+    if (!g || (g->n && (!xq || !yq)))
+        return ag_set_error(AG_ERR_INVALID, "null argument");
+    if (avg_ms)
+        *avg_ms = 0.0;
+    const uint32_t n = g->n;
+    if (n == 0)
+        return AG_OK;
+    AG_HIP(hipSetDevice(g->dev->ordinal));
+    int st = ag_graph_forward(g);
+    if (st == AG_OK)
+        st = ag_ensure_vec(g, (size_t)n * sizeof(uint64_t) * 2);
+    if (st != AG_OK)
+        return st;
+    hipStream_t stream = g->dev->stream;
+    uint64_t *xo = (uint64_t *)g->vec_a;
+    uint64_t *xd = (uint64_t *)g->vec_b;
+    uint64_t *yd = (uint64_t *)((char *)g->vec_b + (size_t)n * sizeof(uint64_t));
+    const uint32_t blk = (n + AG_WG - 1) / AG_WG;
+    AG_HIP(hipMemcpyAsync(xo, xq, (size_t)n * sizeof(uint64_t), hipMemcpyHostToDevice, stream));
+    permute_in_kernel<uint64_t><<<blk, AG_WG, 0, stream>>>(n, xo, g->d2o, xd);
+    AG_HIP(hipGetLastError());
+    const uint32_t m = g->m;
+    const float *fw = g->fwd_w;
+    const uint64_t wc = DoubleToFixed64((double)0.0001f);   // SPMV.c:1080
+    st = run_push(g, reps, avg_ms, yd, 2u * n, [&](uint32_t grid) {
+        if (fw)
+            spmv_push_fixed_kernel<true><<<grid, AG_WG, 0, stream>>>(m, g->fwd_src, g->fwd_dst, fw,
+                                                                     xd, 0ull, yd);
+        else
+            spmv_push_fixed_kernel<false><<<grid, AG_WG, 0, stream>>>(m, g->fwd_src, g->fwd_dst,
+                                                                      NULL, xd, wc, yd);
+    });
+    if (st != AG_OK)
+        return st;
+    permute_out_kernel<uint64_t><<<blk, AG_WG, 0, stream>>>(n, yd, g->d2o, xo);
+    AG_HIP(hipGetLastError());
+    AG_HIP(hipMemcpyAsync(yq, xo, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
     return ag_wait_stream(g->dev);
 }
 

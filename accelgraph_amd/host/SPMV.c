@@ -155,19 +155,86 @@ struct SPMVStats *SPMVPullFixedPointGraphCSR(struct Arguments *arguments, struct
     return stats;
 }
 
-/* SPMV.c:594-621. Only the pull directions are provided (pushpull 1 and 3 are push). */
+/* SPMV.c:752-848: the output accumulates over the iterations (never cleared in between) */
+struct SPMVStats *SPMVPushGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    struct SPMVStats *stats = newSPMVStatsGraphCSR(graph);
+    stats->arguments = arguments;
+    ag_dgraph *dg = accelgraphUploadGraph(graph, 1);
+
+    printf(" -----------------------------------------------------\n");
+    printf("| %-51s | \n", "Starting SPMV-PUSH");
+    printf(" -----------------------------------------------------\n");
+    printf("| %-21s | %-27s | \n", "Iteration", "Time (S)");
+    printf(" -----------------------------------------------------\n");
+
+    reference_input(graph, stats->vector_input);
+    double t0 = wall_s();
+    double avg_ms = 0.0;
+    uint32_t reps = arguments->iterations;
+    int st = ag_spmv_push(dg, stats->vector_input, stats->vector_output, reps, &avg_ms);
+    if (st != AG_OK)
+        accelgraphFail("ag_spmv_push", st);
+    for (stats->iterations = 0; stats->iterations < reps; stats->iterations++)
+        printf("| %-21u | %-27f | \n", stats->iterations, avg_ms * 1e-3);
+    double sum = checksum(stats->vector_output, graph->num_vertices);
+    stats->time_total = wall_s() - t0;
+    print_tail(stats, sum);
+    return stats;
+}
+
+/* SPMV.c:1017-1136: Q32.32 x and weights, MULFixed64V1 products, u64 sums accumulated over
+ * the iterations, converted back with Fixed64ToDouble */
+struct SPMVStats *SPMVPushFixedPointGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    struct SPMVStats *stats = newSPMVStatsGraphCSR(graph);
+    stats->arguments = arguments;
+    ag_dgraph *dg = accelgraphUploadGraph(graph, 1);
+    uint32_t n = graph->num_vertices;
+
+    printf(" -----------------------------------------------------\n");
+    printf("| %-51s | \n", "Starting SPMV-PUSH Fixed-Point");
+    printf(" -----------------------------------------------------\n");
+    printf("| %-21s | %-27s | \n", "Iteration", "Time (S)");
+    printf(" -----------------------------------------------------\n");
+
+    reference_input(graph, stats->vector_input);
+    uint64_t *xq = (uint64_t *)calloc(n ? n : 1, sizeof(uint64_t));
+    uint64_t *yq = (uint64_t *)calloc(n ? n : 1, sizeof(uint64_t));
+    if (!xq || !yq)
+        accelgraphFail("SPMVPushFixedPointGraphCSR buffers", AG_ERR_OOM);
+    for (uint32_t v = 0; v < n; v++)
+        xq[v] = DoubleToFixed64(stats->vector_input[v]);
+    double t0 = wall_s();
+    double avg_ms = 0.0;
+    uint32_t reps = arguments->iterations;
+    int st = ag_spmv_push_fixed(dg, xq, yq, reps, &avg_ms);
+    if (st != AG_OK)
+        accelgraphFail("ag_spmv_push_fixed", st);
+    for (stats->iterations = 0; stats->iterations < reps; stats->iterations++)
+        printf("| %-21u | %-27f | \n", stats->iterations, avg_ms * 1e-3);
+    for (uint32_t v = 0; v < n; v++)
+        stats->vector_output[v] = (float)Fixed64ToDouble(yq[v]);
+    double sum = checksum(stats->vector_output, n);
+    stats->time_total = wall_s() - t0;
+    free(xq);
+    free(yq);
+    print_tail(stats, sum);
+    return stats;
+}
+
+/* SPMV.c:594-621 */
 struct SPMVStats *SPMVGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
 {
     switch (arguments->pushpull) {
     case 0:
         return SPMVPullGraphCSR(arguments, graph);
+    case 1:
+        return SPMVPushGraphCSR(arguments, graph);
     case 2:
         return SPMVPullFixedPointGraphCSR(arguments, graph);
-    case 1:
     case 3:
-        fprintf(stderr, "[accelgraph] SPMVGraphCSR: pushpull=%u is not a pull variant "
-                        "provided by this library\n", arguments->pushpull);
-        exit(EXIT_FAILURE);
+        return SPMVPushFixedPointGraphCSR(arguments, graph);
     default:
         return SPMVPullGraphCSR(arguments, graph);
     }

@@ -8,6 +8,8 @@
  *   SPMVGraphCSR                  SPMV.h:51      (SPMV.c:594-621)
  *   SPMVPullGraphCSR              SPMV.h:52      (SPMV.c:622-751)
  *   SPMVPullFixedPointGraphCSR    SPMV.h:55      (SPMV.c:849-1016)
+ *   SPMVPushGraphCSR              SPMV.h:53      (SPMV.c:752-848)
+ *   SPMVPushFixedPointGraphCSR    SPMV.h:56      (SPMV.c:1017-1136)
  *
  * y = A^T-pull of x over the inverse CSR: y[v] = sum over in-edges (u->v) of w(u,v)*x[u].
  * When the graph carries no weights every edge weighs 0.0001f, the reference's
@@ -41,6 +43,8 @@ void freeSPMVStats(struct SPMVStats *stats);
 struct SPMVStats *SPMVGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
 struct SPMVStats *SPMVPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
 struct SPMVStats *SPMVPullFixedPointGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+struct SPMVStats *SPMVPushGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+struct SPMVStats *SPMVPushFixedPointGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
 
 #ifdef __cplusplus
 }

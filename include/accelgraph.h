@@ -246,6 +246,16 @@ int ag_spmv_pull(ag_dgraph *g, const float *x, float *y, uint32_t repetitions,
  * w_q = FloatToFixed32(w) per edge, or the codes given to ag_graph_set_fixed_weights. */
 int ag_spmv_pull_fixed(ag_dgraph *g, const uint32_t *x_q, uint64_t *y_q,
                        uint32_t repetitions, double *avg_kernel_ms);
+/* Push forms (SPMVPushGraphCSR / SPMVPushFixedPointGraphCSR, SPMV.c:752-848, 1017-1136):
+ * every source adds w*x[src] to its out-neighbours with atomics. As in the reference the
+ * output is not cleared between repetitions, so y = the sum of `repetitions` products
+ * (0 repetitions: y = 0). FP32: binary32 products, added in the atomics' order. Fixed point:
+ * x_q and y_q in Q32.32 (DoubleToFixed64), weights DoubleToFixed64(w) (0.0001f unweighted),
+ * products MULFixed64V1, exact u64 sums. The forward CSR is built on the device on first use. */
+int ag_spmv_push(ag_dgraph *g, const float *x, float *y, uint32_t repetitions,
+                 double *avg_kernel_ms);
+int ag_spmv_push_fixed(ag_dgraph *g, const uint64_t *x_q, uint64_t *y_q, uint32_t repetitions,
+                       double *avg_kernel_ms);
 
 /* ---------------------------------------------------------------- personalised PageRank */
 

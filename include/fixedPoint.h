@@ -51,6 +51,13 @@ static inline double Fixed64ToDouble(uint64_t q)
     return (double)q * (1.0 / 4294967296.0);
 }
 
+/* Q32.32 product used by the fixed-point SpMV push (SPMV.c:1092); OpenGraph's definition is
+ * absent, so here it is the exact 128-bit product shifted right by 32, truncated to 64 bits. */
+static inline uint64_t MULFixed64V1(uint64_t a, uint64_t b)
+{
+    return (uint64_t)(((unsigned __int128)a * b) >> 32);
+}
+
 static inline uint32_t FloatToFixed32(float x)
 {
     double s = (double)x * 65536.0;

@@ -61,6 +61,12 @@ def lib() -> C.CDLL:
         L.or_pagerank_push.argtypes = [u32, P(u32), P(u32), P(u32), P(u32), u32, u32, f64, u32,
                                        f64, u32, P(f32), C.c_int, P(f32), P(f32), P(u32),
                                        P(f64), P(_PrOut)]
+        L.or_spmv_push_f32.argtypes = [u32, P(u32), P(u32), P(u32), P(f32), P(f32), u32,
+                                       C.c_int, P(f32)]
+        L.or_spmv_push_fixed.argtypes = [u32, P(u32), P(u32), P(u32), P(f32), P(u64), u32,
+                                         C.c_int, P(u64)]
+        L.or_mul_fixed64_check.restype = u64
+        L.or_mul_fixed64_check.argtypes = [u64, u64]
         L.or_quantize.restype = f32
         L.or_quantize.argtypes = [u32, P(f32), u32, P(u64)]
         L.or_pagerank_f64.restype = C.c_int
@@ -133,6 +139,32 @@ def spmv_fixed(idx, deg, src, wq, wconst, xq, threads: int = 0) -> np.ndarray:
     lib().or_spmv_fixed(idx.size, _p(idx, u32), _p(deg, u32), _p(src, u32), _p(wq, u32),
                         wconst, _p(xq, u32), _p(y, u64), threads)
     return y
+
+
+def spmv_push_f32(fwd_idx, fwd_deg, fwd_dst, fw, x, reps: int = 1, threads: int = 0):
+    """SpMV push (SPMV.c:752-848): reps products accumulated onto y = 0."""
+    fwd_idx, fwd_deg, fwd_dst = _u32(fwd_idx), _u32(fwd_deg), _u32(fwd_dst)
+    fw = None if fw is None else np.ascontiguousarray(fw, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.zeros(max(x.size, 1), np.float32)
+    lib().or_spmv_push_f32(x.size, _p(fwd_idx, u32), _p(fwd_deg, u32), _p(fwd_dst, u32),
+                           _p(fw, f32), _p(x, f32), reps, threads, _p(y, f32))
+    return y[:x.size]
+
+
+def spmv_push_fixed(fwd_idx, fwd_deg, fwd_dst, fw, xq, reps: int = 1, threads: int = 0):
+    """Fixed-point SpMV push (SPMV.c:1017-1136): Q32.32, MULFixed64V1 products."""
+    fwd_idx, fwd_deg, fwd_dst = _u32(fwd_idx), _u32(fwd_deg), _u32(fwd_dst)
+    fw = None if fw is None else np.ascontiguousarray(fw, np.float32)
+    xq = np.ascontiguousarray(xq, np.uint64)
+    y = np.zeros(max(xq.size, 1), np.uint64)
+    lib().or_spmv_push_fixed(xq.size, _p(fwd_idx, u32), _p(fwd_deg, u32), _p(fwd_dst, u32),
+                             _p(fw, f32), _p(xq, u64), reps, threads, _p(y, u64))
+    return y[:xq.size]
+
+
+def mul_fixed64(a: int, b: int) -> int:
+    return int(lib().or_mul_fixed64_check(a, b))
 
 
 @dataclass

@@ -21,6 +21,7 @@
  *   or_spmv_f32/fixed  SPMV.c:622-751 / :849-1016 with the FPGA datapath arithmetic
  *                      (FloatPoint cu_sum_kernel_control.sv:345-367: fp32 product then stream
  *                      add; FixedPoint :155-164: (u32)((x*w) >> 16) into a u64 sum).
+ *   or_spmv_push_*     the SpMV push directions, SPMV.c:752-848 / :1017-1136.
  *   or_gather_*        the bare sum kernel (cu_sum_kernel_control.sv), float and integer.
  *   or_build_csr       an independent sort-based CSR + inverse-CSR builder (the reference's
  *                      builder lives in the absent OpenGraph submodule).
@@ -154,6 +155,57 @@ void or_spmv_fixed(uint32_t n, const uint32_t *idx, const uint32_t *deg, const u
         yq[v] = acc;
     }
 }
+
+/* SPMVPushGraphCSR / SPMVPushFixedPointGraphCSR (SPMV.c:752-848, 1017-1136): every source v
+ * adds w * x[v] to its out-neighbours with `omp atomic` (:815-816, :1091-1092) along the forward
+ * CSR (fwd_idx/fwd_deg/fwd_dst, weights fw in the same order or NULL for 0.0001f); y is never
+ * cleared between iterations, so `reps` products accumulate onto y = 0. Fixed point: x and w
+ * in Q32.32 (DoubleToFixed64), products MULFixed64V1 = the 128-bit product >> 32 (the
+ * library's definition: OpenGraph's is absent), exact u64 sums. Sequentially each y[u] adds
+ * its terms in ascending source order, the inverse CSR's order. */
+static uint64_t or_mul_fixed64(uint64_t a, uint64_t b)
+{
+    return (uint64_t)(((unsigned __int128)a * b) >> 32);
+}
+
+void or_spmv_push_f32(uint32_t n, const uint32_t *fwd_idx, const uint32_t *fwd_deg,
+                      const uint32_t *fwd_dst, const float *fw, const float *x, uint32_t reps,
+                      int threads, float *y)
+{
+    memset(y, 0, (size_t)n * sizeof(float));
+    for (uint32_t r = 0; r < reps; r++) {
+        #pragma omp parallel for schedule(dynamic, 1024) if (threads > 0) num_threads(threads > 0 ? threads : 1)
+        for (uint32_t v = 0; v < n; v++) {
+            for (uint32_t k = 0; k < fwd_deg[v]; k++) {
+                const uint32_t j = fwd_idx[v] + k;
+                const float t = (fw ? fw[j] : 0.0001f) * x[v];
+                #pragma omp atomic
+                y[fwd_dst[j]] += t;
+            }
+        }
+    }
+}
+
+void or_spmv_push_fixed(uint32_t n, const uint32_t *fwd_idx, const uint32_t *fwd_deg,
+                        const uint32_t *fwd_dst, const float *fw, const uint64_t *xq,
+                        uint32_t reps, int threads, uint64_t *y)
+{
+    const uint64_t wc = or_fixed_encode(64, 0.0001f);
+    memset(y, 0, (size_t)n * sizeof(uint64_t));
+    for (uint32_t r = 0; r < reps; r++) {
+        #pragma omp parallel for schedule(dynamic, 1024) if (threads > 0) num_threads(threads > 0 ? threads : 1)
+        for (uint32_t v = 0; v < n; v++) {
+            for (uint32_t k = 0; k < fwd_deg[v]; k++) {
+                const uint32_t j = fwd_idx[v] + k;
+                const uint64_t t = or_mul_fixed64(fw ? or_fixed_encode(64, fw[j]) : wc, xq[v]);
+                #pragma omp atomic
+                y[fwd_dst[j]] += t;
+            }
+        }
+    }
+}
+
+uint64_t or_mul_fixed64_check(uint64_t a, uint64_t b) { return or_mul_fixed64(a, b); }
 
 /* ------------------------------------------------------------------ PageRank */
 

@@ -96,7 +96,7 @@ def test_pagerank_entry_points(graph, pushpull):
 
 
 def test_spmv_entry_points(graph):
-    say("SPMVGraphCSR pushpull 0 and 2")
+    say("SPMVGraphCSR pushpull 0, 2, 1 and 3")
     L = ag.lib()
     g = graph
     x = np.where(g.out_degree > 0, np.float32(1) / np.maximum(g.out_degree, 1).astype(np.float32),
@@ -118,4 +118,21 @@ def test_spmv_entry_points(graph):
     st = L.SPMVGraphCSR(C.byref(a), g.ptr)
     y = np.ctypeslib.as_array(st.contents.vector_output, shape=(g.n,)).copy()
     np.testing.assert_array_equal(y, oracle.fixed_decode(32, yq).astype(np.float32))
+    L.freeSPMVStats(st)
+
+    # push (1): three accumulated products; fixed push (3): Q32.32, two products
+    fwd = (g.edges_idx, g.out_degree, g.fwd_dest)
+    a = arguments(1, iterations=3)
+    st = L.SPMVGraphCSR(C.byref(a), g.ptr)
+    y = np.ctypeslib.as_array(st.contents.vector_output, shape=(g.n,)).copy()
+    ref = oracle.spmv_push_f32(*fwd, None, x, reps=3)
+    bound = (3 * g.in_degree.astype(np.float64) + 2) * 2.0 ** -24 * 2 * np.abs(ref)
+    assert st.contents.iterations == 3
+    assert np.all(np.abs(y.astype(np.float64) - ref) <= bound + 1e-30)
+    L.freeSPMVStats(st)
+    a = arguments(3, iterations=2)
+    st = L.SPMVGraphCSR(C.byref(a), g.ptr)
+    y = np.ctypeslib.as_array(st.contents.vector_output, shape=(g.n,)).copy()
+    yq = oracle.spmv_push_fixed(*fwd, None, oracle.fixed_encode(64, x), reps=2)
+    np.testing.assert_array_equal(y, oracle.fixed_decode(64, yq).astype(np.float32))
     L.freeSPMVStats(st)

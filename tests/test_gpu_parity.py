@@ -573,6 +573,31 @@ def test_spmv_f32_and_fixed(device, pname):
     dg.close()
 
 
+@pytest.mark.parametrize("weighted", [False, True])
+@pytest.mark.parametrize("pname", ["default", "blocked"])
+def test_spmv_push(device, pname, weighted):
+    """SpMV push: repetitions accumulate; FP32 within the rounding of reps*d additions taken
+    in the atomics' order, Q32.32 fixed point bit-exact with the oracle."""
+    say(f"spmv push {pname} weighted={weighted}")
+    g = ag.Graph.rmat(14)
+    rng = np.random.default_rng(21)
+    if weighted:
+        g.set_weights(rng.random(g.m).astype(np.float32))
+    fw = g.fwd_weight if weighted else None
+    fwd = (g.edges_idx, g.out_degree, g.fwd_dest)
+    dg = upload(device, g, pname, weights=weighted)
+    x = rng.random(g.n).astype(np.float32)
+    xq = oracle.fixed_encode(64, x)
+    for reps in (0, 1, 3):
+        y, _ = dg.spmv_push(x, reps)
+        ref = oracle.spmv_push_f32(*fwd, fw, x, reps)
+        bound = (reps * g.in_degree.astype(np.float64) + 2) * 2.0 ** -24 * 2 * np.abs(ref)
+        assert np.all(np.abs(y.astype(np.float64) - ref) <= bound + 1e-30)
+        yq, _ = dg.spmv_push_fixed(xq, reps)
+        np.testing.assert_array_equal(yq, oracle.spmv_push_fixed(*fwd, fw, xq, reps))
+    dg.close()
+
+
 def test_spmv_unweighted(device):
     say("spmv rmat-14 unweighted")
     g = ag.Graph.rmat(14)

@@ -367,3 +367,49 @@ def test_dd_pull_threads_equal_sequential():
     np.testing.assert_array_equal(a.scaled.view(np.uint32), b.scaled.view(np.uint32))
     np.testing.assert_array_equal(a.hist_active, b.hist_active)
     np.testing.assert_allclose(a.hist_err, b.hist_err, rtol=1e-12)
+
+
+# ------------------------------------------------------------------ SpMV push
+
+def test_mul_fixed64_is_the_q32_32_product():
+    """MULFixed64V1 as defined here: bits [32, 96) of the exact 128-bit product."""
+    rng = np.random.default_rng(4)
+    vals = [0, 1, (1 << 32), (1 << 32) + 1, (1 << 64) - 1, 0x123456789ABCDEF0,
+            *rng.integers(0, 2 ** 63, 40, dtype=np.uint64).tolist()]
+    for a in vals:
+        for b in vals[:12]:
+            assert oracle.mul_fixed64(a, b) == ((a * b) >> 32) % (1 << 64)
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_spmv_push_sequential(weighted):
+    """Sequentially the push adds each y[u]'s terms in ascending source order, the inverse
+    CSR's: one repetition equals the pull bit for bit. Repetitions accumulate (fixed point:
+    exactly reps times one product; FP32 within rounding), and the fixed and FP32 results
+    agree to the Q32.32 resolution."""
+    import accelgraph_amd as ag
+    g = ag.Graph.rmat(11)
+    rng = np.random.default_rng(8)
+    if weighted:
+        g.set_weights(rng.random(g.m).astype(np.float32))
+    fw, iw = g.fwd_weight, g.inv_weight
+    x = rng.random(g.n).astype(np.float32)
+    fwd = (g.edges_idx, g.out_degree, g.fwd_dest)
+    y1 = oracle.spmv_push_f32(*fwd, fw, x)
+    pull = oracle.spmv_f32(g.inv_idx, g.in_degree, g.inv_src, iw, x)
+    np.testing.assert_array_equal(y1.view(np.uint32), pull.view(np.uint32))
+    y3 = oracle.spmv_push_f32(*fwd, fw, x, reps=3)
+    # 3d float additions of positive terms: relative error below about 3d * 2^-24
+    bound = (3 * g.in_degree.astype(np.float64) + 3) * 2.0 ** -24 * 3 * np.abs(y1) + 1e-30
+    assert np.all(np.abs(y3 - 3 * y1.astype(np.float64)) <= bound)
+    np.testing.assert_array_equal(oracle.spmv_push_f32(*fwd, fw, x, reps=0), np.zeros(g.n))
+    xq = oracle.fixed_encode(64, x)
+    q1 = oracle.spmv_push_fixed(*fwd, fw, xq)
+    q3 = oracle.spmv_push_fixed(*fwd, fw, xq, reps=3)
+    np.testing.assert_array_equal(q3, q1 * np.uint64(3))
+    np.testing.assert_array_equal(oracle.spmv_push_fixed(*fwd, fw, xq, threads=4), q1)
+    deg = np.maximum(g.in_degree, 1).astype(np.float64)
+    # each Q32.32 term is within 2^-32 of its product (and the product of the FP32 path is
+    # rounded once), so the two sums differ by rounding only
+    diff = np.abs(q1.astype(np.float64) / 2 ** 32 - y1)
+    assert np.all(diff <= 1e-5 * np.abs(y1) + 4 * deg * 2.0 ** -32)
