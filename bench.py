@@ -269,6 +269,15 @@ def spmv_section(ag, dev, scale: int, reps: int, cpu: bool) -> dict:
         out[name] = {"ms": round(ms, 4), "gteps": round(g.m / (ms * 1e-3) / 1e9, 2),
                      "algorithmic_bytes": nb, "gb_per_s": round(gbs, 1),
                      "roofline_frac": round(gbs / HBM_PEAK_GBS, 4)}
+    # push forms (SPMV.c:752-848, 1017-1136): atomics along the forward CSR; bytes: forward
+    # src + dst + weight streamed, x gathered, one 4/8-byte atomic per edge
+    dg.spmv_push(x, reps=1)                                 # warm (builds the forward CSR)
+    xq64 = dev.fixed_encode(64, x)
+    for name, fn, arg, nb in (("push_f32", dg.spmv_push, x, 20 * g.m + 4 * g.n),
+                              ("push_fixed64", dg.spmv_push_fixed, xq64, 28 * g.m + 8 * g.n)):
+        _, ms = fn(arg, reps=reps)
+        out[name] = {"ms": round(ms, 4), "gteps": round(g.m / (ms * 1e-3) / 1e9, 2),
+                     "algorithmic_bytes": nb, "gb_per_s": round(nb / (ms * 1e-3) / 1e9, 1)}
     y, _ = dg.spmv(x)
     # fixed-32 PageRank iteration on the same graph (Q16.16 contributions, exact u64 sums)
     fs = dg.session(kind=ag.KIND_FIXED32, max_iterations=3 + 20, stop_rule=ag.STOP_NONE,
@@ -294,6 +303,11 @@ def spmv_section(ag, dev, scale: int, reps: int, cpu: bool) -> dict:
         out["cpu_baseline"] = {"ms": round(dt * 1e3, 2), "cores": threads, "kind": "port",
                                "gteps": round(g.m / dt / 1e9, 3)}
         out["f32"]["max_rel_err_vs_cpu"] = float(np.max(np.abs(y - ref) / np.maximum(np.abs(ref), 1e-30)))
+        t0 = time.perf_counter()
+        oracle.spmv_push_f32(g.edges_idx, g.out_degree, g.fwd_dest, g.fwd_weight, x, 1, threads)
+        dt = time.perf_counter() - t0
+        out["push_f32"]["cpu_ms"] = round(dt * 1e3, 2)
+        out["push_f32"]["gpu_speedup"] = round(dt * 1e3 / out["push_f32"]["ms"], 1)
     return out
 
 
