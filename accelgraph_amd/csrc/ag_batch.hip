@@ -113,7 +113,7 @@ __global__ __launch_bounds__(AG_BATCH_WG) void batch_kernel(BatchArgs A)
         for (it = 0; it < cfg.max_iter; it++) {
             const float *cin = (it & 1u) ? c1 : c0;
             float *cout = (it & 1u) ? c0 : c1;
-            Red red{0.0, 0u};
+            Red red{};
             for (uint32_t r = tid; r < n; r += AG_BATCH_WG) {
                 const uint32_t b = off[r], e = off[r + 1];
                 if (e - b > AG_BATCH_HEAVY_DEG)
@@ -200,16 +200,16 @@ extern "C" int ag_batch_free(ag_batch *b)
 {
     if (!b)
         return AG_OK;
-    hipSetDevice(b->dev->ordinal);
-    hipStreamSynchronize(b->dev->stream);
+    (void)hipSetDevice(b->dev->ordinal);
+    (void)hipStreamSynchronize(b->dev->stream);
     if (b->captured) {
-        hipGraphExecDestroy(b->exec);
-        hipGraphDestroy(b->graph);
+        (void)hipGraphExecDestroy(b->exec);
+        (void)hipGraphDestroy(b->graph);
     }
     for (int i = 0; i < b->nalloc; i++)
-        hipFree(b->allocs[i]);
+        (void)hipFree(b->allocs[i]);
     if (b->cfg_host)
-        hipHostFree(b->cfg_host);
+        (void)hipHostFree(b->cfg_host);
     if (b->dev->live_graphs)
         b->dev->live_graphs--;
     free(b);
@@ -412,9 +412,9 @@ extern "C" int ag_batch_pagerank(ag_batch *b, const ag_batch_options *opts, floa
                     hipError_t ce = hipStreamEndCapture(s, &b->graph);
                     if (est != AG_OK) {
                         if (ce == hipSuccess)
-                            hipGraphDestroy(b->graph);
-                        hipEventDestroy(e0);
-                        hipEventDestroy(e1);
+                            (void)hipGraphDestroy(b->graph);
+                        (void)hipEventDestroy(e0);
+                        (void)hipEventDestroy(e1);
                         return est;
                     }
                     he = ce;
@@ -429,8 +429,8 @@ extern "C" int ag_batch_pagerank(ag_batch *b, const ag_batch_options *opts, floa
         } else {
             int est = batch_enqueue(b);
             if (est != AG_OK) {
-                hipEventDestroy(e0);
-                hipEventDestroy(e1);
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
                 return est;
             }
         }
@@ -453,7 +453,7 @@ extern "C" int ag_batch_pagerank(ag_batch *b, const ag_batch_options *opts, floa
             st = ag_hip_fail(he, "hipEventElapsedTime", __FILE__, __LINE__);
         *device_ms = ms;
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     return st;
 }

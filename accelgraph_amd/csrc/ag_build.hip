@@ -302,7 +302,7 @@ struct Pool {
     void release(void)
     {
         for (int i = 0; i < k; i++)
-            hipFree(p[i]);
+            (void)hipFree(p[i]);
         k = 0;
     }
 };
@@ -319,7 +319,7 @@ int device_max(ag_device *dev, uint64_t m, const T *a, const T *b, T *out)
     if (e == hipSuccess)
         e = hipMemcpyAsync(host, part, nb * sizeof(T), hipMemcpyDeviceToHost, dev->stream);
     int st = (e == hipSuccess) ? ag_wait_stream(dev) : ag_hip_fail(e, "max", __FILE__, __LINE__);
-    hipFree(part);
+    (void)hipFree(part);
     if (st != AG_OK)
         return st;
     T v = (T)0;
@@ -341,7 +341,7 @@ int sort_pairs(ag_device *dev, uint64_t m, uint64_t *key, uint64_t *key2, uint32
     hipError_t e = rocprim::radix_sort_pairs(t, tmp, key, key2, val, val2, (size_t)m, 0u, end_bit,
                                              dev->stream);
     int st = (e == hipSuccess) ? ag_wait_stream(dev) : ag_hip_fail(e, "radix sort", __FILE__, __LINE__);
-    hipFree(t);
+    (void)hipFree(t);
     return st;
 }
 
@@ -355,7 +355,7 @@ int exclusive_scan_u32(ag_device *dev, uint64_t m, const uint32_t *in, uint32_t 
     hipError_t e = rocprim::exclusive_scan(t, tmp, in, out, 0u, (size_t)m,
                                            rocprim::plus<uint32_t>(), dev->stream);
     int st = (e == hipSuccess) ? ag_wait_stream(dev) : ag_hip_fail(e, "scan", __FILE__, __LINE__);
-    hipFree(t);
+    (void)hipFree(t);
     return st;
 }
 
@@ -639,7 +639,7 @@ int ag_graph_forward(ag_dgraph *g)
         else
             st = ag_hip_fail(e, "forward CSR sort", __FILE__, __LINE__);
         if (t)
-            hipFree(t);
+            (void)hipFree(t);
     }
     if (st == AG_OK && n) {
         offsets_kernel<<<grid_of(n, BUILD_WG), BUILD_WG, 0, dev->stream>>>(n, m, fsrc, off, deg);
@@ -655,10 +655,10 @@ int ag_graph_forward(ag_dgraph *g)
     }
     T.release();
     if (st != AG_OK) {
-        hipFree(off);
-        hipFree(fsrc);
-        hipFree(fdst);
-        hipFree(fw);
+        (void)hipFree(off);
+        (void)hipFree(fsrc);
+        (void)hipFree(fdst);
+        (void)hipFree(fw);
         return st;
     }
     g->fwd_off = off;
@@ -692,7 +692,7 @@ extern "C" int ag_generate_rmat_edges(ag_device *dev, uint32_t scale, uint64_t m
         e = hipMemcpyAsync(dst_out, d + m, m * 4, hipMemcpyDeviceToHost, dev->stream);
     if (st == AG_OK)
         st = (e == hipSuccess) ? ag_wait_stream(dev) : ag_hip_fail(e, "edge download", __FILE__, __LINE__);
-    hipFree(d);
+    (void)hipFree(d);
     return st;
 }
 
@@ -714,7 +714,7 @@ extern "C" int ag_generate_er_edges(ag_device *dev, uint32_t n, uint64_t m, uint
         e = hipMemcpyAsync(dst_out, d + m, m * 4, hipMemcpyDeviceToHost, dev->stream);
     if (st == AG_OK)
         st = (e == hipSuccess) ? ag_wait_stream(dev) : ag_hip_fail(e, "edge download", __FILE__, __LINE__);
-    hipFree(d);
+    (void)hipFree(d);
     return st;
 }
 
@@ -753,9 +753,9 @@ extern "C" int ag_graph_csr_build(ag_device *dev, uint32_t n, uint64_t m, const 
         st = ag_set_error(AG_ERR_INVALID, "vertex id %u >= num_vertices %u", mx, n);
     if (st == AG_OK)
         st = build_on_device(dev, n, m, d, d + m, dw, simplify, out);
-    hipFree(d);
+    (void)hipFree(d);
     if (dw)
-        hipFree(dw);
+        (void)hipFree(dw);
     return st;
 }
 
@@ -779,7 +779,7 @@ extern "C" int ag_graph_csr_generate_rmat(ag_device *dev, uint32_t scale, uint32
     st = generate_device(dev, m, seed, scale, f, d, d + m);
     if (st == AG_OK)
         st = build_on_device(dev, 1u << scale, m, d, d + m, NULL, simplify, out);
-    hipFree(d);
+    (void)hipFree(d);
     return st;
 }
 
@@ -795,6 +795,6 @@ extern "C" int ag_graph_csr_generate_er(ag_device *dev, uint32_t n, uint64_t m, 
     int st = generate_device(dev, m, seed, 2u, ErEdge{n}, d, d + m);
     if (st == AG_OK)
         st = build_on_device(dev, n, m, d, d + m, NULL, simplify, out);
-    hipFree(d);
+    (void)hipFree(d);
     return st;
 }

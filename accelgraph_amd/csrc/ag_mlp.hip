@@ -73,8 +73,6 @@ __device__ __forceinline__ uint16_t bf16_rne(float x)
     return (uint16_t)(u >> 16);
 }
 
-__device__ __forceinline__ float bf16_val(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
-
 // A fragment from a row-major bf16 matrix M (leading dimension ld): A[r][k] = M[r][k]
 __device__ __forceinline__ mlp_s4 frag_a(const uint16_t *M, int ld, int r0, int k0, int l)
 {
@@ -483,9 +481,9 @@ static int mlp_grow(ag_mlp *m, uint32_t S)
 {
     if (m->cap >= S)
         return AG_OK;
-    if (m->X) hipFree(m->X);
-    if (m->y) hipFree(m->y);
-    if (m->perm) hipFree(m->perm);
+    if (m->X) (void)hipFree(m->X);
+    if (m->y) (void)hipFree(m->y);
+    if (m->perm) (void)hipFree(m->perm);
     m->X = NULL;
     m->y = NULL;
     m->perm = NULL;
@@ -509,8 +507,8 @@ static int mlp_part(ag_mlp *m, uint32_t grid)
     const size_t need = (size_t)grid * MLP_WAVES;
     if (m->part_cap >= need)
         return AG_OK;
-    if (m->part) hipFree(m->part);
-    if (m->loss_part) hipFree(m->loss_part);
+    if (m->part) (void)hipFree(m->part);
+    if (m->loss_part) (void)hipFree(m->loss_part);
     m->part = NULL;
     m->loss_part = NULL;
     m->part_cap = 0;
@@ -538,7 +536,7 @@ static int mlp_upload(ag_mlp *m, const float *X, const float *y, uint32_t S)
     if (e == hipSuccess && y)
         e = hipMemcpyAsync(m->y, y, (size_t)S * sizeof(float), hipMemcpyHostToDevice, s);
     st = (e == hipSuccess) ? ag_wait_stream(m->dev) : ag_hip_fail(e, "MLP upload", __FILE__, __LINE__);
-    hipFree(tmp);
+    (void)hipFree(tmp);
     return st;
 }
 
@@ -596,12 +594,12 @@ extern "C" int ag_mlp_free(ag_mlp *m)
 {
     if (!m)
         return AG_OK;
-    hipSetDevice(m->dev->ordinal);
-    hipStreamSynchronize(m->dev->stream);
+    (void)hipSetDevice(m->dev->ordinal);
+    (void)hipStreamSynchronize(m->dev->stream);
     void *ptrs[] = {m->p, m->m1, m->m2, m->grad, m->X, m->y, m->perm, m->part, m->loss_part};
     for (void *p : ptrs)
         if (p)
-            hipFree(p);
+            (void)hipFree(p);
     free(m);
     return AG_OK;
 }
@@ -865,11 +863,11 @@ extern "C" int ag_mlp_features(ag_dgraph *g, float *X_out)
     }
     if (s)
         ag_pr_end(s, NULL, NULL);
-    hipStreamSynchronize(hs);
-    if (feat) hipFree(feat);
-    if (r1) hipFree(r1);
-    if (r2) hipFree(r2);
-    if (pred) hipFree(pred);
+    (void)hipStreamSynchronize(hs);
+    if (feat) (void)hipFree(feat);
+    if (r1) (void)hipFree(r1);
+    if (r2) (void)hipFree(r2);
+    if (pred) (void)hipFree(pred);
     return st;
 }
 
@@ -941,15 +939,16 @@ extern "C" int ag_pr_warm_start(ag_pr_session *s, ag_mlp *m, ag_warm_result *res
     }
     if (st == AG_OK && hipEventRecord(e1, hs) == hipSuccess && ag_wait_stream(g->dev) == AG_OK) {
         float ms = 0.0f;
-        hipEventElapsedTime(&ms, e0, e1);
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            ms = 0.0f;   // timing is informational
         res->ms = ms;
     }
-    hipStreamSynchronize(hs);
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
-    if (feat) hipFree(feat);
-    if (r1) hipFree(r1);
-    if (r2) hipFree(r2);
-    if (pred) hipFree(pred);
+    (void)hipStreamSynchronize(hs);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (feat) (void)hipFree(feat);
+    if (r1) (void)hipFree(r1);
+    if (r2) (void)hipFree(r2);
+    if (pred) (void)hipFree(pred);
     return st;
 }

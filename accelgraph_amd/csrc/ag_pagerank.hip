@@ -248,7 +248,7 @@ __device__ __forceinline__ void pr_f32_main(const PullGeom &G, const PrCtl &C,
     if (*(volatile const uint32_t *)&st->done)
         return;
     const uint32_t iter = *(volatile const uint32_t *)&st->iter;
-    Red red{0.0, 0u};
+    Red red{};
     pr_f32_dispatch<HT>(st, iter, A, mixed,
                         [&](const auto &p) { pull_items(G, p, red, s_vals, s_wave); });
     pr_store_partial(C, red);
@@ -262,7 +262,7 @@ __device__ __forceinline__ void pr_f32_fin(const PullGeom &G, const PrCtl &C,
     if (*(volatile const uint32_t *)&st->done)
         return;
     const uint32_t iter = *(volatile const uint32_t *)&st->iter;
-    Red red{0.0, 0u};
+    Red red{};
     pr_f32_dispatch<HT>(st, iter, A, mixed, [&](const auto &p) { pull_hub_finish(G, p, red); });
     pr_close(C, iter, red);
 }
@@ -280,7 +280,7 @@ __device__ __forceinline__ void pr_f32_blk(const PullGeom &U, const BlockGeom &B
     if (*(volatile const uint32_t *)&st->done)
         return;
     const uint32_t iter = *(volatile const uint32_t *)&st->iter;
-    Red red{0.0, 0u};
+    Red red{};
     pr_f32_dispatch<HT>(st, iter, A, mixed, [&](const auto &p) {
         pull_blocked_items(U, B, p, p, red, s_vals, s_wave, L);
     });
@@ -298,7 +298,7 @@ __device__ __forceinline__ void pr_f32_cmb(const BlockGeom &B, const PrCtl &C,
     if (*(volatile const uint32_t *)&st->done)
         return;
     const uint32_t iter = *(volatile const uint32_t *)&st->iter;
-    Red red{0.0, 0u};
+    Red red{};
     pr_f32_dispatch<HT>(st, iter, A, mixed, [&](const auto &p) {
         pull_combine_items(B, p, red, s_vals, s_wave, L);
     });
@@ -549,7 +549,7 @@ __global__ __launch_bounds__(AG_WG) void pr_int_kernel(PullGeom G, PrCtl C, PrF3
     if (*(volatile const uint32_t *)&st->done)
         return;
     const uint32_t iter = *(volatile const uint32_t *)&st->iter;
-    Red red{0.0, 0u};
+    Red red{};
     pull_items(G, int_policy<KC>(st, iter, A), red, s_vals, s_wave);
     pr_store_partial(C, red);
 }
@@ -565,7 +565,7 @@ __global__ __launch_bounds__(AG_WG) void pr_int_blk_kernel(PullGeom U, BlockGeom
     if (*(volatile const uint32_t *)&st->done)
         return;
     const uint32_t iter = *(volatile const uint32_t *)&st->iter;
-    Red red{0.0, 0u};
+    Red red{};
     const auto p = int_policy<KC>(st, iter, A);
     pull_blocked_items(U, B, p, p, red, s_vals, s_wave, L);
     pr_store_partial(C, red);
@@ -581,7 +581,7 @@ __global__ __launch_bounds__(AG_WG) void pr_int_cmb_kernel(BlockGeom B, PrCtl C,
     if (*(volatile const uint32_t *)&st->done)
         return;
     const uint32_t iter = *(volatile const uint32_t *)&st->iter;
-    Red red{0.0, 0u};
+    Red red{};
     pull_combine_items(B, int_policy<KC>(st, iter, A), red, s_vals, s_wave, L);
     pr_store_partial(C, red);
 }
@@ -593,7 +593,7 @@ __global__ __launch_bounds__(AG_FIN_WG) void pr_int_finish(PullGeom G, PrCtl C, 
     if (*(volatile const uint32_t *)&st->done)
         return;
     const uint32_t iter = *(volatile const uint32_t *)&st->iter;
-    Red red{0.0, 0u};
+    Red red{};
     pull_hub_finish(G, int_policy<KC>(st, iter, A), red);
     pr_close(C, iter, red);
 }
@@ -659,7 +659,7 @@ __global__ __launch_bounds__(AG_WG) void pr_ddf_blk_kernel(PullGeom U, BlockGeom
     if (*(volatile const uint32_t *)&st->done)
         return;
     const uint32_t iter = *(volatile const uint32_t *)&st->iter;
-    Red red{0.0, 0u};
+    Red red{};
     const PrDdPol p = dd_policy(iter, A);
     pull_blocked_items(U, B, p, p, red, s_vals, s_wave, L);
     pr_store_partial(C, red);
@@ -674,7 +674,7 @@ __global__ __launch_bounds__(AG_WG) void pr_ddf_cmb_kernel(BlockGeom B, PrCtl C,
     if (*(volatile const uint32_t *)&st->done)
         return;
     const uint32_t iter = *(volatile const uint32_t *)&st->iter;
-    Red red{0.0, 0u};
+    Red red{};
     pull_combine_items(B, dd_policy(iter, A), red, s_vals, s_wave, L);
     pr_store_partial(C, red);
 }
@@ -685,7 +685,7 @@ __global__ __launch_bounds__(AG_FIN_WG) void pr_ddf_finish(PullGeom G, PrCtl C, 
     if (*(volatile const uint32_t *)&st->done)
         return;
     const uint32_t iter = *(volatile const uint32_t *)&st->iter;
-    Red red{0.0, 0u};
+    Red red{};
     pull_hub_finish(G, dd_policy(iter, A), red);
     pr_close(C, iter, red);
 }
@@ -729,7 +729,7 @@ __global__ __launch_bounds__(AG_WG) void pr_push_update_kernel(PrCtl C, PrF32Arg
     if (*(volatile const uint32_t *)&st->done)
         return;
     const uint32_t iter = *(volatile const uint32_t *)&st->iter;
-    Red red{0.0, 0u};
+    Red red{};
     if constexpr (KC == 0) {
         const uint32_t rd = iter & 1u, wr = rd ^ 1u;
         const PrF32Pol<float, float> p{(const float *)A.c32[rd], (float *)A.c32[wr], A.rank,
@@ -759,7 +759,7 @@ __global__ __launch_bounds__(AG_FIN_WG) void pr_plain_finish(PrCtl C)
     if (*(volatile const uint32_t *)&st->done)
         return;
     const uint32_t iter = *(volatile const uint32_t *)&st->iter;
-    pr_close(C, iter, Red{0.0, 0u});
+    pr_close(C, iter, Red{});
 }
 
 // ---------------------------------------------------------------- data-driven pull
@@ -801,7 +801,7 @@ __global__ __launch_bounds__(AG_WG) void pr_dd_gather_kernel(PrCtl C, PrF32Args 
     const uint32_t cnt = *(volatile const uint32_t *)&st->dd_count[iter & 1u];
     const uint32_t hubs = *(volatile const uint32_t *)&st->dd_hubs[iter & 1u];
     const float *c = (const float *)A.c32[0];
-    Red red{0.0, 0u};
+    Red red{};
     // hub rows (in-degree > AG_DD_SEQ_MAX), listed from the far end: one workgroup each, four
     // independent lane accumulators (a fixed order, so the sum is deterministic)
     for (uint32_t h = blockIdx.x; h < hubs; h += gridDim.x) {
@@ -969,7 +969,7 @@ __global__ __launch_bounds__(AG_FIN_WG) void pr_dd_finish(PrCtl C)
     if (*(volatile const uint32_t *)&st->done)
         return;
     const uint32_t iter = *(volatile const uint32_t *)&st->iter;
-    pr_close(C, iter, Red{0.0, 0u});
+    pr_close(C, iter, Red{});
     if (threadIdx.x == 0) {
         st->dd_count[iter & 1u] = 0u;
         st->dd_hubs[iter & 1u] = 0u;
@@ -1096,25 +1096,25 @@ static void session_free(ag_pr_session *s)
 {
     if (!s)
         return;
-    hipSetDevice(s->g->dev->ordinal);
-    hipStreamSynchronize(s->g->dev->stream);
+    (void)hipSetDevice(s->g->dev->ordinal);
+    (void)hipStreamSynchronize(s->g->dev->stream);
     void *ptrs[] = {s->st, s->part_err, s->part_act, s->part_cmax, s->hist_err, s->hist_act,
                     s->rank, s->c[0], s->c[1], s->c16[0], s->c16[1], s->next, s->dd_queue[0],
                     s->dd_queue[1], s->dd_flag, s->fin_part, s->fin_sum, s->user};
     for (void *p : ptrs)
         if (p)
-            hipFree(p);
+            (void)hipFree(p);
     if (s->st_host)
-        hipHostFree(s->st_host);
+        (void)hipHostFree(s->st_host);
     if (s->ev) {
         for (uint32_t i = 0; i < s->ev_count; i++)
-            hipEventDestroy(s->ev[i]);
+            (void)hipEventDestroy(s->ev[i]);
         free(s->ev);
     }
     if (s->step_ev[0])
-        hipEventDestroy(s->step_ev[0]);
+        (void)hipEventDestroy(s->step_ev[0]);
     if (s->step_ev[1])
-        hipEventDestroy(s->step_ev[1]);
+        (void)hipEventDestroy(s->step_ev[1]);
     free(s);
 }
 

@@ -483,7 +483,8 @@ extern "C" int ag_ppr_pull(ag_dgraph *g, uint32_t k, const uint32_t *seeds,
     }
     if (st == AG_OK) {
         float ms = 0.0f;
-        hipEventElapsedTime(&ms, e0, e1);
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            ms = 0.0f;   // timing is informational; the ranks are already valid
         res->loop_ms = ms;
         res->iterations = sth->done ? sth->stop_iter : sth->iter;
         res->executed = sth->done ? sth->stop_iter + 1 : sth->iter;
@@ -502,13 +503,13 @@ extern "C" int ag_ppr_pull(ag_dgraph *g, uint32_t k, const uint32_t *seeds,
         st = (e == hipSuccess) ? ag_wait_stream(g->dev) : ag_hip_fail(e, "PPR output", __FILE__, __LINE__);
     }
     if (e0)
-        hipEventDestroy(e0);
+        (void)hipEventDestroy(e0);
     if (e1)
-        hipEventDestroy(e1);
-    hipStreamSynchronize(s);
+        (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(s);
     for (int i = 0; i < nb; i++)
-        hipFree(buf[i]);
+        (void)hipFree(buf[i]);
     if (sth)
-        hipHostFree(sth);
+        (void)hipHostFree(sth);
     return st;
 }

@@ -491,7 +491,7 @@ __global__ __launch_bounds__(AG_WG) void pull_kernel(PullGeom G, P pol)
 {
     __shared__ typename P::Acc s_vals[AG_TILE_NNZ_MAX];
     __shared__ typename P::Acc s_wave[AG_WG / AG_WAVE];
-    Red red{0.0, 0u};
+    Red red{};
     pull_items(G, pol, red, s_vals, s_wave);
 }
 
@@ -504,7 +504,7 @@ __global__ __launch_bounds__(AG_WG) void pull_blocked_kernel(PullGeom U, BlockGe
     __shared__ typename P::Acc s_vals[AG_TILE_NNZ_MAX];
     __shared__ typename P::Acc s_wave[AG_WG / AG_WAVE];
     __shared__ BlkLds L;
-    Red red{0.0, 0u};
+    Red red{};
     pull_blocked_items(U, B, pol, upol, red, s_vals, s_wave, L);
 }
 
@@ -514,14 +514,14 @@ __global__ __launch_bounds__(AG_WG) void pull_combine_kernel(BlockGeom B, P pol)
     __shared__ typename P::Acc s_vals[AG_TILE_NNZ_MAX];
     __shared__ typename P::Acc s_wave[AG_WG / AG_WAVE];
     __shared__ BlkLds L;
-    Red red{0.0, 0u};
+    Red red{};
     pull_combine_items(B, pol, red, s_vals, s_wave, L);
 }
 
 template <class P>
 __global__ __launch_bounds__(AG_FIN_WG) void pull_finish_kernel(PullGeom G, P pol)
 {
-    Red red{0.0, 0u};
+    Red red{};
     pull_hub_finish(G, pol, red);
 }
 

@@ -119,9 +119,9 @@ extern "C" int ag_device_close(ag_device *d)
     if (d->live_graphs)
         return ag_set_error(AG_ERR_INVALID, "%u graph(s) still uploaded on this device; free "
                             "them before closing it", d->live_graphs);
-    hipSetDevice(d->ordinal);
-    hipStreamSynchronize(d->stream);
-    hipStreamDestroy(d->stream);
+    (void)hipSetDevice(d->ordinal);
+    (void)hipStreamSynchronize(d->stream);
+    (void)hipStreamDestroy(d->stream);
     free(d);
     return AG_OK;
 }
@@ -154,7 +154,7 @@ int ag_wait_stream(ag_device *d)
     AG_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     hipError_t e = hipEventRecord(ev, d->stream);
     if (e != hipSuccess) {
-        hipEventDestroy(ev);
+        (void)hipEventDestroy(ev);
         return ag_hip_fail(e, "hipEventRecord", __FILE__, __LINE__);
     }
     uint64_t deadline = now_ms() + d->run_timeout_ms;
@@ -163,17 +163,17 @@ int ag_wait_stream(ag_device *d)
         if (e == hipSuccess)
             break;
         if (e != hipErrorNotReady) {
-            hipEventDestroy(ev);
+            (void)hipEventDestroy(ev);
             return ag_hip_fail(e, "hipEventQuery", __FILE__, __LINE__);
         }
         if (now_ms() >= deadline) {
-            hipEventDestroy(ev);
+            (void)hipEventDestroy(ev);
             return ag_set_error(AG_ERR_TIMEOUT, "device work did not finish within %llu ms",
                                 (unsigned long long)d->run_timeout_ms);
         }
         usleep(d->poll_us);
     }
-    hipEventDestroy(ev);
+    (void)hipEventDestroy(ev);
     return AG_OK;
 }
 
@@ -202,8 +202,8 @@ extern "C" int ag_graph_free(ag_dgraph *g)
 {
     if (!g)
         return AG_OK;
-    hipSetDevice(g->dev->ordinal);
-    hipStreamSynchronize(g->dev->stream);
+    (void)hipSetDevice(g->dev->ordinal);
+    (void)hipStreamSynchronize(g->dev->stream);
     void *ptrs[] = {g->d2o, g->off, g->src, g->w, g->wq, g->outdeg, g->tile_row, g->seg_row,
                     g->seg_lo, g->seg_hi, g->hub_seg0, g->hub_part, g->vec_a, g->vec_b,
                     g->blk.voff, g->blk.vsrc, g->blk.vw, g->blk.item, g->blk.coff, g->blk.cidx,
@@ -212,7 +212,7 @@ extern "C" int ag_graph_free(ag_dgraph *g)
                     g->fwd_w};
     for (void *p : ptrs)
         if (p)
-            hipFree(p);
+            (void)hipFree(p);
     free(g->h_d2o);
     if (g->dev->live_graphs)
         g->dev->live_graphs--;
@@ -536,12 +536,12 @@ extern "C" int ag_device_copy_bandwidth(ag_device *dev, uint64_t bytes, uint32_t
             *gbps = (double)(2 * n16 * 16) * reps / ((double)ms * 1e-3) / 1e9;
     }
     if (e0)
-        hipEventDestroy(e0);
+        (void)hipEventDestroy(e0);
     if (e1)
-        hipEventDestroy(e1);
-    hipFree(a);
+        (void)hipEventDestroy(e1);
+    (void)hipFree(a);
     if (b)
-        hipFree(b);
+        (void)hipFree(b);
     return st;
 }
 
@@ -611,9 +611,9 @@ extern "C" int ag_fixed_encode(ag_device *dev, uint32_t bits, const float *in, u
     if (e == hipSuccess)
         e = hipMemcpyAsync(out, dout, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToHost, dev->stream);
     int rc = (e == hipSuccess) ? ag_wait_stream(dev) : ag_hip_fail(e, "fixed encode", __FILE__, __LINE__);
-    hipFree(din);
+    (void)hipFree(din);
     if (dout)
-        hipFree(dout);
+        (void)hipFree(dout);
     return rc;
 }
 
@@ -638,8 +638,8 @@ extern "C" int ag_fixed_decode(ag_device *dev, uint32_t bits, const uint64_t *in
     if (e == hipSuccess)
         e = hipMemcpyAsync(out, dout, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, dev->stream);
     int rc = (e == hipSuccess) ? ag_wait_stream(dev) : ag_hip_fail(e, "fixed decode", __FILE__, __LINE__);
-    hipFree(din);
+    (void)hipFree(din);
     if (dout)
-        hipFree(dout);
+        (void)hipFree(dout);
     return rc;
 }

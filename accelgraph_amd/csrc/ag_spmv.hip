@@ -154,9 +154,9 @@ static int run_pull(ag_dgraph *g, const P &pol, uint32_t reps, double *avg_ms)
         *avg_ms = reps ? (double)ms / reps : 0.0;
     }
     if (e0)
-        hipEventDestroy(e0);
+        (void)hipEventDestroy(e0);
     if (e1)
-        hipEventDestroy(e1);
+        (void)hipEventDestroy(e1);
     return st;
 }
 
@@ -195,9 +195,9 @@ static int run_pull_blocked(ag_dgraph *g, const P &pv, const P &pu, uint32_t rep
         *avg_ms = reps ? (double)ms / reps : 0.0;
     }
     if (e0)
-        hipEventDestroy(e0);
+        (void)hipEventDestroy(e0);
     if (e1)
-        hipEventDestroy(e1);
+        (void)hipEventDestroy(e1);
     return st;
 }
 
@@ -343,9 +343,9 @@ static int run_push(ag_dgraph *g, uint32_t reps, double *avg_ms, void *yd, uint3
         *avg_ms = reps ? (double)ms / reps : 0.0;
     }
     if (e0)
-        hipEventDestroy(e0);
+        (void)hipEventDestroy(e0);
     if (e1)
-        hipEventDestroy(e1);
+        (void)hipEventDestroy(e1);
     return st;
 }
 
