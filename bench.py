@@ -380,6 +380,8 @@ def main() -> int:
     ap.add_argument("--no-spmv", action="store_true", help="skip the RMAT-22 SpMV")
     ap.add_argument("--spmv-scale", type=int, default=22)
     ap.add_argument("--no-er", action="store_true", help="skip the ER PageRank iteration")
+    ap.add_argument("--no-rows", action="store_true",
+                    help="skip the fixed-32, quantised, push and data-driven rows")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -451,17 +453,21 @@ def main() -> int:
     log(f"{K} steps: {ms_per_step:.3f} ms/step wall, kernel avg {kern_ms:.3f} ms over "
         f"{launches} launches, {gteps:.2f} GTEPS, {achieved:.0f} GB/s algorithmic")
 
-    # Q16.16 fixed-point iteration (the reference's pushpull 9 path: u32 contributions, exact
-    # u64 sums), same graph, same algorithmic bytes
-    fixed32 = int_kind_iteration(ag, dev, dg, m, ag.KIND_FIXED32, algorithmic_bytes(n, m, "f32"))
-    log(f"fixed-32 iteration: {fixed32}")
-    # quantised iterations (pushpull 4/12/13): B-bit codes gathered, plus the code pass
-    quant = {f"quant{b}": int_kind_iteration(ag, dev, dg, m, kind,
-                                             algorithmic_bytes(n, m, f"q{b}"))
-             for b, kind in ((32, ag.KIND_QUANT32), (16, ag.KIND_QUANT16), (8, ag.KIND_QUANT8))}
-    log(f"quantised iterations: {quant}")
-    directions = directions_section(ag, dev, dg, m, n)
-    log(f"push / data-driven: {directions}")
+    fixed32 = quant = directions = None
+    if not args.no_rows:
+        # Q16.16 fixed-point iteration (the reference's pushpull 9 path: u32 contributions,
+        # exact u64 sums), same graph, same algorithmic bytes
+        fixed32 = int_kind_iteration(ag, dev, dg, m, ag.KIND_FIXED32,
+                                     algorithmic_bytes(n, m, "f32"))
+        log(f"fixed-32 iteration: {fixed32}")
+        # quantised iterations (pushpull 4/12/13): B-bit codes gathered, plus the code pass
+        quant = {f"quant{b}": int_kind_iteration(ag, dev, dg, m, kind,
+                                                 algorithmic_bytes(n, m, f"q{b}"))
+                 for b, kind in ((32, ag.KIND_QUANT32), (16, ag.KIND_QUANT16),
+                                 (8, ag.KIND_QUANT8))}
+        log(f"quantised iterations: {quant}")
+        directions = directions_section(ag, dev, dg, m, n)
+        log(f"push / data-driven: {directions}")
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
@@ -493,10 +499,11 @@ def main() -> int:
                      "measured_copy_gb_per_s": round(copy_gbs, 1)},
     }
 
-    out["fixed32_iteration"] = fixed32
-    out["quantized_iteration"] = quant
-    out["directions"] = directions
-    if not args.no_cpu:
+    if fixed32 is not None:
+        out["fixed32_iteration"] = fixed32
+        out["quantized_iteration"] = quant
+        out["directions"] = directions
+    if fixed32 is not None and not args.no_cpu:
         t0 = time.perf_counter()
         cpu_rows(g, quant, directions)
         log(f"CPU legs of the quantised / push / data-driven rows ({time.perf_counter() - t0:.1f} s)")
