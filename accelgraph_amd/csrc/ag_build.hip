@@ -582,6 +582,62 @@ __global__ void last_offset_kernel(uint32_t n, uint32_t m, uint32_t *off)
         off[n] = m;
 }
 
+// Push bin of every out-edge: the last bin whose first row is <= its destination.
+__global__ __launch_bounds__(BUILD_WG) void pb_key_kernel(uint32_t m, const uint32_t *fdst,
+                                                          const uint32_t *brow, uint32_t nbins,
+                                                          uint32_t *key)
+{
+    for (uint32_t j = blockIdx.x * BUILD_WG + threadIdx.x; j < m; j += gridDim.x * BUILD_WG) {
+        const uint32_t d = fdst[j];
+        uint32_t lo = 0, hi = nbins;   // brow[lo] <= d < brow[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (brow[mid] <= d)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        key[j] = lo;
+    }
+}
+
+// Slot k holds out-edge order[k] (bins ascending, forward order inside a bin).
+__global__ __launch_bounds__(BUILD_WG) void pb_slot_kernel(uint32_t m, const uint32_t *order,
+                                                           const uint32_t *skey,
+                                                           const uint32_t *fdst,
+                                                           const uint32_t *brow, uint32_t *pos,
+                                                           uint16_t *dst)
+{
+    for (uint32_t k = blockIdx.x * BUILD_WG + threadIdx.x; k < m; k += gridDim.x * BUILD_WG) {
+        const uint32_t e = order[k];
+        pos[e] = k;
+        dst[k] = (uint16_t)(fdst[e] - brow[skey[k]]);
+    }
+}
+
+// Greedy push bins over rows [0, n): a bin takes rows while it has fewer than `rows` rows and
+// its in-edges stay within `slots`; a row with more in-edges than `slots` is a bin alone.
+// Writes the first row of every bin to brow (when not null); returns the bin count.
+uint32_t push_bins_host(uint32_t n, const uint32_t *off, uint32_t rows, uint32_t slots,
+                        uint32_t *brow)
+{
+    uint32_t nb = 0, r = 0;
+    while (r < n) {
+        const uint32_t r0 = r;
+        uint64_t e = off[r + 1] - off[r];
+        r++;
+        if (e <= slots)
+            while (r < n && r - r0 < rows && e + (off[r + 1] - off[r]) <= slots) {
+                e += off[r + 1] - off[r];
+                r++;
+            }
+        if (brow)
+            brow[nb] = r0;
+        nb++;
+    }
+    return nb;
+}
+
 } // namespace
 
 // Forward CSR of an uploaded graph in device row ids: the inverse CSR (off/src) transposed
@@ -666,6 +722,104 @@ int ag_graph_forward(ag_dgraph *g)
     g->fwd_dst = fdst;
     g->fwd_w = fw;
     g->device_bytes += ((size_t)n + 1 + (fw ? 3 : 2) * (size_t)m) * sizeof(uint32_t);
+    return AG_OK;
+}
+
+// Bins and slots of the propagation-blocked push (ag_push.h), once per graph: bins are ranges
+// of device rows (greedy on the host over the in-degrees); the slots of a bin are its rows'
+// in-edges in forward order (sources ascending), found by one stable radix sort of the
+// forward edges by the bin of their destination.
+int ag_graph_push_bins(ag_dgraph *g)
+{
+    if (g->pb_pos)
+        return AG_OK;
+    int st = ag_graph_forward(g);
+    if (st != AG_OK)
+        return st;
+    ag_device *dev = g->dev;
+    AG_HIP(hipSetDevice(dev->ordinal));
+    const uint32_t n = g->n, m = g->m;
+    uint32_t *hoff = (uint32_t *)malloc(((size_t)n + 1) * sizeof(uint32_t));
+    if (!hoff)
+        return ag_set_error(AG_ERR_OOM, "push bins: host offsets");
+    hipError_t e = hipMemcpy(hoff, g->off, ((size_t)n + 1) * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        free(hoff);
+        return ag_hip_fail(e, "push bins: offsets download", __FILE__, __LINE__);
+    }
+    const uint32_t nb = push_bins_host(n, hoff, AG_PB_ROWS, AG_PB_SLOTS, NULL);
+    uint32_t *hb = (uint32_t *)malloc(2 * ((size_t)nb + 1) * sizeof(uint32_t));
+    if (!hb) {
+        free(hoff);
+        return ag_set_error(AG_ERR_OOM, "push bins: host bins");
+    }
+    uint32_t *hrow = hb, *hboff = hb + nb + 1;
+    push_bins_host(n, hoff, AG_PB_ROWS, AG_PB_SLOTS, hrow);
+    hrow[nb] = n;
+    for (uint32_t b = 0; b <= nb; b++)
+        hboff[b] = hoff[hrow[b]];
+    free(hoff);
+
+    uint32_t *brow = NULL, *pos = NULL;
+    uint16_t *dst = NULL;
+    Pool T{};
+    T.status = AG_OK;
+    e = hipMalloc((void **)&brow, 2 * ((size_t)nb + 1) * sizeof(uint32_t));
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&pos, (size_t)(m ? m : 1) * sizeof(uint32_t));
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&dst, (size_t)(m ? m : 1) * sizeof(uint16_t));
+    if (e == hipSuccess)
+        e = hipMemcpy(brow, hb, 2 * ((size_t)nb + 1) * sizeof(uint32_t), hipMemcpyHostToDevice);
+    free(hb);
+    if (e != hipSuccess)
+        st = ag_hip_fail(e, "push bins allocation", __FILE__, __LINE__);
+    uint32_t *key = T.get<uint32_t>(m);
+    uint32_t *skey = T.get<uint32_t>(m);
+    uint32_t *idx = T.get<uint32_t>(m);
+    uint32_t *order = T.get<uint32_t>(m);
+    if (st == AG_OK)
+        st = T.status;
+    if (st == AG_OK && m) {
+        const uint32_t gm = grid_of(m, BUILD_WG) < 8192u ? grid_of(m, BUILD_WG) : 8192u;
+        pb_key_kernel<<<gm, BUILD_WG, 0, dev->stream>>>(m, g->fwd_dst, brow, nb, key);
+        iota_kernel<<<gm, BUILD_WG, 0, dev->stream>>>(m, idx);
+        e = hipGetLastError();
+        size_t tmp = 0;
+        if (e == hipSuccess)
+            e = rocprim::radix_sort_pairs(nullptr, tmp, key, skey, idx, order, (size_t)m, 0u,
+                                          bits_for(nb), dev->stream);
+        void *t = NULL;
+        if (e == hipSuccess)
+            e = hipMalloc(&t, tmp ? tmp : 8);
+        if (e == hipSuccess)
+            e = rocprim::radix_sort_pairs(t, tmp, key, skey, idx, order, (size_t)m, 0u,
+                                          bits_for(nb), dev->stream);
+        if (e == hipSuccess) {
+            pb_slot_kernel<<<gm, BUILD_WG, 0, dev->stream>>>(m, order, skey, g->fwd_dst, brow,
+                                                             pos, dst);
+            e = hipGetLastError();
+        }
+        st = (e == hipSuccess) ? ag_wait_stream(dev)
+                               : ag_hip_fail(e, "push bins sort", __FILE__, __LINE__);
+        if (t)
+            (void)hipFree(t);
+    }
+    T.release();
+    if (st != AG_OK) {
+        (void)hipFree(brow);
+        (void)hipFree(pos);
+        (void)hipFree(dst);
+        return st;
+    }
+    g->pb_nbins = nb;
+    g->pb_row = brow;
+    g->pb_off = brow + nb + 1;
+    g->pb_pos = pos;
+    g->pb_dst = dst;
+    g->device_bytes += 2 * ((size_t)nb + 1) * sizeof(uint32_t) +
+                       (size_t)m * (sizeof(uint32_t) + sizeof(uint16_t));
     return AG_OK;
 }
 

@@ -9,6 +9,15 @@
 #include "../host/ag_plan.h"
 
 #define AG_MAX_EVENTS 2
+// Push bins (ag_push.h): rows per bin (the LDS row array of phase 2, and 16-bit local row
+// ids) and slots per multi-row bin (one workgroup's work); a row with more in-edges than
+// AG_PB_SLOTS is a bin by itself.
+#ifndef AG_PB_ROWS
+#define AG_PB_ROWS 4096u
+#endif
+#ifndef AG_PB_SLOTS
+#define AG_PB_SLOTS 131072u
+#endif
 
 struct ag_device {
     int ordinal;
@@ -54,6 +63,13 @@ struct ag_dgraph {
     uint32_t *fwd_src;   // [m] source of every out-edge (row id of the forward CSR)
     uint32_t *fwd_dst;   // [m] destination, ascending within a row
     float *fwd_w;        // [m] weight of every out-edge (null for an unweighted graph)
+    // push bins (ag_graph_push_bins): ranges of destination rows, and the slot of every
+    // out-edge in the bin buffer (bins in row order, forward order inside a bin)
+    uint32_t pb_nbins;
+    uint32_t *pb_row;    // [pb_nbins+1] first row of every bin (one allocation with pb_off)
+    uint32_t *pb_off;    // [pb_nbins+1] first slot of every bin
+    uint32_t *pb_pos;    // [m] slot of every out-edge
+    uint16_t *pb_dst;    // [m] row of every slot minus its bin's first row
     // scratch for permuted vectors (x, y) of SpMV / gather, sized lazily
     void *vec_a;
     void *vec_b;
@@ -130,7 +146,7 @@ struct ag_pr_session {
     float *rank;
     void *c[2];
     void *c16[2];
-    void *next;                 // push: scattered sums [n] (f32 or u64)
+    void *push_vals;            // push: the bin buffer, one value per out-edge slot [m]
     uint32_t *dd_queue[2];      // data-driven pull: worklists by iteration parity
     uint8_t *dd_flag;           // data-driven pull: next-worklist marks [n rounded up to 4]
     double *fin_part;
@@ -150,6 +166,9 @@ int ag_pr_launch_iteration(ag_pr_session *s);
 // Builds g->fwd_off / fwd_src / fwd_dst (the forward CSR in device row ids) from the uploaded
 // inverse CSR, once; later calls return at once.
 int ag_graph_forward(ag_dgraph *g);
+
+// Builds g->pb_* (push bins and slots, after the forward CSR), once; later calls return at once.
+int ag_graph_push_bins(ag_dgraph *g);
 
 int ag_set_error(int status, const char *fmt, ...);
 int ag_hip_fail(hipError_t e, const char *what, const char *file, int line);

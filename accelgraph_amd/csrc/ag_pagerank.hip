@@ -32,6 +32,7 @@
 #include "ag_internal.h"
 #include "ag_fixed_dev.h"
 #include "ag_pull_kernel.h"
+#include "ag_push.h"
 
 // Accumulator of the data-driven pull on a source-blocked plan: the FP32 sum of the in-edges
 // and whether any in-neighbour took a new rank in the previous iteration (0 or 1, reduced
@@ -694,37 +695,46 @@ __global__ __launch_bounds__(AG_FIN_WG) void pr_ddf_finish(PullGeom G, PrCtl C, 
 //
 // pageRankPushGraphCSR (pageRank.c:1111-1272) and pageRankPushFixedPointGraphCSR
 // (:1992-2117) keep the pull's host loop and form next[u] by scattering every source's
-// contribution along its out-edges. Here: an edge-parallel scatter over the forward CSR with
-// hardware atomics into `next` (FP32: the order of the additions is the atomics' order;
-// fixed point: exact u64 sums, equal to the pull's), then one pass over every row that reads
-// and clears next[r] and applies the kind's update, then the one-workgroup finish.
+// contribution along its out-edges. Here the scatter is propagation-blocked (ag_push.h):
+// phase 1 copies every out-edge's source contribution into its bin slot, phase 2 adds each
+// bin's slots per row in LDS and applies the kind's update to every row of the bin, then the
+// one-workgroup finish closes the iteration. Fixed-point sums are exact u64 sums, equal to
+// the pull's bit for bit; FP32 sums take the LDS atomics' order (a hub row's bin: a fixed
+// tree), which may differ from run to run (FP32 bar).
+
+template <int KC>
+struct PushVal {
+    using T = typename FixT<KC>::T;
+    using A = unsigned long long;
+};
+template <>
+struct PushVal<0> {
+    using T = float;
+    using A = float;
+};
 
 template <int KC>
 __global__ __launch_bounds__(AG_WG) void pr_push_scatter_kernel(PrCtl C, PrF32Args A, uint32_t m,
                                                                 const uint32_t *fsrc,
-                                                                const uint32_t *fdst, void *next)
+                                                                const uint32_t *pos, void *vals)
 {
     const PrState *st = C.st;
     if (*(volatile const uint32_t *)&st->done)
         return;
-    const uint32_t rd = (*(volatile const uint32_t *)&st->iter) & 1u;
-    for (uint32_t j = blockIdx.x * AG_WG + threadIdx.x; j < m; j += gridDim.x * AG_WG) {
-        const uint32_t s = __builtin_nontemporal_load(&fsrc[j]);
-        const uint32_t d = __builtin_nontemporal_load(&fdst[j]);
-        if constexpr (KC == 0) {
-            unsafeAtomicAdd(&((float *)next)[d], ((const float *)A.c32[rd])[s]);
-        } else {
-            using CT = typename FixT<KC>::T;
-            atomicAdd(&((unsigned long long *)next)[d],
-                      (unsigned long long)((const CT *)A.c32[rd])[s]);
-        }
-    }
+    using VT = typename PushVal<KC>::T;
+    const VT *c = (const VT *)A.c32[(*(volatile const uint32_t *)&st->iter) & 1u];
+    pb_scatter(m, pos, (VT *)vals,
+               [&](uint32_t j) { return c[__builtin_nontemporal_load(&fsrc[j])]; });
 }
 
 template <int KC>
-__global__ __launch_bounds__(AG_WG) void pr_push_update_kernel(PrCtl C, PrF32Args A, uint32_t n,
-                                                               void *next)
+__global__ __launch_bounds__(AG_WG) void pr_push_bins_kernel(PrCtl C, PrF32Args A, PushBins P,
+                                                             const void *vals)
 {
+    using VT = typename PushVal<KC>::T;
+    using AT = typename PushVal<KC>::A;
+    __shared__ AT s_acc[AG_PB_ROWS];
+    __shared__ AT s_wave[AG_WG / AG_WAVE];
     const PrState *st = C.st;
     if (*(volatile const uint32_t *)&st->done)
         return;
@@ -734,20 +744,12 @@ __global__ __launch_bounds__(AG_WG) void pr_push_update_kernel(PrCtl C, PrF32Arg
         const uint32_t rd = iter & 1u, wr = rd ^ 1u;
         const PrF32Pol<float, float> p{(const float *)A.c32[rd], (float *)A.c32[wr], A.rank,
                                        A.outdeg, A.base, A.damp, A.dn, A.eps};
-        float *nx = (float *)next;
-        for (uint32_t r = blockIdx.x * AG_WG + threadIdx.x; r < n; r += gridDim.x * AG_WG) {
-            const float sum = nx[r];
-            nx[r] = 0.0f;
-            p.finish(r, sum, red);
-        }
+        pb_bins(P, (const VT *)vals, s_acc, s_wave,
+                [&](uint32_t r, float sum) { p.finish(r, sum, red); });
     } else {
         const PrFixPol<KC> p = fix_policy<KC>(iter, A);
-        uint64_t *nx = (uint64_t *)next;
-        for (uint32_t r = blockIdx.x * AG_WG + threadIdx.x; r < n; r += gridDim.x * AG_WG) {
-            const uint64_t sum = nx[r];
-            nx[r] = 0ull;
-            p.finish(r, sum, red);
-        }
+        pb_bins(P, (const VT *)vals, s_acc, s_wave,
+                [&](uint32_t r, AT sum) { p.finish(r, (uint64_t)sum, red); });
     }
     pr_store_partial(C, red);
 }
@@ -984,12 +986,6 @@ __global__ __launch_bounds__(AG_WG) void pr_dd_init_kernel(uint32_t n, uint8_t *
         flag[v] = v < n ? 1 : 0;
 }
 
-__global__ __launch_bounds__(AG_WG) void pr_zero_kernel(uint64_t *p, uint32_t count)
-{
-    for (uint32_t i = blockIdx.x * AG_WG + threadIdx.x; i < count; i += gridDim.x * AG_WG)
-        p[i] = 0ull;
-}
-
 // ---------------------------------------------------------------- init / finalise
 
 // kind: 0 = f32 contributions, 1 = bf16, 2 = f16, 64/32/16/8 = fixed point, 1xx = quantised
@@ -1099,7 +1095,7 @@ static void session_free(ag_pr_session *s)
     (void)hipSetDevice(s->g->dev->ordinal);
     (void)hipStreamSynchronize(s->g->dev->stream);
     void *ptrs[] = {s->st, s->part_err, s->part_act, s->part_cmax, s->hist_err, s->hist_act,
-                    s->rank, s->c[0], s->c[1], s->c16[0], s->c16[1], s->next, s->dd_queue[0],
+                    s->rank, s->c[0], s->c[1], s->c16[0], s->c16[1], s->push_vals, s->dd_queue[0],
                     s->dd_queue[1], s->dd_flag, s->fin_part, s->fin_sum, s->user};
     for (void *p : ptrs)
         if (p)
@@ -1220,7 +1216,7 @@ extern "C" int ag_pr_begin(ag_dgraph *g, const ag_pr_options *opts, ag_pr_sessio
     if (!(o.epsilon >= 0.0))
         return ag_set_error(AG_ERR_INVALID, "epsilon must be >= 0");
     if (o.direction == AG_DIR_PUSH || dd_lists) {
-        const int fst = ag_graph_forward(g);
+        const int fst = dd_lists ? ag_graph_forward(g) : ag_graph_push_bins(g);
         if (fst != AG_OK)
             return fst;
     }
@@ -1261,8 +1257,8 @@ extern "C" int ag_pr_begin(ag_dgraph *g, const ag_pr_options *opts, ag_pr_sessio
         AG_TRY(salloc(s, &s->c16[0], (size_t)n * 2));
         AG_TRY(salloc(s, &s->c16[1], (size_t)n * 2));
     }
-    if (o.direction == AG_DIR_PUSH)
-        AG_TRY(salloc(s, &s->next, (size_t)n * sizeof(uint64_t)));
+    if (o.direction == AG_DIR_PUSH)   // one contribution per out-edge slot, stored as c[]
+        AG_TRY(salloc(s, &s->push_vals, (size_t)g->m * celem));
     if (dd_lists) {
         AG_TRY(salloc(s, (void **)&s->dd_queue[0], (size_t)n * sizeof(uint32_t)));
         AG_TRY(salloc(s, (void **)&s->dd_queue[1], (size_t)n * sizeof(uint32_t)));
@@ -1300,8 +1296,6 @@ extern "C" int ag_pr_begin(ag_dgraph *g, const ag_pr_options *opts, ag_pr_sessio
                                                                      s->st);
             ag_launch_quant(s);
         }
-        if (s->next)
-            pr_zero_kernel<<<ag_quant_grid(g), AG_WG, 0, stream>>>((uint64_t *)s->next, n);
         if (s->dd_queue[0]) {
             // data-driven pull: every vertex starts listed
             pr_dd_init_kernel<<<ag_quant_grid(g), AG_WG, 0, stream>>>(n, s->dd_flag,
@@ -1404,13 +1398,15 @@ int ag_pr_launch_iteration(ag_pr_session *s)
     const uint32_t lds = (s->kind_code >= 8) ? AG_TILE_NNZ_MAX * 8u : AG_TILE_NNZ_MAX * 4u;
     dim3 block(AG_WG), fin(AG_FIN_WG);
     if (s->o.direction == AG_DIR_PUSH) {
-        const uint32_t gs = g->dev->num_cus * 16u, gu = ag_quant_grid(g);
+        const uint32_t gs = (g->dev->num_cus * 16u + AG_NXCD - 1u) / AG_NXCD * AG_NXCD;
+        const uint32_t gu = ag_quant_grid(g);
+        const PushBins P{g->pb_nbins, g->pb_row, g->pb_off, g->pb_pos, g->pb_dst};
         C.nparts = gu;
         switch (s->kind_code) {
 #define AG_PUSH(KC)                                                                              \
-    pr_push_scatter_kernel<KC><<<gs, block, 0, stream>>>(C, A, g->m, g->fwd_src, g->fwd_dst,   \
-                                                         s->next);                              \
-    pr_push_update_kernel<KC><<<gu, block, 0, stream>>>(C, A, g->n, s->next);                  \
+    pr_push_scatter_kernel<KC><<<gs, block, 0, stream>>>(C, A, g->m, g->fwd_src, g->pb_pos,    \
+                                                         s->push_vals);                         \
+    pr_push_bins_kernel<KC><<<gu, block, 0, stream>>>(C, A, P, s->push_vals);                  \
     break
         case 0: AG_PUSH(0);
         case 64: AG_PUSH(64);

@@ -209,7 +209,7 @@ extern "C" int ag_graph_free(ag_dgraph *g)
                     g->blk.voff, g->blk.vsrc, g->blk.vw, g->blk.item, g->blk.coff, g->blk.cidx,
                     g->blk.lpos, g->blk.ct_row, g->blk.ct_start, g->blk.ct_pre, g->blk.c_item,
                     g->vpart, g->vwq, g->fwd_off, g->fwd_src, g->fwd_dst,
-                    g->fwd_w};
+                    g->fwd_w, g->pb_row, g->pb_pos, g->pb_dst};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);

@@ -19,6 +19,7 @@
 #include "ag_internal.h"
 #include "ag_fixed_dev.h"
 #include "ag_pull_kernel.h"
+#include "ag_push.h"
 
 namespace {
 
@@ -67,36 +68,52 @@ __global__ void permute_out_kernel(uint32_t n, const T *dev, const uint32_t *d2o
         orig[d2o[r]] = dev[r];
 }
 
-// SpMV push (SPMV.c:752-848, 1017-1136): every source scatters w * x[src] to its out-edges'
-// destinations with atomics. The reference never clears the output between iterations, so
-// repetitions accumulate (y = reps * A x, added edge by edge). FP32: binary32 product, the
-// atomics' order of additions. Fixed point: Q32.32 x and weights, MULFixed64V1 products,
-// exact u64 sums.
+// SpMV push (SPMV.c:752-848, 1017-1136): every source adds w * x[src] into its out-edges'
+// destinations. The reference never clears the output between iterations, so repetitions
+// accumulate (y = reps * A x). Propagation-blocked (ag_push.h): phase 1 writes every
+// out-edge's product into its bin slot, phase 2 adds each bin's slots per row in LDS and adds
+// the row's sum to y. FP32: binary32 products, summed in the LDS atomics' order (a hub row's
+// bin: a fixed tree). Fixed point: Q32.32 x and weights, MULFixed64V1 products, exact u64 sums.
 template <bool WEIGHTED>
-__global__ __launch_bounds__(AG_WG) void spmv_push_kernel(uint32_t m, const uint32_t *fsrc,
-                                                          const uint32_t *fdst, const float *fw,
-                                                          const float *x, float *y)
+__global__ __launch_bounds__(AG_WG) void spmv_push_scatter_kernel(uint32_t m, const uint32_t *fsrc,
+                                                                  const uint32_t *pos,
+                                                                  const float *fw, const float *x,
+                                                                  float *vals)
 {
-    for (uint32_t j = blockIdx.x * AG_WG + threadIdx.x; j < m; j += gridDim.x * AG_WG) {
+    pb_scatter(m, pos, vals, [&](uint32_t j) {
         const float w = WEIGHTED ? __builtin_nontemporal_load(&fw[j]) : 0.0001f;
-        unsafeAtomicAdd(&y[__builtin_nontemporal_load(&fdst[j])],
-                        __fmul_rn(w, x[__builtin_nontemporal_load(&fsrc[j])]));
-    }
+        return __fmul_rn(w, x[__builtin_nontemporal_load(&fsrc[j])]);
+    });
 }
 
 template <bool WEIGHTED>
-__global__ __launch_bounds__(AG_WG) void spmv_push_fixed_kernel(uint32_t m, const uint32_t *fsrc,
-                                                                const uint32_t *fdst,
-                                                                const float *fw,
-                                                                const uint64_t *xq, uint64_t wc,
-                                                                uint64_t *y)
+__global__ __launch_bounds__(AG_WG) void spmv_push_fixed_scatter_kernel(
+    uint32_t m, const uint32_t *fsrc, const uint32_t *pos, const float *fw, const uint64_t *xq,
+    uint64_t wc, uint64_t *vals)
 {
-    for (uint32_t j = blockIdx.x * AG_WG + threadIdx.x; j < m; j += gridDim.x * AG_WG) {
+    pb_scatter(m, pos, vals, [&](uint32_t j) {
         const uint64_t w =
             WEIGHTED ? ag_dev_double_to_fixed64((double)__builtin_nontemporal_load(&fw[j])) : wc;
-        atomicAdd((unsigned long long *)&y[__builtin_nontemporal_load(&fdst[j])],
-                  (unsigned long long)ag_dev_mul_fixed64(w, xq[__builtin_nontemporal_load(&fsrc[j])]));
-    }
+        return ag_dev_mul_fixed64(w, xq[__builtin_nontemporal_load(&fsrc[j])]);
+    });
+}
+
+__global__ __launch_bounds__(AG_WG) void spmv_push_bins_kernel(PushBins P, const float *vals,
+                                                               float *y)
+{
+    __shared__ float s_acc[AG_PB_ROWS];
+    __shared__ float s_wave[AG_WG / AG_WAVE];
+    pb_bins(P, vals, s_acc, s_wave, [&](uint32_t r, float sum) { y[r] = y[r] + sum; });
+}
+
+__global__ __launch_bounds__(AG_WG) void spmv_push_fixed_bins_kernel(PushBins P,
+                                                                     const uint64_t *vals,
+                                                                     uint64_t *y)
+{
+    __shared__ unsigned long long s_acc[AG_PB_ROWS];
+    __shared__ unsigned long long s_wave[AG_WG / AG_WAVE];
+    pb_bins(P, vals, s_acc, s_wave,
+            [&](uint32_t r, unsigned long long sum) { y[r] += (uint64_t)sum; });
 }
 
 __global__ __launch_bounds__(AG_WG) void zero_u32_kernel(uint32_t *p, uint32_t count)
@@ -312,26 +329,34 @@ extern "C" int ag_spmv_pull_fixed(ag_dgraph *g, const uint32_t *xq, uint64_t *yq
     return ag_wait_stream(g->dev);
 }
 
-// reps scatter passes of the push into yd (ywords u32 words, zeroed first), timed with HIP
-// events.
+// reps propagation-blocked push passes into yd (ywords u32 words, zeroed first) through a
+// bin buffer of m values of vbytes each, timed with HIP events. launch(p1, p2, vals) enqueues
+// one pass with phase-1 and phase-2 grids p1 and p2.
 template <class L>
 static int run_push(ag_dgraph *g, uint32_t reps, double *avg_ms, void *yd, uint32_t ywords,
-                    const L &launch)
+                    size_t vbytes, const L &launch)
 {
     hipStream_t stream = g->dev->stream;
+    void *vals = NULL;
+    AG_HIP(hipMalloc(&vals, (g->m ? g->m : 1u) * vbytes));
     const uint32_t zb = (ywords + AG_WG - 1) / AG_WG;
     zero_u32_kernel<<<zb < 4096u ? (zb ? zb : 1u) : 4096u, AG_WG, 0, stream>>>((uint32_t *)yd,
                                                                                ywords);
     hipEvent_t e0 = NULL, e1 = NULL;
+    hipError_t e = hipSuccess;
     if (avg_ms) {
-        AG_HIP(hipEventCreate(&e0));
-        AG_HIP(hipEventCreate(&e1));
-        AG_HIP(hipEventRecord(e0, stream));
+        e = hipEventCreate(&e0);
+        if (e == hipSuccess)
+            e = hipEventCreate(&e1);
+        if (e == hipSuccess)
+            e = hipEventRecord(e0, stream);
     }
-    const uint32_t grid = g->dev->num_cus * 16u;
-    for (uint32_t i = 0; i < reps && g->m; i++)
-        launch(grid);
-    hipError_t e = hipGetLastError();
+    const uint32_t p1 = (g->dev->num_cus * 16u + AG_NXCD - 1u) / AG_NXCD * AG_NXCD;
+    const uint32_t p2 = g->pb_nbins < g->dev->num_cus * 8u ? g->pb_nbins : g->dev->num_cus * 8u;
+    for (uint32_t i = 0; e == hipSuccess && i < reps && g->m; i++)
+        launch(p1, p2, vals);
+    if (e == hipSuccess)
+        e = hipGetLastError();
     if (e == hipSuccess && avg_ms)
         e = hipEventRecord(e1, stream);
     int st = (e == hipSuccess) ? ag_wait_stream(g->dev) : ag_hip_fail(e, "push launch", __FILE__, __LINE__);
@@ -346,6 +371,8 @@ static int run_push(ag_dgraph *g, uint32_t reps, double *avg_ms, void *yd, uint3
         (void)hipEventDestroy(e0);
     if (e1)
         (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(stream);
+    (void)hipFree(vals);
     return st;
 }
 
@@ -359,7 +386,7 @@ extern "C" int ag_spmv_push(ag_dgraph *g, const float *x, float *y, uint32_t rep
     if (n == 0)
         return AG_OK;
     AG_HIP(hipSetDevice(g->dev->ordinal));
-    int st = ag_graph_forward(g);
+    int st = ag_graph_push_bins(g);
     if (st == AG_OK)
         st = ag_ensure_vec(g, (size_t)n * sizeof(uint64_t));
     if (st != AG_OK)
@@ -374,12 +401,15 @@ extern "C" int ag_spmv_push(ag_dgraph *g, const float *x, float *y, uint32_t rep
     AG_HIP(hipGetLastError());
     const uint32_t m = g->m;
     const float *fw = g->fwd_w;
-    st = run_push(g, reps, avg_ms, yd, n, [&](uint32_t grid) {
+    const PushBins P{g->pb_nbins, g->pb_row, g->pb_off, g->pb_pos, g->pb_dst};
+    st = run_push(g, reps, avg_ms, yd, n, sizeof(float), [&](uint32_t p1, uint32_t p2, void *v) {
         if (fw)
-            spmv_push_kernel<true><<<grid, AG_WG, 0, stream>>>(m, g->fwd_src, g->fwd_dst, fw, xd, yd);
+            spmv_push_scatter_kernel<true><<<p1, AG_WG, 0, stream>>>(m, g->fwd_src, g->pb_pos, fw,
+                                                                     xd, (float *)v);
         else
-            spmv_push_kernel<false><<<grid, AG_WG, 0, stream>>>(m, g->fwd_src, g->fwd_dst, NULL, xd,
-                                                                yd);
+            spmv_push_scatter_kernel<false><<<p1, AG_WG, 0, stream>>>(m, g->fwd_src, g->pb_pos,
+                                                                      NULL, xd, (float *)v);
+        spmv_push_bins_kernel<<<p2, AG_WG, 0, stream>>>(P, (const float *)v, yd);
     });
     if (st != AG_OK)
         return st;
@@ -400,7 +430,7 @@ extern "C" int ag_spmv_push_fixed(ag_dgraph *g, const uint64_t *xq, uint64_t *yq
     if (n == 0)
         return AG_OK;
     AG_HIP(hipSetDevice(g->dev->ordinal));
-    int st = ag_graph_forward(g);
+    int st = ag_graph_push_bins(g);
     if (st == AG_OK)
         st = ag_ensure_vec(g, (size_t)n * sizeof(uint64_t) * 2);
     if (st != AG_OK)
@@ -416,13 +446,16 @@ extern "C" int ag_spmv_push_fixed(ag_dgraph *g, const uint64_t *xq, uint64_t *yq
     const uint32_t m = g->m;
     const float *fw = g->fwd_w;
     const uint64_t wc = DoubleToFixed64((double)0.0001f);   // SPMV.c:1080
-    st = run_push(g, reps, avg_ms, yd, 2u * n, [&](uint32_t grid) {
+    const PushBins P{g->pb_nbins, g->pb_row, g->pb_off, g->pb_pos, g->pb_dst};
+    st = run_push(g, reps, avg_ms, yd, 2u * n, sizeof(uint64_t),
+                  [&](uint32_t p1, uint32_t p2, void *v) {
         if (fw)
-            spmv_push_fixed_kernel<true><<<grid, AG_WG, 0, stream>>>(m, g->fwd_src, g->fwd_dst, fw,
-                                                                     xd, 0ull, yd);
+            spmv_push_fixed_scatter_kernel<true><<<p1, AG_WG, 0, stream>>>(
+                m, g->fwd_src, g->pb_pos, fw, xd, 0ull, (uint64_t *)v);
         else
-            spmv_push_fixed_kernel<false><<<grid, AG_WG, 0, stream>>>(m, g->fwd_src, g->fwd_dst,
-                                                                      NULL, xd, wc, yd);
+            spmv_push_fixed_scatter_kernel<false><<<p1, AG_WG, 0, stream>>>(
+                m, g->fwd_src, g->pb_pos, NULL, xd, wc, (uint64_t *)v);
+        spmv_push_fixed_bins_kernel<<<p2, AG_WG, 0, stream>>>(P, (const uint64_t *)v, yd);
     });
     if (st != AG_OK)
         return st;

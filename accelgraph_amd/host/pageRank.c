@@ -225,10 +225,12 @@ struct PageRankStats *pageRankDataDrivenPullGraphCSR(struct Arguments *arguments
     return pagerank_run(arguments, graph, AG_KIND_F32, AG_DIR_DATA_DRIVEN);
 }
 
-/* pageRank.c:874-939. Not provided: the quantised push (5: a zero-point quantiser with float
- * atomic sums), the data-driven push and pull-push (7, 8). Those three update shared
- * residuals without ordering in the reference, so their results depend on the thread
- * schedule and have no fixed answer to reproduce. */
+/* pageRank.c:874-939. Not provided: the quantised push (5), whose zero-point quantiser API
+ * (getMinMax, GetScale, GetZeroPoint, quantize; pageRank.c:2725-2733) is absent with
+ * OpenGraph, and the data-driven push and pull-push (7, 8), which read and add to shared
+ * residuals without ordering and decide activity from the value read (pageRank.c:3068-3078,
+ * 3242-3256), so their results depend on the thread schedule and have no fixed answer to
+ * reproduce. */
 struct PageRankStats *pageRankGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
 {
     switch (arguments->pushpull) {

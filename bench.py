@@ -95,14 +95,14 @@ def cpu_rows(g, quant: dict, directions: dict) -> None:
 
 
 def directions_section(ag, dev, dg, m: int, n: int) -> dict:
-    """Push iterations (FP32 atomics, fixed-64 u64 atomics) and the data-driven pull against
-    the plain pull under the reference's stop rule (epsilon 1e-4, scaled units)."""
+    """Push iterations (propagation-blocked, FP32 and fixed-64) and the data-driven pull
+    against the plain pull under the reference's stop rule (epsilon 1e-4, scaled units)."""
     out = {}
     t0 = time.perf_counter()
     s = dg.session(direction=ag.DIR_PUSH, max_iterations=1, stop_rule=ag.STOP_NONE)
     dev.sync()
     s.end()
-    out["forward_csr_build_s"] = round(time.perf_counter() - t0, 3)
+    out["forward_csr_and_push_bins_build_s"] = round(time.perf_counter() - t0, 3)
     for name, kind in (("push_f32", ag.KIND_F32), ("push_fixed64", ag.KIND_FIXED64)):
         s = dg.session(kind=kind, direction=ag.DIR_PUSH, max_iterations=13,
                        stop_rule=ag.STOP_NONE, init=ag.INIT_UNIFORM)
@@ -112,10 +112,11 @@ def directions_section(ag, dev, dg, m: int, n: int) -> dict:
         dev.sync()
         ms, _ = s.last_step_kernel_ms()
         s.end()
-        # forward src + dst streamed, one contribution read and one atomic per edge, then the
-        # update pass: next read + cleared, rank read + written, out-degree, contribution
+        # phase 1: source id + slot per out-edge, contributions read once, one value written
+        # per slot; phase 2: 16-bit row + value per slot, then the update of every row (rank
+        # read + written, out-degree, next contribution)
         cb = 4 if kind == ag.KIND_F32 else 8
-        nbytes = 8 * m + 4 * m + cb * m + (2 * cb + 16) * n
+        nbytes = (10 + 2 * cb) * m + (12 + 2 * cb) * n
         out[name] = {"ms_per_iteration": round(ms, 4), "gteps": round(m / (ms * 1e-3) / 1e9, 2),
                      "algorithmic_bytes": nbytes,
                      "gb_per_s": round(nbytes / (ms * 1e-3) / 1e9, 1)}
@@ -269,12 +270,13 @@ def spmv_section(ag, dev, scale: int, reps: int, cpu: bool) -> dict:
         out[name] = {"ms": round(ms, 4), "gteps": round(g.m / (ms * 1e-3) / 1e9, 2),
                      "algorithmic_bytes": nb, "gb_per_s": round(gbs, 1),
                      "roofline_frac": round(gbs / HBM_PEAK_GBS, 4)}
-    # push forms (SPMV.c:752-848, 1017-1136): atomics along the forward CSR; bytes: forward
-    # src + dst + weight streamed, x gathered, one 4/8-byte atomic per edge
-    dg.spmv_push(x, reps=1)                                 # warm (builds the forward CSR)
+    # push forms (SPMV.c:752-848, 1017-1136), propagation-blocked; bytes: phase 1 streams
+    # source id, slot and weight per out-edge, reads x once and writes one product per slot;
+    # phase 2 reads a 16-bit row and the product per slot and updates y (read + write)
+    dg.spmv_push(x, reps=1)                  # warm (builds the forward CSR and the push bins)
     xq64 = dev.fixed_encode(64, x)
-    for name, fn, arg, nb in (("push_f32", dg.spmv_push, x, 20 * g.m + 4 * g.n),
-                              ("push_fixed64", dg.spmv_push_fixed, xq64, 28 * g.m + 8 * g.n)):
+    for name, fn, arg, nb in (("push_f32", dg.spmv_push, x, 22 * g.m + 12 * g.n),
+                              ("push_fixed64", dg.spmv_push_fixed, xq64, 30 * g.m + 24 * g.n)):
         _, ms = fn(arg, reps=reps)
         out[name] = {"ms": round(ms, 4), "gteps": round(g.m / (ms * 1e-3) / 1e9, 2),
                      "algorithmic_bytes": nb, "gb_per_s": round(nb / (ms * 1e-3) / 1e9, 1)}

@@ -189,9 +189,11 @@ typedef struct ag_pr_options {
 
 /* Direction of the sum (pageRankGraphCSR's pushpull codes, pageRank.c:874-939):
  *   PULL         gather over in-edges (codes 0, 2, 4, 9-13)
- *   PUSH         scatter every contribution along the out-edges with atomics, then update
- *                every vertex (codes 1 and 3, pageRank.c:1111-1272, 1992-2117); AG_KIND_F32
- *                with f32 storage or a fixed-point kind. FP32 sums take the atomics' order;
+ *   PUSH         scatter every contribution along the out-edges, then update every vertex
+ *                (codes 1 and 3, pageRank.c:1111-1272, 1992-2117), propagation-blocked: each
+ *                out-edge writes its source's contribution to a slot of its destination's
+ *                bin, and each bin's slots are added per row in LDS; AG_KIND_F32 with f32
+ *                storage or a fixed-point kind. FP32 sums take the LDS atomics' order;
  *                fixed-point sums are exact and equal the pull's
  *   DATA_DRIVEN  pull over a worklist (code 6, pageRank.c:2810-2969): only listed vertices
  *                gather; a vertex whose |delta| >= epsilon takes its new rank and lists its
@@ -247,11 +249,12 @@ int ag_spmv_pull(ag_dgraph *g, const float *x, float *y, uint32_t repetitions,
 int ag_spmv_pull_fixed(ag_dgraph *g, const uint32_t *x_q, uint64_t *y_q,
                        uint32_t repetitions, double *avg_kernel_ms);
 /* Push forms (SPMVPushGraphCSR / SPMVPushFixedPointGraphCSR, SPMV.c:752-848, 1017-1136):
- * every source adds w*x[src] to its out-neighbours with atomics. As in the reference the
- * output is not cleared between repetitions, so y = the sum of `repetitions` products
- * (0 repetitions: y = 0). FP32: binary32 products, added in the atomics' order. Fixed point:
- * x_q and y_q in Q32.32 (DoubleToFixed64), weights DoubleToFixed64(w) (0.0001f unweighted),
- * products MULFixed64V1, exact u64 sums. The forward CSR is built on the device on first use. */
+ * every source adds w*x[src] to its out-neighbours (propagation-blocked, as the PageRank push).
+ * As in the reference the output is not cleared between repetitions, so y = the sum of
+ * `repetitions` products (0 repetitions: y = 0). FP32: binary32 products, each repetition's
+ * row sum added in the LDS atomics' order, then added to y. Fixed point: x_q and y_q in Q32.32
+ * (DoubleToFixed64), weights DoubleToFixed64(w) (0.0001f unweighted), products MULFixed64V1,
+ * exact u64 sums. The forward CSR and the push bins are built on the device on first use. */
 int ag_spmv_push(ag_dgraph *g, const float *x, float *y, uint32_t repetitions,
                  double *avg_kernel_ms);
 int ag_spmv_push_fixed(ag_dgraph *g, const uint64_t *x_q, uint64_t *y_q, uint32_t repetitions,

@@ -76,7 +76,7 @@ def test_pagerank_entry_points(graph, pushpull):
         assert s.num_vertices == g.n
         if kind == 0 and pushpull != 6:
             # the FP32 bar of the parity suite: rows summed in a tree order (pull) or in the
-            # atomics' order (push) may move the active-rule stop by a few iterations, ranks
+            # LDS atomics' order (push) may move the active-rule stop by a few iterations, ranks
             # agree to 1e-5
             if pushpull == 0:
                 assert abs(int(s.iterations) - int(ref.iterations)) <= 3

@@ -320,7 +320,7 @@ def test_pagerank_quant_inits_and_corners(device, rmat12, kind):
 @pytest.mark.parametrize("kind", [64, 32, 16, 8])
 @pytest.mark.parametrize("pname", ["default", "blocked-all"])
 def test_pagerank_push_fixed_bit_exact(device, rmat16, kind, pname):
-    """Push direction, fixed point: atomically scattered u64 sums are exact, so the ranks equal
+    """Push direction, fixed point: the binned u64 sums are exact, so the ranks equal
     the pull oracle's bit for bit (the plan only relabels rows: the forward CSR is built in
     device row ids for each)."""
     say(f"pagerank push fixed{kind} {pname}")
@@ -349,7 +349,7 @@ def _fp32_bar(run, ref, iteration_slack=3):
 
 @pytest.mark.parametrize("pname", ["default", "small", "blocked"])
 def test_pagerank_push_f32(device, rmat16, pname):
-    """Push direction, FP32: the atomics add in an order of their own, different on every
+    """Push direction, FP32: the LDS atomics add in an order of their own, different on every
     run, so the bar is the FP32 one. With a fixed iteration count the ranks are compared
     directly. Under the reference's stop rule the last vertices at the epsilon threshold have
     ulp-sized deltas, so how long that tail lasts is rounding noise: the run must stop inside
@@ -478,6 +478,58 @@ def test_pagerank_data_driven_user_starts(device, rmat16):
     dg.close()
 
 
+def test_push_bin_shapes(device):
+    """The push's bins (ranges of at most 4096 destination rows and 2^17 in-edges; a heavier
+    row is a bin by itself) on a graph that reaches every case: two hub rows above 2^17
+    in-edges, a run of rows without in-edges, a 240 K-row chain (bins cut at 4096 rows) and a
+    vertex count that is not a multiple of 4096. Fixed point bit-exact with the pull oracle,
+    FP32 within the FP32 bar; SpMV push likewise."""
+    say("push bins: hub rows, empty rows, long chain")
+    n = 250_003
+    hub0 = np.arange(1, 200_001, dtype=np.uint32)
+    hub1 = np.arange(60_000, 200_001, dtype=np.uint32)
+    chain = np.arange(10_000, n - 1, dtype=np.uint32)
+    src = np.concatenate([hub0, hub1, chain, [n - 1]]).astype(np.uint32)
+    dst = np.concatenate([np.zeros_like(hub0), np.ones_like(hub1), chain + 1, [5]])
+    g = ag.Graph.from_edges(src, dst.astype(np.uint32), n=n)
+    assert g.in_degree[0] > 131072 and g.in_degree[1] > 131072
+    dg = device.upload(g)
+    for kind in (64, 32):
+        run = dg.pagerank(kind=kind, direction=ag.DIR_PUSH, max_iterations=30, scaled=True)
+        ref = ref_pr(g, kind=kind, max_iterations=30)
+        assert run.iterations == ref.iterations
+        np.testing.assert_array_equal(run.ranks.view(np.uint32), ref.ranks.view(np.uint32))
+    run = dg.pagerank(direction=ag.DIR_PUSH, max_iterations=20, stop_rule=ag.STOP_NONE)
+    ref = ref_pr(g, max_iterations=20, stop_rule=2)
+    assert run.iterations == ref.iterations == 20
+    # A 200 K-term FP32 sum in sequence (the oracle) is off by about n * 2^-24 / 4 relative
+    # (3e-3 here); the push sums a hub row in a tree. So the FP32 bar holds on the ordinary
+    # rows, and every row must be as close to the same 20 iterations in FP64 as the oracle is.
+    light = g.in_degree <= 4096
+    assert np.abs(run.ranks - ref.ranks)[light].max() <= FP32_TOL
+    np.testing.assert_allclose(run.ranks[light], ref.ranks[light], rtol=5e-5)
+    esrc = g.inv_src.astype(np.int64)
+    erow = np.repeat(np.arange(n), g.in_degree.astype(np.int64))
+    od = g.out_degree.astype(np.float64)
+    r = np.full(n, 0.15)
+    for _ in range(20):
+        c = np.where(od > 0, r / np.maximum(od, 1), 0.0)
+        r = 0.15 + 0.85 * np.bincount(erow, weights=c[esrc], minlength=n)
+    r /= n
+    assert np.abs(run.ranks - r).max() <= 2 * np.abs(ref.ranks - r).max() + 1e-9
+    rng = np.random.default_rng(5)
+    x = rng.random(n).astype(np.float32)
+    fwd = (g.edges_idx, g.out_degree, g.fwd_dest)
+    y, _ = dg.spmv_push(x, 2)
+    ref = oracle.spmv_push_f32(*fwd, None, x, 2)
+    bound = (2 * g.in_degree.astype(np.float64) + 2) * 2.0 ** -24 * 2 * np.abs(ref)
+    assert np.all(np.abs(y.astype(np.float64) - ref) <= bound + 1e-30)
+    xq = oracle.fixed_encode(64, x)
+    yq, _ = dg.spmv_push_fixed(xq, 2)
+    np.testing.assert_array_equal(yq, oracle.spmv_push_fixed(*fwd, None, xq, 2))
+    dg.close()
+
+
 def test_pagerank_directions_corners(device):
     say("push / data-driven corner cases")
     for src, dst, n in [([], [], 1), ([], [], 6), ([0], [1], 2), ([0, 1, 2], [1, 2, 0], 3),
@@ -577,7 +629,7 @@ def test_spmv_f32_and_fixed(device, pname):
 @pytest.mark.parametrize("pname", ["default", "blocked"])
 def test_spmv_push(device, pname, weighted):
     """SpMV push: repetitions accumulate; FP32 within the rounding of reps*d additions taken
-    in the atomics' order, Q32.32 fixed point bit-exact with the oracle."""
+    in an order of the kernel's, Q32.32 fixed point bit-exact with the oracle."""
     say(f"spmv push {pname} weighted={weighted}")
     g = ag.Graph.rmat(14)
     rng = np.random.default_rng(21)
