@@ -97,3 +97,10 @@ def test_no_device_reports_cleanly():
     with pytest.raises(ag.AGError) as e:
         ag.Device(0)
     assert e.value.status == _abi.AG_ERR_NO_DEVICE
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    """There is no fallback implementation: a library that is not built is an error, not a
+    silent switch to another path."""
+    with pytest.raises(RuntimeError, match="not built"):
+        _abi.load(str(tmp_path / "libaccelgraph.so"))
