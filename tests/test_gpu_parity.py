@@ -586,6 +586,38 @@ def test_pagerank_blocked_auto_rmat22(device):
     plain.close()
 
 
+def test_rmat25_full_size(device):
+    """The benchmark's own graph and plan (RMAT-25, 33.5 M vertices, 528.7 M edges, built on
+    the GPU; 2^19-source blocks): the fixed-32 pull and the fixed-64 push bit-exact with the
+    oracle after 4 iterations, the FP32 pull within the FP32 bar, and the data-driven pull
+    with epsilon 0 equal to the FP32 pull bit for bit."""
+    say("rmat-25 full size: build")
+    g = ag.Graph.rmat_gpu(device, 25)
+    assert g.n == 1 << 25 and g.m == 528_693_006
+    dg = device.upload(g)
+    info = dg.info()
+    assert info.src_block_bits == 19 and info.src_blocks == 64
+    kw = dict(max_iterations=4, stop_rule=ag.STOP_NONE)
+    say("rmat-25 full size: fixed-32 pull")
+    fx = dg.pagerank(kind=32, **kw)
+    ref = ref_pr(g, kind=32, max_iterations=4, stop_rule=2, threads=16)
+    np.testing.assert_array_equal(fx.ranks.view(np.uint32), ref.ranks.view(np.uint32))
+    say("rmat-25 full size: fixed-64 push")
+    px = dg.pagerank(kind=64, direction=ag.DIR_PUSH, **kw)
+    ref = ref_pr(g, kind=64, max_iterations=4, stop_rule=2, threads=16)
+    np.testing.assert_array_equal(px.ranks.view(np.uint32), ref.ranks.view(np.uint32))
+    say("rmat-25 full size: FP32 pull and data-driven pull")
+    run = dg.pagerank(**kw)
+    ref = ref_pr(g, max_iterations=4, stop_rule=2, threads=16)
+    assert np.abs(run.ranks - ref.ranks).max() <= FP32_TOL
+    light = g.in_degree <= 1024   # rows whose sequential FP32 sum stays within ~1e-4 relative
+    np.testing.assert_allclose(run.ranks[light], ref.ranks[light], rtol=1e-4, atol=0)
+    dd = dg.pagerank(direction=ag.DIR_DATA_DRIVEN, epsilon=0.0, **kw)
+    np.testing.assert_array_equal(dd.ranks.view(np.uint32), run.ranks.view(np.uint32))
+    dg.close()
+    g.close()
+
+
 # ------------------------------------------------------------------ SpMV
 
 @pytest.mark.parametrize("pname", ["default", "small"] + BLOCKED)
