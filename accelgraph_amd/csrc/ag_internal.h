@@ -18,6 +18,7 @@
 #ifndef AG_PB_SLOTS
 #define AG_PB_SLOTS 131072u
 #endif
+static_assert(AG_PB_ROWS >= 2u && AG_PB_ROWS <= 65536u, "push bins use 16-bit local row ids");
 
 struct ag_device {
     int ordinal;
