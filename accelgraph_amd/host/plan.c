@@ -611,6 +611,11 @@ int ag_plan_build(uint32_t n, uint32_t m, const uint32_t *out_degree,
     int st = ag_plan_params_check(&p->params, err);
     if (st != AG_OK)
         return st;
+    /* keeps n + 1, word-rounded mark counts and grid sizes inside 32 bits */
+    if (n > 0xFFFF0000u) {
+        *err = "num_vertices exceeds 2^32 - 2^16";
+        return AG_ERR_INVALID;
+    }
     if (n > 0 && (!out_degree || !inv_edges_idx || !inv_degree)) {
         *err = "null degree/offset array";
         return AG_ERR_INVALID;

@@ -154,6 +154,11 @@ def test_plan_rejects_bad_input():
     assert st == 1
     st, p, err = build(g, ag.plan_params(thread_max_deg=100, group16_max_deg=50))
     assert st == 1
+    # a vertex count whose index arithmetic would leave 32 bits is refused before any array
+    # is read (the arrays here are far shorter than the count)
+    st, p, err = build(None, n=0xFFFF0001, outdeg=g.out_degree, idx=g.inv_idx, deg=g.in_degree,
+                       src=g.inv_src)
+    assert st == 1 and b"num_vertices" in err
 
 
 # ---------------------------------------------------------------- source blocking
