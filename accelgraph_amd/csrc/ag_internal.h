@@ -119,7 +119,6 @@ struct PrState {
     uint32_t iter;          // iterations executed so far
     uint32_t done;          // set by the last block of the stopping iteration
     uint32_t stop_iter;     // index of the stopping iteration
-    uint32_t tickets;       // block counter of the running launch (reset by its last block)
     int32_t refine_iter;    // first iteration that writes f32 contributions (-1: none yet)
     uint32_t last_active;
     double last_err;
