@@ -329,11 +329,19 @@ def config1_section(ag, dev, cpu: bool) -> dict:
            "ms_per_iteration": round(float(run.hist_ms.mean()), 4)}
     if cpu:
         import oracle
+        threads = len(os.sched_getaffinity(0))
+        args = (g.out_degree, g.inv_idx, g.in_degree, g.inv_src)
         # FP64 after the same number of iterations from the same start: the FP32 rounding error
-        f64 = oracle.pagerank_f64(g.out_degree, g.inv_idx, g.in_degree, g.inv_src, tol=0.0,
-                                  max_iterations=run.executed,
-                                  threads=len(os.sched_getaffinity(0)))
+        f64 = oracle.pagerank_f64(*args, tol=0.0, max_iterations=run.executed, threads=threads)
         out["max_abs_err_vs_f64_same_iterations"] = float(np.abs(run.ranks - f64).max())
+        # the same solve on the host cores: the oracle's OpenMP restatement of the reference
+        # pull, same start and stop rule
+        t0 = time.perf_counter()
+        ref = oracle.pagerank_pull(*args, max_iterations=100, stop_rule=1, l1_tolerance=1e-4,
+                                   init=1, threads=threads)
+        cpu_ms = (time.perf_counter() - t0) * 1e3
+        out["cpu"] = {"ms": round(cpu_ms, 2), "iterations": int(ref.executed), "cores": threads,
+                      "kind": "port", "gpu_speedup": round(cpu_ms / out["ms"], 1)}
     return out
 
 
