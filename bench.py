@@ -310,6 +310,14 @@ def spmv_section(ag, dev, scale: int, reps: int, cpu: bool) -> dict:
         dt = time.perf_counter() - t0
         out["push_f32"]["cpu_ms"] = round(dt * 1e3, 2)
         out["push_f32"]["gpu_speedup"] = round(dt * 1e3 / out["push_f32"]["ms"], 1)
+        # fixed-32 PageRank: 3 iterations of the restatement, per iteration
+        t0 = time.perf_counter()
+        oracle.pagerank_pull(g.out_degree, g.inv_idx, g.in_degree, g.inv_src, kind=32,
+                             max_iterations=3, stop_rule=2, init=1, threads=threads)
+        dt = (time.perf_counter() - t0) / 3
+        row = out["pagerank_fixed32_iteration"]
+        row["cpu_ms_per_iteration"] = round(dt * 1e3, 2)
+        row["gpu_speedup"] = round(dt * 1e3 / row["ms"], 1)
     return out
 
 
