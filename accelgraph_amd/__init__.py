@@ -410,7 +410,10 @@ class DeviceGraph:
         s = C.c_void_p()
         _check(lib().ag_pr_begin(self._h, C.byref(o), C.byref(s)), "ag_pr_begin")
         r = PrResult()
-        maxit, chunk = o.max_iterations, (o.chunk or 8)
+        # iterations queued per poll: ag_pagerank_pull's automatic rule (32 up to 2^24 edges,
+        # where an iteration costs about what a poll does; else 8)
+        maxit = o.max_iterations
+        chunk = o.chunk or (32 if self.info().num_edges <= (1 << 24) else 8)
         enq = 0
         wres = None
         if warm is not None:

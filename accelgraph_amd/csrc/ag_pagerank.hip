@@ -1697,7 +1697,10 @@ extern "C" int ag_pagerank_pull(ag_dgraph *g, const ag_pr_options *opts, float *
     if (st != AG_OK)
         return st;
     const uint32_t maxit = o.max_iterations;
-    const uint32_t chunk = o.chunk ? o.chunk : 8;
+    // auto: graphs up to 2^24 edges iterate in tens of microseconds, about what a poll costs, so
+    // they queue 32 iterations per poll (RMAT-16 to L1 <= 1e-4: 0.54 ms against 0.63 ms with 8);
+    // larger graphs queue 8, which bounds the no-op launches queued past the stop
+    const uint32_t chunk = o.chunk ? o.chunk : (g->m <= (1u << 24) ? 32u : 8u);
     ag_pr_result r;
     memset(&r, 0, sizeof(r));
     while (s->enqueued < maxit && g->total_blocks) {

@@ -182,7 +182,8 @@ typedef struct ag_pr_options {
     const float *init_ranks;   /* host [n] for AG_INIT_USER                               */
     uint32_t storage;          /* AG_KIND_F32 only: contribution storage AG_STORE_*       */
     double refine_l1;          /* 16-bit storage: switch to f32 once error_total <= this  */
-    uint32_t chunk;            /* iterations enqueued between convergence polls (0=auto)  */
+    uint32_t chunk;            /* iterations enqueued between convergence polls (0 = auto:
+                                  32 up to 2^24 edges, else 8)                             */
     uint32_t record_history;   /* keep per-iteration active/error/time                    */
     uint32_t direction;        /* AG_DIR_*                                                */
 } ag_pr_options;
