@@ -289,7 +289,9 @@ __device__ __forceinline__ void pull_items(const PullGeom &G, const P &pol, Red 
 // ---------------------------------------------------------------- source-blocked pull
 
 #define AG_LANE_MAX 32u    // rows of a tile with at most this many elements: a lane each
-#define AG_LONG_MAX 128u   // longer rows of one tile: at most AG_TILE_NNZ_MAX / 33 < 128
+#define AG_LONG_MAX 128u   // longer rows of one tile: at most AG_TILE_NNZ_MAX / (AG_LANE_MAX + 1)
+static_assert(AG_TILE_NNZ_MAX / (AG_LANE_MAX + 1u) < AG_LONG_MAX,
+              "the long rows of a full tile must fit the tile's LDS list");
 
 // First pass: a virtual row's sum is stored as its partial; the update waits for the combine.
 template <class P>
