@@ -24,6 +24,21 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# host cores for the CPU legs, read before any OpenMP runtime loads: with OMP_PROC_BIND set,
+# the runtime pins the main thread to one core and the affinity mask read later shrinks to it
+HOST_THREADS = len(os.sched_getaffinity(0))
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "PageRank-pull GTEPS + HBM GB/s; time-to-L1≤1e-6 plain vs warm-start/batched"
@@ -67,7 +82,7 @@ def cpu_rows(g, quant: dict, directions: dict) -> None:
     (2 iterations per quantised kind, 1 per push kind, the first 5 data-driven iterations),
     written beside each GPU figure."""
     import oracle
-    threads = len(os.sched_getaffinity(0))
+    threads = HOST_THREADS
     inv = (g.out_degree, g.inv_idx, g.in_degree, g.inv_src)
     fwd = (g.edges_idx, g.out_degree, g.fwd_dest)
     for b, kind in ((32, oracle.KIND_QUANT32), (16, oracle.KIND_QUANT16), (8, oracle.KIND_QUANT8)):
@@ -133,7 +148,7 @@ def directions_section(ag, dev, dg, m: int, n: int) -> dict:
 
 def cpu_baseline(g, budget_s: float) -> dict:
     import oracle
-    threads = len(os.sched_getaffinity(0))
+    threads = HOST_THREADS
     args = (g.out_degree, g.inv_idx, g.in_degree, g.inv_src)
     t0 = time.perf_counter()
     oracle.pagerank_pull(*args, max_iterations=1, stop_rule=2, threads=threads)
@@ -143,7 +158,7 @@ def cpu_baseline(g, budget_s: float) -> dict:
     oracle.pagerank_pull(*args, max_iterations=k, stop_rule=2, threads=threads)
     dt = (time.perf_counter() - t0) / k
     return {"value": round(g.m / dt / 1e9, 4), "unit": "GTEPS", "cores": threads,
-            "kind": "port",
+            "cpu_model": cpu_model(), "kind": "port",
             "sample": f"{k} iterations of the OpenMP restatement (schedule(dynamic,1024)) of "
                       f"the reference pull on the same graph, {dt * 1e3:.1f} ms/iteration",
             "ms_per_iteration": round(dt * 1e3, 3)}
@@ -298,7 +313,7 @@ def spmv_section(ag, dev, scale: int, reps: int, cpu: bool) -> dict:
     dg.close()
     if cpu:   # CPU leg: the oracle's OpenMP restatement, also the FP32 result check
         import oracle
-        threads = len(os.sched_getaffinity(0))
+        threads = HOST_THREADS
         t0 = time.perf_counter()
         ref = oracle.spmv_f32(g.inv_idx, g.in_degree, g.inv_src, g.inv_weight, x, threads)
         dt = time.perf_counter() - t0
@@ -337,7 +352,7 @@ def config1_section(ag, dev, cpu: bool) -> dict:
            "ms_per_iteration": round(float(run.hist_ms.mean()), 4)}
     if cpu:
         import oracle
-        threads = len(os.sched_getaffinity(0))
+        threads = HOST_THREADS
         args = (g.out_degree, g.inv_idx, g.in_degree, g.inv_src)
         # FP64 after the same number of iterations from the same start: the FP32 rounding error
         f64 = oracle.pagerank_f64(*args, tol=0.0, max_iterations=run.executed, threads=threads)
@@ -440,7 +455,7 @@ def main() -> int:
         construction = {"host_s": round(cpu_build_s, 3), "gpu_s": round(gpu_build_s, 3),
                         "speedup": round(cpu_build_s / max(gpu_build_s, 1e-9), 2),
                         "identical_arrays": bool(same),
-                        "host_cores": len(os.sched_getaffinity(0))}
+                        "host_cores": HOST_THREADS}
         log(f"graph construction: {construction}")
     log(f"device copy bandwidth {copy_gbs:.0f} GB/s (1 GiB, read + write)")
     t0 = time.perf_counter()
@@ -534,7 +549,7 @@ def main() -> int:
             import oracle
             f64 = oracle.pagerank_f64(g.out_degree, g.inv_idx, g.in_degree, g.inv_src,
                                       tol=1e-13, max_iterations=args.f64_iters,
-                                      threads=len(os.sched_getaffinity(0)))
+                                      threads=HOST_THREADS)
         log(f"FP64 oracle ({time.perf_counter() - t0:.1f} s)")
         out["time_to_l1_1e-6"] = time_to_l1(ag, dg, g, f64)
         log(f"time to L1<=1e-6: {out['time_to_l1_1e-6']} ({time.perf_counter() - t0:.1f} s)")
