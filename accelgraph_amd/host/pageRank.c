@@ -257,8 +257,12 @@ struct PageRankStats *pageRankGraphCSR(struct Arguments *arguments, struct Graph
     case 13:
         return pageRankPullQuant8BitGraphCSR(arguments, graph);
     case 5: case 7: case 8:
-        fprintf(stderr, "[accelgraph] pageRankGraphCSR: pushpull=%u is not a pull variant "
-                        "provided by this library\n", arguments->pushpull);
+        fprintf(stderr, "[accelgraph] pageRankGraphCSR: pushpull=%u (%s) is not provided by "
+                        "this library: %s\n", arguments->pushpull,
+                arguments->pushpull == 5 ? "quantised push"
+                : arguments->pushpull == 7 ? "data-driven push" : "data-driven pull-push",
+                arguments->pushpull == 5 ? "its quantiser is absent with OpenGraph"
+                                         : "its result depends on the thread schedule");
         exit(EXIT_FAILURE);
     default:
         return pageRankPullGraphCSR(arguments, graph);

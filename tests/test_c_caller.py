@@ -95,3 +95,41 @@ def test_c_caller_runs(tmp_path):
                  np.float32(0.001)).astype(np.float32)
     y_ref = oracle.spmv_f32(g.inv_idx, g.in_degree, g.inv_src, None, x)
     np.testing.assert_allclose(y, y_ref, rtol=1e-6, atol=0)
+
+
+UNSUPPORTED = r"""
+#include <stdlib.h>
+#include <string.h>
+#include "graphCSR.h"
+#include "pageRank.h"
+
+int main(int argc, char **argv)
+{
+    uint32_t src[2] = {0, 1}, dst[2] = {1, 0};
+    struct GraphCSR *g = graphCSRFromEdgeList(2, 2, src, dst, NULL, 1);
+    struct Arguments a;
+    memset(&a, 0, sizeof(a));
+    a.iterations = 10;
+    a.epsilon = 1e-4;
+    a.pushpull = (uint32_t)atoi(argv[1]);
+    pageRankGraphCSR(&a, g);   /* exits for the codes this library does not provide */
+    return 0;
+}
+"""
+
+
+@pytest.mark.parametrize("code,what", [(5, "quantised push"), (7, "data-driven push"),
+                                       (8, "data-driven pull-push")])
+def test_unprovided_codes_exit_with_a_message(tmp_path, code, what):
+    """pageRankGraphCSR codes 5, 7 and 8 print one line naming the variant and exit with
+    EXIT_FAILURE, the reference's convention for a failed accelerator path; nothing touches the
+    device before that."""
+    c = tmp_path / "unsupported.c"
+    c.write_text(UNSUPPORTED)
+    exe = tmp_path / "unsupported"
+    subprocess.run(["gcc", "-O2", f"-I{INCLUDE}", str(c), "-o", str(exe), f"-L{LIBDIR}",
+                    "-laccelgraph", f"-Wl,-rpath,{LIBDIR}"], check=True)
+    res = subprocess.run([str(exe), str(code)], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                         text=True, timeout=60, env=dict(os.environ, HIP_VISIBLE_DEVICES=""))
+    assert res.returncode == 1
+    assert f"pushpull={code} ({what}) is not provided" in res.stderr
