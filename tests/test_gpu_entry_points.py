@@ -3,7 +3,7 @@ call them: a struct Arguments and a struct GraphCSR in, a stats record out. Ever
 code of pageRankGraphCSR (pageRank.c:874-939) and SPMVGraphCSR (SPMV.c:594-621) that this
 library provides is checked against the oracle: integer kinds bit-exact, FP32 within 1e-5.
 The codes this library does not provide (5, 7, 8) print a message and exit the process, so
-they are not called here."""
+they are not called here; tests/test_c_caller.py runs them in a child process."""
 import ctypes as C
 import sys
 
