@@ -8,8 +8,9 @@
 // streaming phases:
 //   phase 1  every out-edge writes its value to its slot of a bin buffer. Bins are ranges of
 //            destination rows; inside a bin the slots follow the sources, so while the grid
-//            walks the sources each bin's slots fill front to back and L2 combines the stores
-//            into whole lines;
+//            walks the sources each bin's slots fill front to back, and L2 can merge the
+//            stores that reach a line while it is resident. On RMAT-25 most lines leave L2
+//            before they fill: a 4-byte store costs about 27 bytes of HBM writes;
 //   phase 2  one workgroup per bin adds the bin's values into an LDS row array (LDS atomics)
 //            and hands every row's sum to the caller (a bin of one hub row is a tree sum).
 // The bins and slots are built once per graph (ag_graph_push_bins).
