@@ -23,6 +23,7 @@ AG_ERR_INVALID, AG_ERR_NO_DEVICE, AG_ERR_HIP, AG_ERR_OOM, AG_ERR_TIMEOUT, AG_ERR
 
 AG_NBINS = 5
 SRC_BLOCK_AUTO, SRC_BLOCK_OFF = 0, 255
+ORDER_FAST, ORDER_REFERENCE = 0, 1
 # graphCSR.h GRAPH_FILE_*: the reference driver's -z codes
 FILE_TEXT_EDGE_LIST, FILE_BINARY_EDGE_LIST, FILE_CSR_SERIALIZED = 0, 1, 2
 BIN_NAMES = ["thread", "group16", "wave", "block", "hub"]
@@ -40,7 +41,8 @@ class PlanParams(C.Structure):
     _fields_ = [("thread_max_deg", u32), ("group16_max_deg", u32), ("wave_max_deg", u32),
                 ("block_max_deg", u32), ("tile_rows", u32), ("tile_nnz", u32),
                 ("segment_nnz", u32), ("order", u32), ("src_block_bits", u32),
-                ("wg_per_cu", u32), ("blocked_min_bin", u32)]
+                ("wg_per_cu", u32), ("blocked_min_bin", u32), ("hot_segments", u32),
+                ("fp32_order", u32)]
 
 
 class GraphInfo(C.Structure):
@@ -48,7 +50,8 @@ class GraphInfo(C.Structure):
                 ("bin_edges", u64 * AG_NBINS), ("bin_blocks", u32 * AG_NBINS),
                 ("num_tiles", u32), ("num_segments", u32), ("has_weights", u32),
                 ("device_bytes", u64), ("params", PlanParams), ("src_block_bits", u32),
-                ("src_blocks", u32), ("virtual_rows", u32), ("blocked_row0", u32)]
+                ("src_blocks", u32), ("virtual_rows", u32), ("blocked_row0", u32),
+                ("hot_segments", u32), ("hot_segment_bits", u32), ("hot_edges", u64)]
 
 
 class PrOptions(C.Structure):
@@ -66,12 +69,14 @@ class PrResult(C.Structure):
 
 class BatchOptions(C.Structure):
     _fields_ = [("max_iterations", u32), ("stop_rule", u32), ("epsilon", f64),
-                ("l1_tolerance", f64), ("init", u32), ("use_hip_graph", u32)]
+                ("l1_tolerance", f64), ("init", u32), ("use_hip_graph", u32),
+                ("part_edges", u32), ("heavy_deg", u32)]
 
 
 class PprOptions(C.Structure):
     _fields_ = [("max_iterations", u32), ("stop_rule", u32), ("l1_tolerance", f64),
-                ("storage", u32), ("refine_l1", f64), ("chunk", u32)]
+                ("storage", u32), ("refine_l1", f64), ("chunk", u32), ("use_hip_graph", u32),
+                ("wg_per_cu", u32)]
 
 
 class PprResult(C.Structure):

@@ -1,4 +1,5 @@
-"""Plan autotuner: picks the pull plan's bin thresholds and launch shape for a graph.
+"""Autotuner: picks the pull plan's bin thresholds and launch shape for a graph, the part
+size and heavy-row threshold of a many-graph batch, and the occupancy of the 64-column PPR.
 
 Coordinate descent over a small grid (source blocking, bin thresholds, occupancy, row order):
 starting from the defaults, each knob in turn is set to every value of its grid while the
@@ -32,6 +33,16 @@ GRID = {
     "block_max_deg": [8192, 16384, 65536],
     "wg_per_cu": [3, 4, 5, 6, 8],
     "order": [0, 1],
+    "hot_segments": [0, 32, 128],
+}
+
+# the batched forms: ag_batch_options fields and ag_ppr_options.wg_per_cu
+BATCH_GRID = {
+    "part_edges": [1024, 2048, 4096, 8192, 16384],
+    "heavy_deg": [16, 32, 64],
+}
+PPR_GRID = {
+    "wg_per_cu": [2, 4, 6, 8],
 }
 
 
@@ -122,3 +133,87 @@ def autotune(dev, graph, iterations: int = 5, repeats: int = 3, passes: int = 2,
     table[sig] = entry
     _save(path, table)
     return plan_params(**cur), dict(entry, cached=False)
+
+
+def _descend(grid: dict, start: dict, timer, passes: int):
+    """Coordinate descent: every knob in turn over its grid values (others fixed), keeping
+    the fastest. timer(dict) -> ms. Returns (best dict, best ms, default ms, trials)."""
+    cur = dict(start)
+    base = timer(cur)
+    best = base
+    trials = 1
+    for _ in range(passes):
+        for knob, values in grid.items():
+            for v in values:
+                if cur.get(knob) == v:
+                    continue
+                cand = dict(cur, **{knob: v})
+                t = timer(cand)
+                trials += 1
+                if t < best:
+                    best, cur = t, cand
+    return cur, best, base, trials
+
+
+def batch_signature(batch) -> str:
+    sizes = np.diff(batch.edge_off.astype(np.int64))
+    h = hashlib.sha1(np.sort(sizes).tobytes()).hexdigest()[:16]
+    return f"batch-G{batch.vert_off.size - 1}-n{int(batch.vert_off[-1])}-m{int(batch.edge_off[-1])}-{h}"
+
+
+def autotune_batch(batch, repeats: int = 3, passes: int = 1, table_path: str | None = None,
+                   force: bool = False, **solve):
+    """Part size and heavy-row threshold of a Batch, timed on the solve itself (median device
+    time of `repeats` captured solves with the caller's options, default L1 <= 1e-6 from the
+    uniform start). Returns (options dict, report)."""
+    path = table_path or default_table_path()
+    sig = batch_signature(batch)
+    table = _load(path)
+    if not force and sig in table:
+        return dict(table[sig]["options"]), dict(table[sig], cached=True)
+    kw = dict(dict(max_iterations=300, stop_rule=_abi.STOP_L1, l1_tolerance=1e-6,
+                   init=_abi.INIT_UNIFORM), **solve)
+
+    def timer(opts):
+        ms = []
+        for _ in range(repeats + 1):             # the first run captures the solve
+            ms.append(batch.pagerank(use_hip_graph=1, **dict(kw, **opts))[3])
+        return float(np.median(ms[1:]))
+
+    start = {"part_edges": 4096, "heavy_deg": 32}
+    best, best_ms, base_ms, trials = _descend(BATCH_GRID, start, timer, passes)
+    entry = {"options": best, "ms": best_ms, "default_ms": base_ms, "default": start,
+             "trials": trials}
+    table[sig] = entry
+    _save(path, table)
+    return dict(best), dict(entry, cached=False)
+
+
+def autotune_ppr(dg, seeds, iterations: int = 8, repeats: int = 3, passes: int = 1,
+                 storage: int = _abi.STORE_BF16, table_path: str | None = None,
+                 force: bool = False):
+    """Occupancy of the PPR iteration kernel for one graph and column count: each candidate
+    times `iterations` fixed iterations of a captured solve (median of `repeats`).
+    Returns (options dict, report)."""
+    path = table_path or default_table_path()
+    info = dg.info()
+    sig = f"ppr-k{len(seeds)}-s{storage}-n{info.num_vertices}-m{info.num_edges}"
+    table = _load(path)
+    if not force and sig in table:
+        return dict(table[sig]["options"]), dict(table[sig], cached=True)
+
+    def timer(opts):
+        ms = []
+        for _ in range(repeats + 1):
+            _, _, r = dg.ppr(seeds, max_iterations=iterations, stop_rule=_abi.STOP_NONE,
+                             storage=storage, use_hip_graph=1, **opts)
+            ms.append(r.loop_ms / max(r.executed, 1))
+        return float(np.median(ms[1:]))
+
+    start = {"wg_per_cu": 0}
+    best, best_ms, base_ms, trials = _descend(PPR_GRID, start, timer, passes)
+    entry = {"options": best, "ms_per_iteration": best_ms, "default_ms_per_iteration": base_ms,
+             "default": start, "trials": trials}
+    table[sig] = entry
+    _save(path, table)
+    return dict(best), dict(entry, cached=False)

@@ -1,26 +1,37 @@
-// ag_batch.hip -- many independent small graphs solved by one persistent launch.
+// ag_batch.hip -- many independent graphs solved together.
 //
-// Each workgroup takes the next graph from a list sorted by decreasing edge count (an atomic
-// ticket per graph, 512 tickets for 512 graphs) and runs the whole PageRank-pull iteration of
-// pageRank.c:942-1109 for it: light rows (<= AG_BATCH_HEAVY_DEG in-edges) are summed by one
-// lane each in the reference's stream order, heavy rows by a wavefront; the damping update,
-// the FP64 L1 / active reductions and the next contribution are fused as in the single-graph
-// kernel; the workgroup barrier separates iterations. The launch (a tiny ticket-reset kernel
-// plus the solver) is captured once into a hipGraph and replayed; options live in a device
-// buffer written before each replay, so the graph never needs re-capturing.
+// Every graph is cut into parts: runs of consecutive rows with at most part_edges in-edges (a
+// row with more is a part by itself) and at most AG_BATCH_PART_ROWS rows; the parts of the
+// largest graphs come first. One iteration of pageRank.c:942-1109 for every graph still
+// iterating is two launches:
+//   batch_part_kernel   every part of every unstopped graph: light rows (<= heavy_deg
+//                       in-edges) summed by one lane in the reference's stream order, heavy
+//                       rows by a wavefront (four strided accumulators, fixed tree); the fused
+//                       damping update and next contribution; the part's FP64 L1 and active
+//                       partial (fixed-order workgroup reduction)
+//   batch_fin_kernel    per graph: its parts' partials added in part order, the stop rule
+//                       (active count or L1), the iteration count, the graph's done flag
+// The kernel boundary is the only synchronisation between the parts of a graph. A stopped
+// graph's parts are skipped; once every graph has stopped, both launches return at once. The
+// solve -- state reset, init, max_iterations x (parts, finish), normalised output -- is
+// captured once per iteration budget and part layout into a hipGraph and replayed as one
+// launch; options live in a device buffer written before each replay (no memset nodes).
 #include <stdlib.h>
 #include <string.h>
 
 #include "ag_internal.h"
 #include "ag_pull_kernel.h"
 
-// 1024 lanes (16 wavefronts) per graph: the largest graphs of a batch set the makespan, and
-// their iterations are latency-bound chains of gathers that more wavefronts overlap (512 lanes
-// measured 1.9x slower on 512 graphs of scale 10-16)
-#define AG_BATCH_WG 1024
+#define AG_BATCH_PART_ROWS 2048u
 
 struct BatchGraph {
-    uint32_t v0, n, e0, m, ob, h0, nh, pad;
+    uint32_t v0, n, e0, m, ob, pad0, pad1, pad2;
+};
+
+struct BatchPart {
+    uint32_t g;         // graph index
+    uint32_t r0, r1;    // graph-local rows [r0, r1)
+    uint32_t h0, h1;    // the part's heavy rows: heavy[h0 .. h1) (graph-local row ids)
 };
 
 struct BatchCfg {
@@ -30,28 +41,44 @@ struct BatchCfg {
 
 struct BatchArgs {
     const BatchGraph *graphs;
-    const uint32_t *order;
-    const uint32_t *off;      // per graph: n+1 local offsets, graph g starts at ob
+    const BatchPart *parts;
+    const uint32_t *gpart;    // [2G] first and end part of each graph
+    const uint32_t *off;      // per graph: n+1 local offsets, graph g's start at ob
     const uint32_t *src;      // graph-local source ids
     const uint32_t *outdeg;
-    const uint32_t *heavy;    // graph-local heavy row ids
+    const uint32_t *heavy;    // graph-local heavy row ids, graph by graph, ascending
     float *rank, *c0, *c1, *out;
     uint32_t *iters;
     double *err;
-    uint32_t *ticket;
+    uint32_t *done;           // [G]
+    uint32_t *ndone;          // graphs stopped so far
+    double *perr;             // [2 * nparts] part L1 partials by iteration parity
+    uint32_t *pact;           // [2 * nparts]
     const BatchCfg *cfg;
+    uint32_t nparts, heavy_deg, total_n;
 };
 
 struct ag_batch {
     ag_device *dev;
-    uint32_t G, total_n, total_m, total_h;
+    uint32_t G, total_n, total_m;
     BatchArgs a;
-    void *allocs[16];
+    void *allocs[24];
     int nalloc;
-    BatchCfg *cfg_host;   // pinned
+    // part layout (rebuilt when part_edges / heavy_deg change)
+    uint32_t part_edges, heavy_deg;
+    BatchPart *d_parts;
+    uint32_t *d_gpart, *d_heavy;
+    double *d_perr;
+    uint32_t *d_pact;
+    // host copies kept for re-partitioning
+    BatchGraph *h_graphs;
+    uint32_t *h_off;
+    uint32_t *h_order;        // graphs by decreasing edge count
+    BatchCfg *cfg_host;       // pinned
+    uint32_t *ndone_host;     // pinned
     hipGraph_t graph;
     hipGraphExec_t exec;
-    int captured;
+    uint32_t exec_maxit;      // iterations of the captured solve (0: none)
 };
 
 namespace {
@@ -70,116 +97,136 @@ __device__ __forceinline__ void batch_finish(uint32_t r, float sum, float *rank,
     cout[r] = d ? __fdiv_rn(nw, __uint2float_rn(d)) : 0.0f;
 }
 
-__global__ void batch_reset_kernel(uint32_t *ticket) { *ticket = 0u; }
-
-__global__ __launch_bounds__(AG_BATCH_WG) void batch_kernel(BatchArgs A)
+__global__ void batch_reset_kernel(BatchArgs A)
 {
-    __shared__ uint32_t s_g;
-    __shared__ uint32_t s_stop;
-    __shared__ double s_e[AG_BATCH_WG / AG_WAVE];
-    __shared__ uint32_t s_a[AG_BATCH_WG / AG_WAVE];
-    const uint32_t tid = threadIdx.x;
-    const BatchCfg cfg = *A.cfg;
+    const uint32_t G = A.cfg->num_graphs, maxit = A.cfg->max_iter;
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < G; g += gridDim.x * blockDim.x) {
+        A.done[g] = 0u;
+        A.iters[g] = maxit;
+        A.err[g] = 0.0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        *A.ndone = 0u;
+}
+
+__global__ void batch_init_kernel(BatchArgs A)
+{
     const float damp = 0.85f;
     const float base = __fsub_rn(1.0f, damp);
-    for (;;) {
-        if (tid == 0)
-            s_g = atomicAdd(A.ticket, 1u);
-        __syncthreads();
-        const uint32_t k = s_g;
-        __syncthreads();
-        if (k >= cfg.num_graphs)
-            return;
-        const uint32_t gi = A.order[k];
-        const BatchGraph gr = A.graphs[gi];
-        const uint32_t n = gr.n;
+    const float x = A.cfg->init == AG_INIT_REFERENCE ? base : 1.0f;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < A.total_n;
+         v += gridDim.x * blockDim.x) {
+        A.rank[v] = x;
+        const uint32_t d = A.outdeg[v];
+        A.c0[v] = d ? __fdiv_rn(x, __uint2float_rn(d)) : 0.0f;
+    }
+}
+
+__global__ __launch_bounds__(AG_WG) void batch_part_kernel(BatchArgs A, uint32_t it)
+{
+    __shared__ double s_e[AG_WG / AG_WAVE];
+    __shared__ uint32_t s_a[AG_WG / AG_WAVE];
+    const BatchCfg cfg = *A.cfg;
+    if (*(volatile const uint32_t *)A.ndone >= cfg.num_graphs)
+        return;
+    const uint32_t tid = threadIdx.x, wave = tid / AG_WAVE, lane = tid % AG_WAVE;
+    const float damp = 0.85f;
+    const float base = __fsub_rn(1.0f, damp);
+    for (uint32_t p = blockIdx.x; p < A.nparts; p += gridDim.x) {
+        const BatchPart pt = A.parts[p];
+        if (A.done[pt.g])        // the same for every lane: written only by earlier launches
+            continue;
+        const BatchGraph gr = A.graphs[pt.g];
         const uint32_t *off = A.off + gr.ob;
         const uint32_t *src = A.src + gr.e0;
         const uint32_t *od = A.outdeg + gr.v0;
-        const uint32_t *heavy = A.heavy + gr.h0;
         float *rank = A.rank + gr.v0;
-        float *c0 = A.c0 + gr.v0;
-        float *c1 = A.c1 + gr.v0;
-        const double dn = (double)n;
-        for (uint32_t r = tid; r < n; r += AG_BATCH_WG) {
-            const float x = cfg.init == AG_INIT_REFERENCE ? base : 1.0f;
-            rank[r] = x;
-            const uint32_t d = od[r];
-            c0[r] = d ? __fdiv_rn(x, __uint2float_rn(d)) : 0.0f;
-        }
-        __syncthreads();
-        uint32_t it = 0, stopped = 0, act = 0;
-        double E = 0.0;
-        for (it = 0; it < cfg.max_iter; it++) {
-            const float *cin = (it & 1u) ? c1 : c0;
-            float *cout = (it & 1u) ? c0 : c1;
-            Red red{};
-            for (uint32_t r = tid; r < n; r += AG_BATCH_WG) {
-                const uint32_t b = off[r], e = off[r + 1];
-                if (e - b > AG_BATCH_HEAVY_DEG)
-                    continue;
-                float s = 0.0f;
-                if (e > b) {
-                    s = cin[src[b]];
-                    uint32_t j = b + 1;
-                    for (; j + 3 < e; j += 4) {
-                        const uint32_t u0 = src[j], u1 = src[j + 1], u2 = src[j + 2], u3 = src[j + 3];
-                        const float x0 = cin[u0], x1 = cin[u1], x2 = cin[u2], x3 = cin[u3];
-                        s = s + x0;
-                        s = s + x1;
-                        s = s + x2;
-                        s = s + x3;
-                    }
-                    for (; j < e; j++)
-                        s = s + cin[src[j]];
+        const float *cin = ((it & 1u) ? A.c1 : A.c0) + gr.v0;
+        float *cout = ((it & 1u) ? A.c0 : A.c1) + gr.v0;
+        const double dn = (double)gr.n;
+        Red red{};
+        for (uint32_t r = pt.r0 + tid; r < pt.r1; r += AG_WG) {
+            const uint32_t b = off[r], e = off[r + 1];
+            if (e - b > A.heavy_deg)
+                continue;
+            float s = 0.0f;
+            if (e > b) {
+                s = cin[src[b]];   // the first element starts the sum, as the FPGA's does
+                uint32_t j = b + 1;
+                for (; j + 3 < e; j += 4) {
+                    const uint32_t u0 = src[j], u1 = src[j + 1], u2 = src[j + 2], u3 = src[j + 3];
+                    const float x0 = cin[u0], x1 = cin[u1], x2 = cin[u2], x3 = cin[u3];
+                    s = s + x0;
+                    s = s + x1;
+                    s = s + x2;
+                    s = s + x3;
                 }
+                for (; j < e; j++)
+                    s = s + cin[src[j]];
+            }
+            batch_finish(r, s, rank, cout, od, base, damp, dn, cfg.eps, red);
+        }
+        for (uint32_t h = pt.h0 + wave; h < pt.h1; h += AG_WG / AG_WAVE) {
+            const uint32_t r = A.heavy[h];
+            const uint32_t b = off[r], e = off[r + 1];
+            float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+            uint32_t j = b + lane;
+            for (; j + 3 * AG_WAVE < e; j += 4 * AG_WAVE) {
+                const uint32_t u0 = src[j], u1 = src[j + AG_WAVE], u2 = src[j + 2 * AG_WAVE],
+                               u3 = src[j + 3 * AG_WAVE];
+                a0 = a0 + cin[u0];
+                a1 = a1 + cin[u1];
+                a2 = a2 + cin[u2];
+                a3 = a3 + cin[u3];
+            }
+            for (; j < e; j += AG_WAVE)
+                a0 = a0 + cin[src[j]];
+            const float s = group_reduce<float, AG_WAVE>((a0 + a1) + (a2 + a3));
+            if (lane == 0)
                 batch_finish(r, s, rank, cout, od, base, damp, dn, cfg.eps, red);
-            }
-            const uint32_t wave = tid / AG_WAVE, lane = tid % AG_WAVE;
-            for (uint32_t h = wave; h < gr.nh; h += AG_BATCH_WG / AG_WAVE) {
-                const uint32_t r = heavy[h];
-                const uint32_t b = off[r], e = off[r + 1];
-                float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
-                uint32_t j = b + lane;
-                for (; j + 3 * AG_WAVE < e; j += 4 * AG_WAVE) {
-                    const uint32_t u0 = src[j], u1 = src[j + AG_WAVE], u2 = src[j + 2 * AG_WAVE],
-                                   u3 = src[j + 3 * AG_WAVE];
-                    a0 = a0 + cin[u0];
-                    a1 = a1 + cin[u1];
-                    a2 = a2 + cin[u2];
-                    a3 = a3 + cin[u3];
-                }
-                for (; j < e; j += AG_WAVE)
-                    a0 = a0 + cin[src[j]];
-                float s = group_reduce<float, AG_WAVE>((a0 + a1) + (a2 + a3));
-                if (lane == 0)
-                    batch_finish(r, s, rank, cout, od, base, damp, dn, cfg.eps, red);
-            }
-            const Red t = block_reduce_red<AG_BATCH_WG>(red, s_e, s_a);
-            if (tid == 0) {
-                E = t.err;
-                act = t.active;
-                uint32_t stop = 0;
-                if (cfg.stop_rule == AG_STOP_ACTIVE)
-                    stop = (act == 0);
-                else if (cfg.stop_rule == AG_STOP_L1)
-                    stop = (E <= cfg.l1_tol);
-                s_stop = stop;
-            }
-            __syncthreads();
-            if (s_stop) {
-                stopped = 1;
-                break;
-            }
         }
-        const float nf = __uint2float_rn(n);
-        for (uint32_t r = tid; r < n; r += AG_BATCH_WG)
-            A.out[gr.v0 + r] = __fdiv_rn(rank[r], nf);
+        const Red t = block_reduce_red<AG_WG>(red, s_e, s_a);
         if (tid == 0) {
-            A.iters[gi] = stopped ? it : cfg.max_iter;
-            A.err[gi] = E;
+            A.perr[(size_t)(it & 1u) * A.nparts + p] = t.err;
+            A.pact[(size_t)(it & 1u) * A.nparts + p] = t.active;
         }
-        __syncthreads();
+    }
+}
+
+// One thread per graph: the graph's L1 delta and active count of iteration `it` (its parts'
+// partials in part order), then the stop rule.
+__global__ void batch_fin_kernel(BatchArgs A, uint32_t it)
+{
+    const BatchCfg cfg = *A.cfg;
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= cfg.num_graphs || A.done[g])
+        return;
+    double E = 0.0;
+    uint32_t act = 0;
+    const size_t pb = (size_t)(it & 1u) * A.nparts;
+    for (uint32_t p = A.gpart[2 * g]; p < A.gpart[2 * g + 1]; p++) {
+        E += A.perr[pb + p];
+        act += A.pact[pb + p];
+    }
+    A.err[g] = E;
+    const bool stop = (cfg.stop_rule == AG_STOP_ACTIVE && act == 0) ||
+                      (cfg.stop_rule == AG_STOP_L1 && E <= cfg.l1_tol);
+    if (stop) {
+        A.done[g] = 1u;
+        A.iters[g] = it;
+        atomicAdd(A.ndone, 1u);
+    }
+}
+
+// Normalised ranks (scaled / n of the vertex's graph); a workgroup per graph.
+__global__ void batch_out_kernel(BatchArgs A)
+{
+    const uint32_t G = A.cfg->num_graphs;
+    for (uint32_t g = blockIdx.x; g < G; g += gridDim.x) {
+        const BatchGraph gr = A.graphs[g];
+        const float nf = __uint2float_rn(gr.n);
+        for (uint32_t r = threadIdx.x; r < gr.n; r += blockDim.x)
+            A.out[gr.v0 + r] = __fdiv_rn(A.rank[gr.v0 + r], nf);
     }
 }
 
@@ -194,6 +241,30 @@ extern "C" void ag_batch_options_default(ag_batch_options *o)
     o->l1_tolerance = 1e-6;
     o->init = AG_INIT_UNIFORM;
     o->use_hip_graph = 1;
+    o->part_edges = AG_BATCH_PART_EDGES;
+    o->heavy_deg = AG_BATCH_HEAVY_DEG;
+}
+
+static void drop_capture(ag_batch *b)
+{
+    if (b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+        (void)hipGraphDestroy(b->graph);
+        b->exec = NULL;
+        b->graph = NULL;
+    }
+    b->exec_maxit = 0;
+}
+
+static void free_layout(ag_batch *b)
+{
+    void *p[] = {b->d_parts, b->d_gpart, b->d_heavy, b->d_perr, b->d_pact};
+    for (void *q : p)
+        if (q)
+            (void)hipFree(q);
+    b->d_parts = NULL;
+    b->d_gpart = b->d_heavy = b->d_pact = NULL;
+    b->d_perr = NULL;
 }
 
 extern "C" int ag_batch_free(ag_batch *b)
@@ -202,14 +273,17 @@ extern "C" int ag_batch_free(ag_batch *b)
         return AG_OK;
     (void)hipSetDevice(b->dev->ordinal);
     (void)hipStreamSynchronize(b->dev->stream);
-    if (b->captured) {
-        (void)hipGraphExecDestroy(b->exec);
-        (void)hipGraphDestroy(b->graph);
-    }
+    drop_capture(b);
+    free_layout(b);
     for (int i = 0; i < b->nalloc; i++)
         (void)hipFree(b->allocs[i]);
     if (b->cfg_host)
         (void)hipHostFree(b->cfg_host);
+    if (b->ndone_host)
+        (void)hipHostFree(b->ndone_host);
+    free(b->h_graphs);
+    free(b->h_off);
+    free(b->h_order);
     if (b->dev->live_graphs)
         b->dev->live_graphs--;
     free(b);
@@ -227,6 +301,111 @@ static int balloc(ag_batch *b, T **p, size_t count, const T *src)
     return AG_OK;
 }
 
+// Builds and uploads the parts, the per-graph part ranges and the heavy-row lists for
+// (part_edges, heavy_deg), replacing any earlier layout (and the captured solve).
+static int build_layout(ag_batch *b, uint32_t part_edges, uint32_t heavy_deg)
+{
+    const uint32_t G = b->G;
+    // heavy rows (graph by graph, ascending) and parts (graphs by decreasing edge count)
+    size_t nh = 0, np = 0;
+    for (uint32_t g = 0; g < G; g++) {
+        const BatchGraph &gr = b->h_graphs[g];
+        const uint32_t *off = b->h_off + gr.ob;
+        uint32_t r = 0;
+        while (r < gr.n) {
+            uint32_t rows = 0, e = 0;
+            while (r < gr.n && rows < AG_BATCH_PART_ROWS) {
+                const uint32_t d = off[r + 1] - off[r];
+                if (rows > 0 && (uint64_t)e + d > part_edges)
+                    break;
+                e += d;
+                rows++;
+                r++;
+            }
+            np++;
+        }
+        for (uint32_t q = 0; q < gr.n; q++)
+            nh += (off[q + 1] - off[q] > heavy_deg);
+    }
+    BatchPart *parts = (BatchPart *)malloc((np ? np : 1) * sizeof(BatchPart));
+    uint32_t *gpart = (uint32_t *)malloc((size_t)G * 2 * sizeof(uint32_t));
+    uint32_t *heavy = (uint32_t *)malloc((nh ? nh : 1) * sizeof(uint32_t));
+    uint32_t *h0 = (uint32_t *)malloc((size_t)(G + 1) * sizeof(uint32_t));
+    if (!parts || !gpart || !heavy || !h0) {
+        free(parts); free(gpart); free(heavy); free(h0);
+        return ag_set_error(AG_ERR_OOM, "host allocation failed");
+    }
+    size_t k = 0;
+    for (uint32_t g = 0; g < G; g++) {
+        const BatchGraph &gr = b->h_graphs[g];
+        const uint32_t *off = b->h_off + gr.ob;
+        h0[g] = (uint32_t)k;
+        for (uint32_t q = 0; q < gr.n; q++)
+            if (off[q + 1] - off[q] > heavy_deg)
+                heavy[k++] = q;
+    }
+    h0[G] = (uint32_t)k;
+    size_t p = 0;
+    for (uint32_t i = 0; i < G; i++) {
+        const uint32_t g = b->h_order[i];
+        const BatchGraph &gr = b->h_graphs[g];
+        const uint32_t *off = b->h_off + gr.ob;
+        gpart[2 * g] = (uint32_t)p;
+        uint32_t r = 0, h = h0[g];
+        while (r < gr.n) {
+            const uint32_t r0 = r, hs = h;
+            uint32_t rows = 0, e = 0;
+            while (r < gr.n && rows < AG_BATCH_PART_ROWS) {
+                const uint32_t d = off[r + 1] - off[r];
+                if (rows > 0 && (uint64_t)e + d > part_edges)
+                    break;
+                e += d;
+                rows++;
+                r++;
+            }
+            while (h < h0[g + 1] && heavy[h] < r)
+                h++;
+            parts[p++] = BatchPart{g, r0, r, hs, h};
+        }
+        gpart[2 * g + 1] = (uint32_t)p;
+    }
+    free(h0);
+    drop_capture(b);
+    free_layout(b);
+    int st = AG_OK;
+    hipStream_t s = b->dev->stream;
+    hipError_t e = hipMalloc((void **)&b->d_parts, (np ? np : 1) * sizeof(BatchPart));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_gpart, (size_t)G * 2 * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_heavy, (nh ? nh : 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_perr, (np ? np : 1) * 2 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_pact, (np ? np : 1) * 2 * sizeof(uint32_t));
+    if (e == hipSuccess && np)
+        e = hipMemcpyAsync(b->d_parts, parts, np * sizeof(BatchPart), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->d_gpart, gpart, (size_t)G * 2 * sizeof(uint32_t), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && nh)
+        e = hipMemcpyAsync(b->d_heavy, heavy, nh * sizeof(uint32_t), hipMemcpyHostToDevice, s);
+    st = (e == hipSuccess) ? ag_wait_stream(b->dev)   // the host arrays must outlive the copies
+                           : ag_hip_fail(e, "batch layout upload", __FILE__, __LINE__);
+    free(parts);
+    free(gpart);
+    free(heavy);
+    if (st != AG_OK) {
+        free_layout(b);
+        return st;
+    }
+    b->part_edges = part_edges;
+    b->heavy_deg = heavy_deg;
+    b->a.parts = b->d_parts;
+    b->a.gpart = b->d_gpart;
+    b->a.heavy = b->d_heavy;
+    b->a.perr = b->d_perr;
+    b->a.pact = b->d_pact;
+    b->a.nparts = (uint32_t)np;
+    b->a.heavy_deg = heavy_deg;
+    return AG_OK;
+}
+
 extern "C" int ag_batch_upload(ag_device *dev, uint32_t G, const uint32_t *vert_off,
                                const uint32_t *edge_off, const uint32_t *out_degree,
                                const uint32_t *inv_edges_idx, const uint32_t *inv_degree,
@@ -240,16 +419,16 @@ extern "C" int ag_batch_upload(ag_device *dev, uint32_t G, const uint32_t *vert_
         return ag_set_error(AG_ERR_INVALID, "offsets must start at 0");
     if ((N && (!out_degree || !inv_edges_idx || !inv_degree)) || (M && !inv_src))
         return ag_set_error(AG_ERR_INVALID, "null array");
+    if ((uint64_t)N + G > 0xFFFF0000ull)
+        return ag_set_error(AG_ERR_INVALID, "too many vertices in one batch");
     BatchGraph *hg = (BatchGraph *)calloc(G, sizeof(BatchGraph));
     uint32_t *hoff = (uint32_t *)malloc(((size_t)N + G) * sizeof(uint32_t));
     uint32_t *hsrc = (uint32_t *)malloc((size_t)(M ? M : 1) * sizeof(uint32_t));
     uint32_t *horder = (uint32_t *)malloc((size_t)G * sizeof(uint32_t));
     uint64_t *keys = (uint64_t *)malloc((size_t)G * sizeof(uint64_t));
-    uint32_t *hheavy = (uint32_t *)malloc((size_t)(N ? N : 1) * sizeof(uint32_t));
     int st = AG_OK;
-    if (!hg || !hoff || !hsrc || !horder || !keys || !hheavy)
+    if (!hg || !hoff || !hsrc || !horder || !keys)
         st = ag_set_error(AG_ERR_OOM, "host allocation failed");
-    uint32_t nh_total = 0;
     for (uint32_t g = 0; g < G && st == AG_OK; g++) {
         const uint32_t v0 = vert_off[g], v1 = vert_off[g + 1], e0 = edge_off[g], e1 = edge_off[g + 1];
         if (v1 < v0 || e1 < e0) {
@@ -257,22 +436,20 @@ extern "C" int ag_batch_upload(ag_device *dev, uint32_t G, const uint32_t *vert_
             break;
         }
         const uint32_t n = v1 - v0, m = e1 - e0;
-        BatchGraph b;
-        b.v0 = v0;
-        b.n = n;
-        b.e0 = e0;
-        b.ob = v0 + g;
-        b.h0 = nh_total;
-        b.nh = 0;
-        b.pad = 0;
+        BatchGraph bg;
+        memset(&bg, 0, sizeof(bg));
+        bg.v0 = v0;
+        bg.n = n;
+        bg.e0 = e0;
+        bg.ob = v0 + g;
         uint64_t acc = 0;
         for (uint32_t r = 0; r < n && st == AG_OK; r++) {
             const uint32_t idx = inv_edges_idx[v0 + r], d = inv_degree[v0 + r];
-            if ((uint64_t)idx + d > m) {
+            if ((uint64_t)idx + d > m || acc + d > m) {
                 st = ag_set_error(AG_ERR_INVALID, "graph %u row %u runs past its edges", g, r);
                 break;
             }
-            hoff[b.ob + r] = (uint32_t)acc;
+            hoff[bg.ob + r] = (uint32_t)acc;
             for (uint32_t k = 0; k < d; k++) {
                 const uint32_t s = inv_src[e0 + idx + k];
                 if (s >= n) {
@@ -282,23 +459,16 @@ extern "C" int ag_batch_upload(ag_device *dev, uint32_t G, const uint32_t *vert_
                 hsrc[e0 + acc + k] = s;
             }
             acc += d;
-            if (d > AG_BATCH_HEAVY_DEG)
-                hheavy[nh_total + b.nh++] = r;
         }
         if (st != AG_OK)
             break;
-        if (acc > m) {
-            st = ag_set_error(AG_ERR_INVALID, "graph %u degrees exceed its edge range", g);
-            break;
-        }
-        hoff[b.ob + n] = (uint32_t)acc;
-        b.m = (uint32_t)acc;
-        nh_total += b.nh;
-        hg[g] = b;
-        keys[g] = ((uint64_t)(0xFFFFFFFFu - b.m) << 32) | g;  // largest first, then by index
+        hoff[bg.ob + n] = (uint32_t)acc;
+        bg.m = (uint32_t)acc;
+        hg[g] = bg;
+        keys[g] = ((uint64_t)(0xFFFFFFFFu - bg.m) << 32) | g;  // largest first, then by index
     }
     if (st != AG_OK) {
-        free(hg); free(hoff); free(hsrc); free(horder); free(keys); free(hheavy);
+        free(hg); free(hoff); free(hsrc); free(horder); free(keys);
         return st;
     }
     qsort(keys, G, sizeof(uint64_t), [](const void *a, const void *b) {
@@ -307,11 +477,12 @@ extern "C" int ag_batch_upload(ag_device *dev, uint32_t G, const uint32_t *vert_
     });
     for (uint32_t g = 0; g < G; g++)
         horder[g] = (uint32_t)keys[g];
+    free(keys);
 
     hipError_t he = hipSetDevice(dev->ordinal);
     ag_batch *b = (ag_batch *)calloc(1, sizeof(ag_batch));
     if (he != hipSuccess || !b) {
-        free(hg); free(hoff); free(hsrc); free(horder); free(keys); free(hheavy); free(b);
+        free(hg); free(hoff); free(hsrc); free(horder); free(b);
         return he != hipSuccess ? ag_hip_fail(he, "hipSetDevice", __FILE__, __LINE__)
                                 : ag_set_error(AG_ERR_OOM, "host allocation failed");
     }
@@ -319,57 +490,96 @@ extern "C" int ag_batch_upload(ag_device *dev, uint32_t G, const uint32_t *vert_
     b->G = G;
     b->total_n = N;
     b->total_m = M;
-    b->total_h = nh_total;
+    b->h_graphs = hg;
+    b->h_off = hoff;
+    b->h_order = horder;
+    dev->live_graphs++;
     BatchGraph *dgraphs = NULL;
-    uint32_t *dorder = NULL, *doff = NULL, *dsrc = NULL, *dod = NULL, *dheavy = NULL, *dit = NULL,
-             *dticket = NULL;
+    uint32_t *doff = NULL, *dsrc = NULL, *dod = NULL, *dit = NULL, *ddone = NULL, *dnd = NULL;
     float *drank = NULL, *dc0 = NULL, *dc1 = NULL, *dout = NULL;
     double *derr = NULL;
     BatchCfg *dcfg = NULL;
     if (st == AG_OK) st = balloc(b, &dgraphs, G, hg);
-    if (st == AG_OK) st = balloc(b, &dorder, G, horder);
     if (st == AG_OK) st = balloc(b, &doff, (size_t)N + G, hoff);
     if (st == AG_OK) st = balloc(b, &dsrc, M, hsrc);
     if (st == AG_OK) st = balloc(b, &dod, N, out_degree);
-    if (st == AG_OK) st = balloc(b, &dheavy, nh_total, hheavy);
     if (st == AG_OK) st = balloc<float>(b, &drank, N, NULL);
     if (st == AG_OK) st = balloc<float>(b, &dc0, N, NULL);
     if (st == AG_OK) st = balloc<float>(b, &dc1, N, NULL);
     if (st == AG_OK) st = balloc<float>(b, &dout, N, NULL);
     if (st == AG_OK) st = balloc<uint32_t>(b, &dit, G, NULL);
     if (st == AG_OK) st = balloc<double>(b, &derr, G, NULL);
-    if (st == AG_OK) st = balloc<uint32_t>(b, &dticket, 1, NULL);
+    if (st == AG_OK) st = balloc<uint32_t>(b, &ddone, G, NULL);
+    if (st == AG_OK) st = balloc<uint32_t>(b, &dnd, 1, NULL);
     if (st == AG_OK) st = balloc<BatchCfg>(b, &dcfg, 1, NULL);
     if (st == AG_OK) {
         he = hipHostMalloc((void **)&b->cfg_host, sizeof(BatchCfg), hipHostMallocDefault);
+        if (he == hipSuccess)
+            he = hipHostMalloc((void **)&b->ndone_host, sizeof(uint32_t), hipHostMallocDefault);
         if (he != hipSuccess)
             st = ag_hip_fail(he, "hipHostMalloc", __FILE__, __LINE__);
     }
     if (st == AG_OK)
         st = ag_wait_stream(dev);  // host staging buffers must outlive the copies
-    free(hg); free(hoff); free(hsrc); free(horder); free(keys); free(hheavy);
-    dev->live_graphs++;
+    free(hsrc);
+    if (st == AG_OK) {
+        memset(&b->a, 0, sizeof(b->a));
+        b->a.graphs = dgraphs;
+        b->a.off = doff;
+        b->a.src = dsrc;
+        b->a.outdeg = dod;
+        b->a.rank = drank;
+        b->a.c0 = dc0;
+        b->a.c1 = dc1;
+        b->a.out = dout;
+        b->a.iters = dit;
+        b->a.err = derr;
+        b->a.done = ddone;
+        b->a.ndone = dnd;
+        b->a.cfg = dcfg;
+        b->a.total_n = N;
+        st = build_layout(b, AG_BATCH_PART_EDGES, AG_BATCH_HEAVY_DEG);
+    }
     if (st != AG_OK) {
         ag_batch_free(b);
         return st;
     }
-    b->a = BatchArgs{dgraphs, dorder, doff, dsrc, dod, dheavy, drank, dc0, dc1, dout,
-                     dit, derr, dticket, dcfg};
     *out = b;
     return AG_OK;
 }
 
-static int batch_enqueue(ag_batch *b)
+static uint32_t part_grid(const ag_batch *b)
 {
-    hipStream_t s = b->dev->stream;
-    static const uint32_t per_cu = ag_resident_per_cu(batch_kernel, AG_BATCH_WG);
-    uint32_t grid = b->dev->num_cus * per_cu;
-    if (grid > b->G)
-        grid = b->G;
-    batch_reset_kernel<<<1, 1, 0, s>>>(b->a.ticket);
-    batch_kernel<<<grid, AG_BATCH_WG, 0, s>>>(b->a);
-    AG_HIP(hipGetLastError());
-    return AG_OK;
+    static const uint32_t per_cu = ag_resident_per_cu(batch_part_kernel, AG_WG);
+    uint32_t grid = b->dev->num_cus * (per_cu < 8u ? per_cu : 8u);
+    if (grid > b->a.nparts)
+        grid = b->a.nparts;
+    return grid ? grid : 1u;
+}
+
+static hipError_t enqueue_head(ag_batch *b, hipStream_t s)
+{
+    const uint32_t g1 = (b->G + 255) / 256;
+    batch_reset_kernel<<<g1, 256, 0, s>>>(b->a);
+    const uint32_t gv = (b->total_n + 255) / 256 < 4096u ? (b->total_n + 255) / 256 : 4096u;
+    batch_init_kernel<<<gv ? gv : 1u, 256, 0, s>>>(b->a);
+    return hipGetLastError();
+}
+
+static hipError_t enqueue_iterations(ag_batch *b, uint32_t it0, uint32_t it1, hipStream_t s)
+{
+    const uint32_t grid = part_grid(b), gf = (b->G + 255) / 256;
+    for (uint32_t it = it0; it < it1; it++) {
+        batch_part_kernel<<<grid, AG_WG, 0, s>>>(b->a, it);
+        batch_fin_kernel<<<gf, 256, 0, s>>>(b->a, it);
+    }
+    return hipGetLastError();
+}
+
+static hipError_t enqueue_tail(ag_batch *b, hipStream_t s)
+{
+    batch_out_kernel<<<b->G < 4096u ? b->G : 4096u, 256, 0, s>>>(b->a);
+    return hipGetLastError();
 }
 
 extern "C" int ag_batch_pagerank(ag_batch *b, const ag_batch_options *opts, float *ranks_out,
@@ -384,14 +594,22 @@ extern "C" int ag_batch_pagerank(ag_batch *b, const ag_batch_options *opts, floa
         ag_batch_options_default(&o);
     if (o.stop_rule > AG_STOP_NONE || o.init > AG_INIT_UNIFORM)
         return ag_set_error(AG_ERR_INVALID, "unsupported stop rule or init mode");
+    const uint32_t part_edges = o.part_edges ? o.part_edges : AG_BATCH_PART_EDGES;
+    const uint32_t heavy_deg = o.heavy_deg ? o.heavy_deg : AG_BATCH_HEAVY_DEG;
+    const uint32_t maxit = o.max_iterations;
     AG_HIP(hipSetDevice(b->dev->ordinal));
     hipStream_t s = b->dev->stream;
-    // options go through a device buffer so a captured graph stays valid
-    int st = ag_wait_stream(b->dev);  // the pinned staging buffer may still feed a copy
+    int st = ag_wait_stream(b->dev);  // the pinned staging buffers may still feed a copy
     if (st != AG_OK)
         return st;
+    if (part_edges != b->part_edges || heavy_deg != b->heavy_deg) {
+        st = build_layout(b, part_edges, heavy_deg);
+        if (st != AG_OK)
+            return st;
+    }
+    // options go through a device buffer so a captured solve stays valid
     BatchCfg c;
-    c.max_iter = o.max_iterations;
+    c.max_iter = maxit;
     c.stop_rule = o.stop_rule;
     c.init = o.init;
     c.num_graphs = b->G;
@@ -403,49 +621,60 @@ extern "C" int ag_batch_pagerank(ag_batch *b, const ag_batch_options *opts, floa
     AG_HIP(hipEventCreate(&e0));
     AG_HIP(hipEventCreate(&e1));
     hipError_t he = hipEventRecord(e0, s);
-    if (he == hipSuccess) {
-        if (o.use_hip_graph) {
-            if (!b->captured) {
-                he = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
-                if (he == hipSuccess) {
-                    int est = batch_enqueue(b);
-                    hipError_t ce = hipStreamEndCapture(s, &b->graph);
-                    if (est != AG_OK) {
-                        if (ce == hipSuccess)
-                            (void)hipGraphDestroy(b->graph);
-                        (void)hipEventDestroy(e0);
-                        (void)hipEventDestroy(e1);
-                        return est;
-                    }
-                    he = ce;
+    if (he == hipSuccess && o.use_hip_graph) {
+        if (b->exec && b->exec_maxit != maxit)
+            drop_capture(b);
+        if (!b->exec) {
+            he = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+            if (he == hipSuccess) {
+                hipError_t qe = enqueue_head(b, s);
+                if (qe == hipSuccess)
+                    qe = enqueue_iterations(b, 0, maxit, s);
+                if (qe == hipSuccess)
+                    qe = enqueue_tail(b, s);
+                hipError_t ce = hipStreamEndCapture(s, &b->graph);
+                he = qe != hipSuccess ? qe : ce;
+                if (qe != hipSuccess && ce == hipSuccess) {
+                    (void)hipGraphDestroy(b->graph);
+                    b->graph = NULL;
                 }
-                if (he == hipSuccess)
-                    he = hipGraphInstantiate(&b->exec, b->graph, NULL, NULL, 0);
-                if (he == hipSuccess)
-                    b->captured = 1;
             }
             if (he == hipSuccess)
-                he = hipGraphLaunch(b->exec, s);
-        } else {
-            int est = batch_enqueue(b);
-            if (est != AG_OK) {
-                (void)hipEventDestroy(e0);
-                (void)hipEventDestroy(e1);
-                return est;
-            }
+                he = hipGraphInstantiate(&b->exec, b->graph, NULL, NULL, 0);
+            if (he == hipSuccess)
+                b->exec_maxit = maxit;
         }
+        if (he == hipSuccess)
+            he = hipGraphLaunch(b->exec, s);
+    } else if (he == hipSuccess) {
+        // eager: iterations queued in chunks, polling the stopped-graph count in between
+        he = enqueue_head(b, s);
+        const uint32_t chunk = 16;
+        for (uint32_t it = 0; he == hipSuccess && st == AG_OK && it < maxit; it += chunk) {
+            he = enqueue_iterations(b, it, it + chunk < maxit ? it + chunk : maxit, s);
+            if (he == hipSuccess)
+                he = hipMemcpyAsync(b->ndone_host, b->a.ndone, sizeof(uint32_t),
+                                    hipMemcpyDeviceToHost, s);
+            if (he == hipSuccess)
+                st = ag_wait_stream(b->dev);
+            if (st == AG_OK && he == hipSuccess && *b->ndone_host >= b->G)
+                break;
+        }
+        if (he == hipSuccess && st == AG_OK)
+            he = enqueue_tail(b, s);
     }
-    if (he == hipSuccess)
+    if (he == hipSuccess && st == AG_OK)
         he = hipEventRecord(e1, s);
-    if (he == hipSuccess && ranks_out && b->total_n)
+    if (he == hipSuccess && st == AG_OK && ranks_out && b->total_n)
         he = hipMemcpyAsync(ranks_out, b->a.out, (size_t)b->total_n * sizeof(float),
                             hipMemcpyDeviceToHost, s);
-    if (he == hipSuccess && iterations)
+    if (he == hipSuccess && st == AG_OK && iterations)
         he = hipMemcpyAsync(iterations, b->a.iters, (size_t)b->G * sizeof(uint32_t),
                             hipMemcpyDeviceToHost, s);
-    if (he == hipSuccess && error)
+    if (he == hipSuccess && st == AG_OK && error)
         he = hipMemcpyAsync(error, b->a.err, (size_t)b->G * sizeof(double), hipMemcpyDeviceToHost, s);
-    st = (he == hipSuccess) ? ag_wait_stream(b->dev) : ag_hip_fail(he, "batch launch", __FILE__, __LINE__);
+    if (st == AG_OK)
+        st = (he == hipSuccess) ? ag_wait_stream(b->dev) : ag_hip_fail(he, "batch launch", __FILE__, __LINE__);
     if (st == AG_OK && device_ms) {
         float ms = 0.0f;
         he = hipEventElapsedTime(&ms, e0, e1);

@@ -75,7 +75,12 @@ struct ag_dgraph {
     void *vec_a;
     void *vec_b;
     size_t vec_bytes;
+    // personalised PageRank: buffers and the captured solve, kept between calls
+    struct ag_ppr_ws *ppr;
 };
+
+// Frees g->ppr (ag_ppr.hip).
+void ag_ppr_free_ws(ag_dgraph *g);
 
 // Geometry passed by value to the pull kernels.
 struct PullGeom {
@@ -89,6 +94,8 @@ struct PullGeom {
     uint64_t *hub_part;
     uint32_t n;
     uint32_t n_hubs;
+    uint32_t seq_hubs;       // plan in reference order: a hub row is one segment, added by one
+                             // lane in sequence (chunks staged through LDS)
     uint32_t bin_row[AG_NBINS + 1];
     uint32_t slot_blk0[AG_NSLOTS + 1];
 };
@@ -107,7 +114,18 @@ struct BlockGeom {
     uint32_t nblocks;
     uint32_t item_off[AG_NXCD + 1];
     uint32_t c_item_off[AG_NXCD + 1];
+    // pass 1 over the L2 takes the items [item_lo[x], item_off[x + 1]) of slice x: from
+    // item_off[x] (every tile), or from the end of the hot-segment tiles when the hot pass
+    // (groups hgrp, hgrp_off) gathers those from LDS
+    uint32_t item_lo[AG_NXCD];
+    const uint32_t *hgrp;    // [3 * n_hgrp] {first item, item count, segment}
+    uint32_t hgrp_off[AG_NXCD + 1];
+    uint32_t hot_bits;
+    uint32_t n_src;          // source (vertex) count: the last segment may be short
 };
+
+// Points pass 1 of B past the hot-segment tiles (when the hot pass runs before it).
+void ag_block_geom_skip_hot(const ag_dgraph *g, BlockGeom &B);
 
 // Persistent-grid size of a pull launch: resident workgroups per CU x CUs, capped by the
 // number of work items. Per CU: the plan's wg_per_cu, else default_per_cu, never more than

@@ -192,6 +192,8 @@ struct PrF32Pol {
     double dn, eps;
 
     __device__ __forceinline__ float edge(uint32_t, uint32_t s) const { return ld_f32(cin[s]); }
+    // the same term from a stored contribution already in hand (hot-segment LDS copy)
+    __device__ __forceinline__ float edge_val(uint32_t, RT v) const { return ld_f32(v); }
     __device__ __forceinline__ void finish(uint32_t r, float sum, Red &red) const
     {
         const float nw = __fadd_rn(base, __fmul_rn(damp, sum));
@@ -304,6 +306,38 @@ __device__ __forceinline__ void pr_f32_cmb(const BlockGeom &B, const PrCtl &C,
         pull_combine_items(B, p, red, s_vals, s_wave, L);
     });
     pr_store_partial(C, red);
+}
+
+// Hot pass of a source-blocked iteration: the hot-segment tiles, gathered from an LDS copy
+// of each tile group's segment of this iteration's contributions (partials only).
+template <typename HT>
+__device__ __forceinline__ void pr_f32_hot(const BlockGeom &B, const PrCtl &C,
+                                           const PrF32Args &A, bool mixed)
+{
+    __shared__ uint4 s_tab[AG_HOT_TAB_BYTES / 16];
+    __shared__ float s_vals[AG_HOT_NNZ];
+    __shared__ uint32_t s_long[AG_LONG_MAX];
+    __shared__ uint32_t s_nlong;
+    const PrState *st = C.st;
+    if (*(volatile const uint32_t *)&st->done)
+        return;
+    const uint32_t iter = *(volatile const uint32_t *)&st->iter;
+    pr_f32_dispatch<HT>(st, iter, A, mixed, [&](const auto &p) {
+        pull_hot_items(B, p, p.cin, s_tab, s_vals, s_long, &s_nlong);
+    });
+}
+
+__global__ __launch_bounds__(AG_WG) void pr_f32_hot_kernel(BlockGeom B, PrCtl C, PrF32Args A)
+{
+    pr_f32_hot<float>(B, C, A, false);
+}
+__global__ __launch_bounds__(AG_WG) void pr_bf16_hot_kernel(BlockGeom B, PrCtl C, PrF32Args A)
+{
+    pr_f32_hot<bf16s>(B, C, A, true);
+}
+__global__ __launch_bounds__(AG_WG) void pr_f16_hot_kernel(BlockGeom B, PrCtl C, PrF32Args A)
+{
+    pr_f32_hot<_Float16>(B, C, A, true);
 }
 
 __global__ __launch_bounds__(AG_WG) void pr_f32_blk_kernel(PullGeom U, BlockGeom B, PrCtl C,
@@ -1360,10 +1394,24 @@ extern "C" int ag_pr_begin(ag_dgraph *g, const ag_pr_options *opts, ag_pr_sessio
         const uint32_t g2 = ag_blocked_grid(g, res2, AG_COMBINE_PER_CU);                    \
         C.nparts = g1 + g2;                                                  \
         C.part_base = 0;                                                     \
-        BLK<<<g1, block, 0, stream>>>(G, B, C, A);                           \
+        BLK<<<g1, block, 0, stream>>>(G, B1, C, A);                          \
         C.part_base = g1;                                                    \
         CMB<<<g2, block, 0, stream>>>(B, C, A);                              \
         FIN<<<1, fin, 0, stream>>>(G, C, A);                                 \
+    } while (0)
+
+// The FP32 kinds (any contribution storage) first gather the hot-segment tiles from LDS
+// (when the plan has hot segments); pass 1 then skips them.
+#define AG_HOT_PER_CU 2u
+#define AG_PR_LAUNCH_BLOCKED_HOT(HOT, BLK, CMB, FIN)                         \
+    do {                                                                     \
+        if (g->blk.n_hgrp) {                                                 \
+            static const uint32_t resh = ag_resident_per_cu(HOT);           \
+            const uint32_t gh = ag_blocked_grid(g, resh, AG_HOT_PER_CU);    \
+            HOT<<<gh, block, 0, stream>>>(B, C, A);                          \
+            ag_block_geom_skip_hot(g, B1);                                   \
+        }                                                                    \
+        AG_PR_LAUNCH_BLOCKED(BLK, CMB, FIN);                                 \
     } while (0)
 
 int ag_pr_launch_iteration(ag_pr_session *s)
@@ -1421,6 +1469,7 @@ int ag_pr_launch_iteration(ag_pr_session *s)
     }
     if (s->o.direction == AG_DIR_DATA_DRIVEN && blocked && !s->dd_queue[0]) {
         const BlockGeom B = ag_make_block_geom(g);
+        const BlockGeom &B1 = B;
         AG_PR_LAUNCH_BLOCKED(pr_ddf_blk_kernel, pr_ddf_cmb_kernel, pr_ddf_finish);
         AG_HIP(hipGetLastError());
         return AG_OK;
@@ -1442,10 +1491,20 @@ int ag_pr_launch_iteration(ag_pr_session *s)
     }
     if (blocked) {
         const BlockGeom B = ag_make_block_geom(g);
+        BlockGeom B1 = B;   // pass 1: past the hot tiles when the hot pass gathers them
         switch (s->kind_code) {
-        case 0: AG_PR_LAUNCH_BLOCKED(pr_f32_blk_kernel, pr_f32_cmb_kernel, pr_f32_finish); break;
-        case 1: AG_PR_LAUNCH_BLOCKED(pr_bf16_blk_kernel, pr_bf16_cmb_kernel, pr_bf16_finish); break;
-        case 2: AG_PR_LAUNCH_BLOCKED(pr_f16_blk_kernel, pr_f16_cmb_kernel, pr_f16_finish); break;
+        case 0:
+            AG_PR_LAUNCH_BLOCKED_HOT(pr_f32_hot_kernel, pr_f32_blk_kernel, pr_f32_cmb_kernel,
+                                     pr_f32_finish);
+            break;
+        case 1:
+            AG_PR_LAUNCH_BLOCKED_HOT(pr_bf16_hot_kernel, pr_bf16_blk_kernel, pr_bf16_cmb_kernel,
+                                     pr_bf16_finish);
+            break;
+        case 2:
+            AG_PR_LAUNCH_BLOCKED_HOT(pr_f16_hot_kernel, pr_f16_blk_kernel, pr_f16_cmb_kernel,
+                                     pr_f16_finish);
+            break;
         case 64:
             AG_PR_LAUNCH_BLOCKED(pr_int_blk_kernel<64>, pr_int_cmb_kernel<64>, pr_int_finish<64>);
             break;

@@ -175,6 +175,34 @@ __device__ __forceinline__ void pull_hub_segment(const PullGeom &G, const P &pol
         G.hub_part[s] = acc_bits<Acc>(acc);
 }
 
+// Reference order: hub segment s (the whole row) added by one lane in edge order, the first
+// element starting the sum, over chunks of AG_TILE_NNZ_MAX values the workgroup gathers into
+// LDS (coalesced source-id reads); the accumulator carries from chunk to chunk.
+template <class P>
+__device__ __forceinline__ void pull_hub_segment_seq(const PullGeom &G, const P &pol,
+                                                     uint32_t s, typename P::Acc *s_vals)
+{
+    using Acc = typename P::Acc;
+    const uint32_t lo = G.seg_lo[s], hi = G.seg_hi[s];
+    Acc acc = Acc(0);
+    for (uint32_t base = lo; base < hi; base += AG_TILE_NNZ_MAX) {
+        const uint32_t nnz = hi - base < AG_TILE_NNZ_MAX ? hi - base : AG_TILE_NNZ_MAX;
+        for (uint32_t k = threadIdx.x; k < nnz; k += AG_WG)
+            s_vals[k] = pol.edge(base + k, ld_stream(&G.src[base + k]));
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t q = 0;
+            if (base == lo)
+                acc = s_vals[q++];
+            for (; q < nnz; q++)
+                acc = acc + s_vals[q];
+        }
+        __syncthreads();   // s_vals is refilled by the next chunk
+    }
+    if (threadIdx.x == 0)
+        G.hub_part[s] = acc_bits<Acc>(acc);
+}
+
 template <class P>
 __device__ __forceinline__ void pull_block_row(const PullGeom &G, const P &pol, uint32_t row,
                                                Red &red, typename P::Acc *s_wave)
@@ -264,9 +292,12 @@ __device__ __forceinline__ void pull_one_item(const PullGeom &G, const P &pol, u
                                               Red &red, typename P::Acc *s_vals,
                                               typename P::Acc *s_wave)
 {
-    if (it < G.slot_blk0[1])
-        pull_hub_segment(G, pol, it - G.slot_blk0[0], s_wave);
-    else if (it < G.slot_blk0[2])
+    if (it < G.slot_blk0[1]) {
+        if (G.seq_hubs)
+            pull_hub_segment_seq(G, pol, it - G.slot_blk0[0], s_vals);
+        else
+            pull_hub_segment(G, pol, it - G.slot_blk0[0], s_wave);
+    } else if (it < G.slot_blk0[2])
         pull_block_row(G, pol, G.bin_row[AG_BIN_BLOCK] + (it - G.slot_blk0[1]), red, s_wave);
     else if (it < G.slot_blk0[3])
         pull_group_rows<P, AG_WAVE>(G, pol, AG_BIN_WAVE, it - G.slot_blk0[2], red);
@@ -410,7 +441,7 @@ __device__ __forceinline__ void pull_blocked_items(const PullGeom &U, const Bloc
     const uint32_t x = blockIdx.x % AG_NXCD;
     const uint32_t step = gridDim.x / AG_NXCD;   // the grid is a multiple of AG_NXCD
     const uint32_t end = B.item_off[x + 1];
-    for (uint32_t i = B.item_off[x] + blockIdx.x / AG_NXCD; i < end; i += step) {
+    for (uint32_t i = B.item_lo[x] + blockIdx.x / AG_NXCD; i < end; i += step) {
         const uint32_t first = B.item[2 * i], kc = B.item[2 * i + 1];
         if ((kc & 15u) == AG_ITEM_REG) {
             pull_one_item(U, upol, first, red, s_vals, s_wave);
@@ -420,6 +451,98 @@ __device__ __forceinline__ void pull_blocked_items(const PullGeom &U, const Bloc
             stage_edges(B.V.src, e0, B.V.off[last] - e0, pp, s_vals);
             __syncthreads();
             mixed_reduce(B.V.off, e0, first, last, pp, red, s_vals, L.lng, &L.nlong);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- hot source segments
+
+// LDS of the hot-segment table: 2^AG_HOT_SEG_BITS contributions of at most 4 bytes.
+#define AG_HOT_TAB_BYTES ((1u << AG_HOT_SEG_BITS) * 4u)
+static_assert(AG_HOT_NNZ <= AG_TILE_NNZ_MAX, "a hot tile is an ordinary tile too");
+static_assert(AG_HOT_NNZ / (AG_LANE_MAX + 1u) < AG_LONG_MAX,
+              "the long rows of a full hot tile must fit the tile's LDS list");
+
+typedef uint32_t ag_u32x4 __attribute__((ext_vector_type(4)));
+
+// stage_edges with the contributions read from the LDS copy `tab` of sources
+// [base, base + 2^hot_bits): every source of the tile lies in that segment. The source ids
+// (nnz <= AG_HOT_NNZ) arrive in 16-byte loads from the 16-byte boundary at or below e0, all
+// of a lane's loads in flight at once; ids outside [e0, e0 + nnz) are dropped, and the last
+// group of four is read id by id when it runs past the tile (it may run past the array).
+template <class P, typename T>
+__device__ __forceinline__ void stage_edges_tab(const uint32_t *src, uint32_t e0, uint32_t nnz,
+                                                const P &pol, const T *tab, uint32_t base,
+                                                typename P::Acc *s_vals)
+{
+    constexpr uint32_t NV = AG_HOT_NNZ / 4u / AG_WG + 1u;   // 4-id groups per lane, at most
+    const uint32_t tid = threadIdx.x;
+    const uint32_t a0 = e0 & ~3u, e1 = e0 + nnz;
+    const uint32_t ngrp = (e1 - a0 + 3u) / 4u;
+    ag_u32x4 q[NV];
+#pragma unroll
+    for (uint32_t u = 0; u < NV; u++) {
+        const uint32_t v = tid + u * AG_WG, a = a0 + 4u * v;
+        q[u] = ag_u32x4{base, base, base, base};
+        if (v < ngrp) {
+            if (a + 4u <= e1) {
+                q[u] = __builtin_nontemporal_load((const ag_u32x4 *)(src + a));
+            } else {
+                for (uint32_t t = 0; t < 4u; t++)
+                    if (a + t < e1)
+                        q[u][t] = ld_stream(&src[a + t]);
+            }
+        }
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < NV; u++) {
+        const uint32_t a = a0 + 4u * (tid + u * AG_WG);
+#pragma unroll
+        for (uint32_t t = 0; t < 4u; t++) {
+            const uint32_t e = a + t;
+            if (e >= e0 && e < e1)
+                s_vals[e - e0] = pol.edge_val(e, tab[q[u][t] - base]);
+        }
+    }
+}
+
+// Hot pass of a source-blocked iteration (before pass 1, which then skips these tiles).
+// Workgroup b serves slice b % AG_NXCD; per group it copies the group's segment of the
+// contribution vector `cin` into LDS with coalesced 16-byte loads, then runs the group's
+// tiles as pass 1 does, every gather an LDS read. Partials only: no row is finished here.
+template <class P, typename T>
+__device__ __forceinline__ void pull_hot_items(const BlockGeom &B, const P &pol, const T *cin,
+                                               uint4 *s_tab, typename P::Acc *s_vals,
+                                               uint32_t *s_long, uint32_t *s_nlong)
+{
+    static_assert(sizeof(T) <= 4 && 16 % sizeof(T) == 0,
+                  "hot segments hold contributions of 1, 2 or 4 bytes");
+    using Acc = typename P::Acc;
+    const PartPol<P> pp{pol, (Acc *)B.part};
+    Red red{};
+    T *tab = (T *)s_tab;
+    const uint32_t x = blockIdx.x % AG_NXCD, step = gridDim.x / AG_NXCD, tid = threadIdx.x;
+    const uint32_t end = B.hgrp_off[x + 1];
+    for (uint32_t gi = B.hgrp_off[x] + blockIdx.x / AG_NXCD; gi < end; gi += step) {
+        const uint32_t i0 = B.hgrp[3 * gi], ni = B.hgrp[3 * gi + 1];
+        const uint32_t base = B.hgrp[3 * gi + 2] << B.hot_bits;
+        const uint32_t room = B.n_src - base;
+        const uint32_t cnt = room < (1u << B.hot_bits) ? room : (1u << B.hot_bits);
+        // base is a multiple of 2^hot_bits >= 256 sources: 16-byte aligned
+        const uint32_t vec = cnt / (16u / (uint32_t)sizeof(T));
+        const uint4 *g4 = (const uint4 *)(cin + base);
+        for (uint32_t i = tid; i < vec; i += AG_WG)
+            s_tab[i] = g4[i];
+        for (uint32_t i = vec * (16u / (uint32_t)sizeof(T)) + tid; i < cnt; i += AG_WG)
+            tab[i] = cin[base + i];
+        __syncthreads();
+        for (uint32_t it = i0; it < i0 + ni; it++) {
+            const uint32_t first = B.item[2 * it], last = first + (B.item[2 * it + 1] >> 4);
+            const uint32_t e0 = B.V.off[first];
+            stage_edges_tab(B.V.src, e0, B.V.off[last] - e0, pol, (const T *)tab, base, s_vals);
+            __syncthreads();
+            // ends with a barrier: s_vals, s_long and (after the last tile) the table are free
+            mixed_reduce(B.V.off, e0, first, last, pp, red, s_vals, s_long, s_nlong);
         }
     }
 }

@@ -204,10 +204,12 @@ extern "C" int ag_graph_free(ag_dgraph *g)
         return AG_OK;
     (void)hipSetDevice(g->dev->ordinal);
     (void)hipStreamSynchronize(g->dev->stream);
+    ag_ppr_free_ws(g);
     void *ptrs[] = {g->d2o, g->off, g->src, g->w, g->wq, g->outdeg, g->tile_row, g->seg_row,
                     g->seg_lo, g->seg_hi, g->hub_seg0, g->hub_part, g->vec_a, g->vec_b,
                     g->blk.voff, g->blk.vsrc, g->blk.vw, g->blk.item, g->blk.coff, g->blk.cidx,
                     g->blk.lpos, g->blk.ct_row, g->blk.ct_start, g->blk.ct_pre, g->blk.c_item,
+                    g->blk.hgrp,
                     g->vpart, g->vwq, g->fwd_off, g->fwd_src, g->fwd_dst,
                     g->fwd_w, g->pb_row, g->pb_pos, g->pb_dst};
     for (void *p : ptrs)
@@ -233,6 +235,7 @@ PullGeom ag_make_geom(const ag_dgraph *g)
     G.hub_part = g->hub_part;
     G.n = g->n;
     G.n_hubs = g->n_hubs;
+    G.seq_hubs = g->params.fp32_order == AG_ORDER_REFERENCE ? 1u : 0u;
     for (int b = 0; b <= AG_NBINS; b++)
         G.bin_row[b] = g->bin_row[b];
     for (int s = 0; s <= AG_NSLOTS; s++)
@@ -272,8 +275,20 @@ BlockGeom ag_make_block_geom(const ag_dgraph *g)
     for (uint32_t x = 0; x <= AG_NXCD; x++) {
         K.item_off[x] = B.item_off[x];
         K.c_item_off[x] = B.c_item_off[x];
+        K.hgrp_off[x] = B.hgrp_off[x];
     }
+    for (uint32_t x = 0; x < AG_NXCD; x++)
+        K.item_lo[x] = B.item_off[x];
+    K.hgrp = B.hgrp;
+    K.hot_bits = B.hot_bits;
+    K.n_src = g->n;
     return K;
+}
+
+void ag_block_geom_skip_hot(const ag_dgraph *g, BlockGeom &K)
+{
+    for (uint32_t x = 0; x < AG_NXCD; x++)
+        K.item_lo[x] = g->blk.hot_end[x];
 }
 
 // Resident four-wave workgroups per CU: the plan's wg_per_cu, by default 4 (16 waves per
@@ -395,6 +410,8 @@ extern "C" int ag_graph_upload(ag_device *dev, uint32_t n, uint32_t m, const uin
         g->blk.ct_row = g->blk.ct_start = g->blk.ct_pre = g->blk.c_item = NULL;
         g->blk.lpos = NULL;
         g->blk.vw = NULL;
+        g->blk.hgrp = NULL;
+        g->blk.hot_seg = NULL;   // host-only array of the plan, freed with it
         const size_t nt = B.c_n_tiles, nb = B.nblocks;
         st = dev_upload(g, &g->blk.voff, B.voff, (size_t)B.nv + 1);
         if (st == AG_OK) st = dev_upload(g, &g->blk.vsrc, B.vsrc, B.mb);
@@ -407,6 +424,7 @@ extern "C" int ag_graph_upload(ag_device *dev, uint32_t n, uint32_t m, const uin
         if (st == AG_OK) st = dev_upload(g, &g->blk.ct_start, B.ct_start, nt * nb);
         if (st == AG_OK) st = dev_upload(g, &g->blk.ct_pre, B.ct_pre, nt * (nb + 1));
         if (st == AG_OK) st = dev_upload(g, &g->blk.c_item, B.c_item, (size_t)B.c_n_items * 2);
+        if (st == AG_OK && B.n_hgrp) st = dev_upload(g, &g->blk.hgrp, B.hgrp, (size_t)B.n_hgrp * 3);
         if (st == AG_OK) {
             size_t pb = (size_t)(B.nv ? B.nv : 1) * sizeof(uint64_t);
             he = hipMalloc((void **)&g->vpart, pb);
@@ -484,6 +502,9 @@ extern "C" int ag_graph_get_info(const ag_dgraph *g, ag_graph_info *info)
     info->src_blocks = g->blk.nblocks;
     info->virtual_rows = g->blk.nv;
     info->blocked_row0 = g->blk.bits ? g->blk.row0 : g->n;
+    info->hot_segments = g->blk.n_hot;
+    info->hot_segment_bits = g->blk.hot_bits;
+    info->hot_edges = g->blk.hot_edges;
     return AG_OK;
 }
 

@@ -31,10 +31,27 @@ struct SpmvF32Pol {
     float *y;           // device order
     __device__ __forceinline__ float edge(uint32_t j, uint32_t s) const
     {
-        return __fmul_rn(x[s], WEIGHTED ? __builtin_nontemporal_load(&w[j]) : 0.0001f);
+        return edge_val(j, x[s]);
+    }
+    // the same term from x[s] already in hand (hot-segment LDS copy)
+    __device__ __forceinline__ float edge_val(uint32_t j, float xs) const
+    {
+        return __fmul_rn(xs, WEIGHTED ? __builtin_nontemporal_load(&w[j]) : 0.0001f);
     }
     __device__ __forceinline__ void finish(uint32_t r, float sum, Red &) const { y[r] = sum; }
 };
+
+// Hot pass of the FP32 SpMV (see pull_hot_items): x gathered from LDS copies of its hot
+// segments, per-edge weights in virtual-row order.
+template <bool WEIGHTED>
+__global__ __launch_bounds__(AG_WG) void spmv_hot_kernel(BlockGeom B, SpmvF32Pol<WEIGHTED> pol)
+{
+    __shared__ uint4 s_tab[AG_HOT_TAB_BYTES / 16];
+    __shared__ float s_vals[AG_HOT_NNZ];
+    __shared__ uint32_t s_long[AG_LONG_MAX];
+    __shared__ uint32_t s_nlong;
+    pull_hot_items(B, pol, pol.x, s_tab, s_vals, s_long, &s_nlong);
+}
 
 template <bool WEIGHTED>
 struct SpmvFixPol {
@@ -177,8 +194,27 @@ static int run_pull(ag_dgraph *g, const P &pol, uint32_t reps, double *avg_ms)
     return st;
 }
 
+// The hot pass of a blocked SpMV: FP32 only (the fixed-point form gathers everything in
+// pass 1). Returns whether it was launched.
+template <class P>
+static bool spmv_hot_launch(const ag_dgraph *, const BlockGeom &, const P &, hipStream_t)
+{
+    return false;
+}
+template <bool WEIGHTED>
+static bool spmv_hot_launch(const ag_dgraph *g, const BlockGeom &B,
+                            const SpmvF32Pol<WEIGHTED> &pv, hipStream_t stream)
+{
+    if (!g->blk.n_hgrp)
+        return false;
+    static const uint32_t res = ag_resident_per_cu(spmv_hot_kernel<WEIGHTED>);
+    spmv_hot_kernel<WEIGHTED><<<ag_blocked_grid(g, res, 2u), AG_WG, 0, stream>>>(B, pv);
+    return true;
+}
+
 // Source-blocked form of run_pull: `pv` reads per-edge data in virtual-row order, `pu` in
-// regular order (unblocked rows); pass 1 then the combine pass per repetition.
+// regular order (unblocked rows); the hot pass (FP32), pass 1 then the combine pass per
+// repetition.
 template <class P>
 static int run_pull_blocked(ag_dgraph *g, const P &pv, const P &pu, uint32_t reps,
                             double *avg_ms)
@@ -186,6 +222,7 @@ static int run_pull_blocked(ag_dgraph *g, const P &pv, const P &pu, uint32_t rep
     hipStream_t stream = g->dev->stream;
     const PullGeom U = ag_make_unblocked_geom(g);
     const BlockGeom B = ag_make_block_geom(g);
+    BlockGeom B1 = B;   // pass 1: past the hot tiles when the hot pass gathers them
     static const uint32_t res1 = ag_resident_per_cu(pull_blocked_kernel<P, P>);
     static const uint32_t res2 = ag_resident_per_cu(pull_combine_kernel<P>);
     const uint32_t g1 = ag_blocked_grid(g, res1, 4u), g2 = ag_blocked_grid(g, res2, 8u);
@@ -196,7 +233,9 @@ static int run_pull_blocked(ag_dgraph *g, const P &pv, const P &pu, uint32_t rep
         AG_HIP(hipEventRecord(e0, stream));
     }
     for (uint32_t i = 0; i < reps; i++) {
-        pull_blocked_kernel<P, P><<<g1, AG_WG, 0, stream>>>(U, B, pv, pu);
+        if (spmv_hot_launch(g, B, pv, stream))
+            ag_block_geom_skip_hot(g, B1);
+        pull_blocked_kernel<P, P><<<g1, AG_WG, 0, stream>>>(U, B1, pv, pu);
         pull_combine_kernel<P><<<g2, AG_WG, 0, stream>>>(B, pu);
     }
     hipError_t e = hipGetLastError();
@@ -218,6 +257,38 @@ static int run_pull_blocked(ag_dgraph *g, const P &pv, const P &pu, uint32_t rep
     return st;
 }
 
+// One FP32 SpMV on device buffers: x (host, original order) -> xo -> permuted into xd (device
+// order) -> y in device order into yd -> permuted back into xo -> y (host). yd may be xo (x
+// in original order is dead once permuted), never xd.
+static int spmv_f32_device(ag_dgraph *g, const float *x, float *y, float *xo, float *xd,
+                           float *yd, uint32_t reps, double *avg_ms)
+{
+    const uint32_t n = g->n;
+    hipStream_t stream = g->dev->stream;
+    const uint32_t blk = (n + AG_WG - 1) / AG_WG;
+    AG_HIP(hipMemcpyAsync(xo, x, (size_t)n * sizeof(float), hipMemcpyHostToDevice, stream));
+    permute_in_kernel<float><<<blk, AG_WG, 0, stream>>>(n, xo, g->d2o, xd);
+    AG_HIP(hipGetLastError());
+    int st;
+    if (g->blk.bits && g->w)
+        st = run_pull_blocked(g, SpmvF32Pol<true>{xd, g->blk.vw, yd},
+                              SpmvF32Pol<true>{xd, g->w, yd}, reps, avg_ms);
+    else if (g->blk.bits)
+        st = run_pull_blocked(g, SpmvF32Pol<false>{xd, NULL, yd}, SpmvF32Pol<false>{xd, NULL, yd},
+                              reps, avg_ms);
+    else if (g->w)
+        st = run_pull(g, SpmvF32Pol<true>{xd, g->w, yd}, reps, avg_ms);
+    else
+        st = run_pull(g, SpmvF32Pol<false>{xd, NULL, yd}, reps, avg_ms);
+    if (st != AG_OK)
+        return st;
+    float *yo = (yd == xo) ? xd : xo;   // y in original order
+    permute_out_kernel<float><<<blk, AG_WG, 0, stream>>>(n, yd, g->d2o, yo);
+    AG_HIP(hipGetLastError());
+    AG_HIP(hipMemcpyAsync(y, yo, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, stream));
+    return ag_wait_stream(g->dev);
+}
+
 extern "C" int ag_spmv_pull(ag_dgraph *g, const float *x, float *y, uint32_t reps, double *avg_ms)
 {
     if (!g || (g->n && (!x || !y)))
@@ -233,30 +304,10 @@ extern "C" int ag_spmv_pull(ag_dgraph *g, const float *x, float *y, uint32_t rep
     int st = ag_ensure_vec(g, (size_t)n * sizeof(uint64_t));
     if (st != AG_OK)
         return st;
-    hipStream_t stream = g->dev->stream;
     float *xo = (float *)g->vec_a;                    // x, original order (then y original)
     float *xd = (float *)g->vec_b;                    // x, device order
     float *yd = (float *)((char *)g->vec_b + (size_t)n * sizeof(float));  // y, device order
-    const uint32_t blk = (n + AG_WG - 1) / AG_WG;
-    AG_HIP(hipMemcpyAsync(xo, x, (size_t)n * sizeof(float), hipMemcpyHostToDevice, stream));
-    permute_in_kernel<float><<<blk, AG_WG, 0, stream>>>(n, xo, g->d2o, xd);
-    AG_HIP(hipGetLastError());
-    if (g->blk.bits && g->w)
-        st = run_pull_blocked(g, SpmvF32Pol<true>{xd, g->blk.vw, yd},
-                              SpmvF32Pol<true>{xd, g->w, yd}, reps, avg_ms);
-    else if (g->blk.bits)
-        st = run_pull_blocked(g, SpmvF32Pol<false>{xd, NULL, yd}, SpmvF32Pol<false>{xd, NULL, yd},
-                              reps, avg_ms);
-    else if (g->w)
-        st = run_pull(g, SpmvF32Pol<true>{xd, g->w, yd}, reps, avg_ms);
-    else
-        st = run_pull(g, SpmvF32Pol<false>{xd, NULL, yd}, reps, avg_ms);
-    if (st != AG_OK)
-        return st;
-    permute_out_kernel<float><<<blk, AG_WG, 0, stream>>>(n, yd, g->d2o, xo);
-    AG_HIP(hipGetLastError());
-    AG_HIP(hipMemcpyAsync(y, xo, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, stream));
-    return ag_wait_stream(g->dev);
+    return spmv_f32_device(g, x, y, xo, xd, yd, reps, avg_ms);
 }
 
 static uint32_t host_fixed32(float x)

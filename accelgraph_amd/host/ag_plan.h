@@ -47,6 +47,18 @@ extern const uint32_t ag_slot_bin[AG_NSLOTS];
 #define AG_AUTO_BLOCK_BITS 19u
 #define AG_AUTO_BLOCK_MIN_N (1u << 21)
 
+/* Hot source segments: 2^AG_HOT_SEG_BITS consecutive device ids (fewer when the block bits
+ * are fewer) whose in-edges are dense enough that a workgroup copies the segment's
+ * contributions into LDS and gathers from there. Each hot segment is a source block of its
+ * own (block ids [0, n_hot), in segment order; the ordinary blocks follow), its pieces are
+ * cut at AG_HOT_NNZ edges, its pass-1 tiles hold at most AG_HOT_NNZ edges, and consecutive
+ * tiles of one (slice, segment) form groups of at most AG_HOT_GROUP_NNZ edges, one LDS copy
+ * of the segment each. */
+#define AG_HOT_SEG_BITS 14u
+#define AG_HOT_NNZ 2048u
+#define AG_HOT_GROUP_NNZ 65536u
+#define AG_HOT_DEFAULT 128u
+
 /* work items of the blocked passes: item = {first, kind | count << 4} */
 enum { AG_ITEM_REG = 0,     /* pass 1: first = item of the unblocked rows' regular schedule */
        AG_ITEM_VTILE = 1,   /* pass 1: virtual rows [first, first + count) as one LDS tile  */
@@ -79,6 +91,17 @@ typedef struct ag_block_plan {
     uint32_t c_item_off[AG_NXCD + 1];
     /* regular schedule restricted to the unblocked rows (the AG_ITEM_REG items)         */
     uint32_t r_slot_blk0[AG_NSLOTS + 1];
+    /* hot segments: the first items of slice x, [item_off[x], hot_end[x]), are the tiles of
+     * its hot blocks; hgrp[3g .. 3g+2] = {first item, item count, segment} of group g, the
+     * groups of slice x are [hgrp_off[x], hgrp_off[x+1])                                  */
+    uint32_t hot_bits;      /* sources per hot segment = 2^hot_bits (0: no hot segments)  */
+    uint32_t n_hot;
+    uint64_t hot_edges;
+    uint32_t *hot_seg;      /* [n_hot] segment (device id >> hot_bits) of hot block b     */
+    uint32_t hot_end[AG_NXCD];
+    uint32_t n_hgrp;
+    uint32_t *hgrp;         /* [3*n_hgrp]                                                */
+    uint32_t hgrp_off[AG_NXCD + 1];
 } ag_block_plan;
 
 typedef struct ag_plan {

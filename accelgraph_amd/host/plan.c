@@ -27,6 +27,8 @@ void ag_plan_params_default(ag_plan_params *p)
     p->src_block_bits = AG_SRC_BLOCK_AUTO;
     p->wg_per_cu = 0;
     p->blocked_min_bin = AG_BIN_THREAD;
+    p->hot_segments = AG_HOT_DEFAULT;
+    p->fp32_order = AG_ORDER_FAST;
 }
 
 int ag_plan_params_check(const ag_plan_params *p, const char **err)
@@ -69,6 +71,14 @@ int ag_plan_params_check(const ag_plan_params *p, const char **err)
         *err = "order must be 0 (vertex id) or 1 (out-degree descending)";
         return AG_ERR_INVALID;
     }
+    if (p->hot_segments > AG_MAX_SRC_BLOCKS) {
+        *err = "hot_segments must be in [0, 256]";
+        return AG_ERR_INVALID;
+    }
+    if (p->fp32_order > AG_ORDER_REFERENCE) {
+        *err = "fp32_order must be AG_ORDER_FAST or AG_ORDER_REFERENCE";
+        return AG_ERR_INVALID;
+    }
     return AG_OK;
 }
 
@@ -98,6 +108,8 @@ void ag_plan_free(ag_plan *p)
     free(p->blk.ct_start);
     free(p->blk.ct_pre);
     free(p->blk.c_item);
+    free(p->blk.hot_seg);
+    free(p->blk.hgrp);
     memset(p, 0, sizeof(*p));
 }
 
@@ -173,16 +185,29 @@ static uint32_t block_bits_for(const ag_plan *p, const char **err)
     return b;
 }
 
+/* Source block of device id s: kmap[s >> kbits] (hot segments first, then the ordinary
+ * blocks; see build_block_plan). */
+typedef struct {
+    const uint32_t *kmap;
+    uint32_t kbits;
+    uint32_t n_hot;         /* blocks below this are hot segments */
+    uint32_t piece;         /* piece limit of an ordinary block */
+    uint32_t hot_piece;     /* piece limit of a hot segment */
+} blk_key;
+
+static inline uint32_t key_of(const blk_key *K, uint32_t s) { return K->kmap[s >> K->kbits]; }
+
 /* Stable sort of one row's in-edges by source block into out_s/out_w. keys: scratch of d. */
-static void sort_row_by_block(const uint32_t *src, const float *w, uint32_t d, uint32_t bits,
+static void sort_row_by_block(const uint32_t *src, const float *w, uint32_t d, const blk_key *K,
                               uint32_t *out_s, float *out_w, uint64_t *keys)
 {
     if (d <= 32) {
         for (uint32_t i = 0; i < d; i++) {
             uint32_t s = src[i];
             float x = w ? w[i] : 0.0f;
+            const uint32_t ks = key_of(K, s);
             uint32_t j = i;
-            while (j > 0 && (out_s[j - 1] >> bits) > (s >> bits)) {
+            while (j > 0 && key_of(K, out_s[j - 1]) > ks) {
                 out_s[j] = out_s[j - 1];
                 if (w)
                     out_w[j] = out_w[j - 1];
@@ -195,7 +220,7 @@ static void sort_row_by_block(const uint32_t *src, const float *w, uint32_t d, u
         return;
     }
     for (uint32_t i = 0; i < d; i++)
-        keys[i] = ((uint64_t)(src[i] >> bits) << 32) | i;
+        keys[i] = ((uint64_t)key_of(K, src[i]) << 32) | i;
     qsort(keys, d, sizeof(uint64_t), cmp_u64);
     for (uint32_t i = 0; i < d; i++) {
         uint32_t k = (uint32_t)keys[i];
@@ -206,16 +231,17 @@ static void sort_row_by_block(const uint32_t *src, const float *w, uint32_t d, u
 }
 
 /* The virtual rows of a block-sorted row: one per run of equal source block, split into
- * pieces of at most `piece` edges. With st != NULL records start, length and block of each.
- * Returns the count. */
-static uint32_t row_pieces(const uint32_t *ss, uint32_t d, uint32_t bits, uint32_t piece,
-                           uint32_t *st, uint32_t *len, uint32_t *blk)
+ * pieces of at most the block's piece limit. With st != NULL records start, length and block
+ * of each. Returns the count. */
+static uint32_t row_pieces(const uint32_t *ss, uint32_t d, const blk_key *K, uint32_t *st,
+                           uint32_t *len, uint32_t *blk)
 {
     uint32_t cnt = 0, i = 0;
     while (i < d) {
-        uint32_t b = ss[i] >> bits, j = i + 1;
-        while (j < d && (ss[j] >> bits) == b)
+        uint32_t b = key_of(K, ss[i]), j = i + 1;
+        while (j < d && key_of(K, ss[j]) == b)
             j++;
+        const uint32_t piece = b < K->n_hot ? K->hot_piece : K->piece;
         for (uint32_t a = i; a < j; a += piece) {
             if (st) {
                 st[cnt] = a;
@@ -239,36 +265,72 @@ static uint32_t row_pieces(const uint32_t *ss, uint32_t d, uint32_t bits, uint32
     } while (0)
 
 /* Pass-1 items of every slice: per block, greedy tiles of consecutive virtual rows (at most
- * tile_nnz edges, so at most that many virtual rows), then the slice's share of the unblocked
- * rows' items. item == NULL only counts them. */
+ * tile_nnz edges, so at most that many virtual rows; at most hot_nnz in a hot segment), then
+ * the slice's share of the unblocked rows' items. The hot segments' tiles come first; runs
+ * of them of one segment, at most AG_HOT_GROUP_NNZ edges, form the groups hgrp (one LDS copy
+ * of the segment each). item == NULL only counts them (and the groups, into *n_hgrp). */
 static uint32_t emit_items(const ag_plan *p, const uint32_t *kstart, const uint32_t *vlen,
-                           uint32_t reg_total, uint32_t *item, uint32_t *item_off)
+                           uint32_t reg_total, uint32_t hot_nnz, uint32_t *item,
+                           uint32_t *item_off, uint32_t *hgrp, uint32_t *n_hgrp)
 {
-    const uint32_t nb = p->blk.nblocks;
-    uint32_t k = 0;
+    const ag_block_plan *B = &p->blk;
+    const uint32_t nb = B->nblocks, H = B->n_hot;
+    uint32_t k = 0, ng = 0;
     for (uint32_t x = 0; x < AG_NXCD; x++) {
-        if (item_off)
+        if (item_off) {
             item_off[x] = k;
+            ((ag_block_plan *)B)->hgrp_off[x] = ng;
+        }
         for (uint32_t b = 0; b < nb; b++) {
+            if (b == H && item_off)
+                ((ag_block_plan *)B)->hot_end[x] = k;
+            const uint32_t lim = b < H ? hot_nnz : p->params.tile_nnz;
             uint32_t v = kstart[x * nb + b];
             const uint32_t e = kstart[x * nb + b + 1];
+            uint32_t gsz = 0, gitems = 0;
             while (v < e) {
                 uint32_t t0 = v, rows = 0, nnz = 0;
                 while (v < e) {
-                    if (rows > 0 && nnz + vlen[v] > p->params.tile_nnz)
+                    if (rows > 0 && nnz + vlen[v] > lim)
                         break;
                     nnz += vlen[v];
                     rows++;
                     v++;
                 }
+                if (b < H) {
+                    if (gitems > 0 && gsz + nnz > AG_HOT_GROUP_NNZ) {
+                        if (hgrp) {
+                            hgrp[3 * ng] = k - gitems;
+                            hgrp[3 * ng + 1] = gitems;
+                            hgrp[3 * ng + 2] = B->hot_seg[b];
+                        }
+                        ng++;
+                        gsz = gitems = 0;
+                    }
+                    gsz += nnz;
+                    gitems++;
+                }
                 AG_EMIT(item, k, t0, AG_ITEM_VTILE, rows);
             }
+            if (gitems > 0) {
+                if (hgrp) {
+                    hgrp[3 * ng] = k - gitems;
+                    hgrp[3 * ng + 1] = gitems;
+                    hgrp[3 * ng + 2] = B->hot_seg[b];
+                }
+                ng++;
+            }
         }
+        if (H == nb && item_off)
+            ((ag_block_plan *)B)->hot_end[x] = k;
         for (uint32_t r = x; r < reg_total; r += AG_NXCD)
             AG_EMIT(item, k, r, AG_ITEM_REG, 0);
     }
-    if (item_off)
+    if (item_off) {
         item_off[AG_NXCD] = k;
+        ((ag_block_plan *)B)->hgrp_off[AG_NXCD] = ng;
+    }
+    *n_hgrp = ng;
     return k;
 }
 
@@ -329,6 +391,85 @@ static uint32_t lower_bound_u32(const uint32_t *key, uint32_t lo, uint32_t hi, u
     return lo;
 }
 
+/* Hot segments among the sources of the blocked rows [row0, n): for every segment of 2^kb
+ * device ids, its in-edges E and its pieces P (rows with at least one in-edge from it). A
+ * segment qualifies when its pieces average at least two edges (2P <= E: staging it in LDS
+ * turns at least half of its gathers into LDS reads) and it has at least four in-edges per
+ * source (the LDS copy of the segment costs less than the gathers it serves). The hmax
+ * qualifying segments with the most in-edges are numbered 0..H-1 in segment order in kmap;
+ * every other entry of kmap is set to UINT32_MAX. */
+static int select_hot_segments(const ag_plan *p, uint32_t row0, uint32_t kb, uint32_t nseg,
+                               uint32_t hmax, uint32_t *kmap, uint32_t *H_out,
+                               uint64_t *hot_edges)
+{
+    const uint32_t n = p->n;
+    for (uint32_t sg = 0; sg < nseg; sg++)
+        kmap[sg] = UINT32_MAX;
+    *H_out = 0;
+    *hot_edges = 0;
+    if (hmax == 0)
+        return AG_OK;
+    const int nt = omp_get_max_threads() > 0 ? omp_get_max_threads() : 1;
+    uint64_t *E = (uint64_t *)zalloc((size_t)nt * nseg, sizeof(uint64_t));
+    uint32_t *P = (uint32_t *)zalloc((size_t)nt * nseg, sizeof(uint32_t));
+    uint32_t *stamp = (uint32_t *)malloc((size_t)nt * nseg * sizeof(uint32_t));
+    uint64_t *cand = (uint64_t *)malloc((size_t)nseg * sizeof(uint64_t));
+    if (!E || !P || !stamp || !cand) {
+        free(E);
+        free(P);
+        free(stamp);
+        free(cand);
+        return AG_ERR_OOM;
+    }
+    memset(stamp, 0xFF, (size_t)nt * nseg * sizeof(uint32_t));
+    #pragma omp parallel num_threads(nt)
+    {
+        const int t = omp_get_thread_num();
+        uint64_t *e = E + (size_t)t * nseg;
+        uint32_t *pc = P + (size_t)t * nseg, *sp = stamp + (size_t)t * nseg;
+        #pragma omp for schedule(dynamic, 1024)
+        for (uint32_t r = row0; r < n; r++) {
+            for (uint32_t j = p->off[r]; j < p->off[r + 1]; j++) {
+                const uint32_t sg = p->src[j] >> kb;
+                e[sg]++;
+                if (sp[sg] != r) {
+                    sp[sg] = r;
+                    pc[sg]++;
+                }
+            }
+        }
+    }
+    uint32_t nc = 0;
+    for (uint32_t sg = 0; sg < nseg; sg++) {
+        uint64_t es = 0, ps = 0;
+        for (int t = 0; t < nt; t++) {
+            es += E[(size_t)t * nseg + sg];
+            ps += P[(size_t)t * nseg + sg];
+        }
+        if (es >= (4ull << kb) && 2 * ps <= es && es < (1ull << 32))
+            cand[nc++] = (es << 32) | (uint64_t)(UINT32_MAX - sg);   /* most edges, then lowest id */
+    }
+    qsort(cand, nc, sizeof(uint64_t), cmp_u64);
+    const uint32_t H = nc < hmax ? nc : hmax;
+    uint64_t he = 0;
+    for (uint32_t i = 0; i < H; i++) {
+        const uint64_t c = cand[nc - 1 - i];
+        kmap[UINT32_MAX - (uint32_t)(c & 0xFFFFFFFFu)] = 0;   /* mark; numbered below */
+        he += c >> 32;
+    }
+    uint32_t id = 0;
+    for (uint32_t sg = 0; sg < nseg; sg++)
+        if (kmap[sg] == 0)
+            kmap[sg] = id++;
+    *H_out = H;
+    *hot_edges = he;
+    free(E);
+    free(P);
+    free(stamp);
+    free(cand);
+    return AG_OK;
+}
+
 static int build_block_plan(ag_plan *p, const char **err)
 {
     ag_block_plan *B = &p->blk;
@@ -346,16 +487,56 @@ static int build_block_plan(ag_plan *p, const char **err)
     if (row0 >= n || p->off[n] == p->off[row0])
         return AG_OK;
     const uint32_t e0 = p->off[row0], mb = p->off[n] - e0;
-    const uint32_t nb = (uint32_t)((((uint64_t)n) + (1ull << bits) - 1) >> bits);
-    const uint32_t nK = AG_NXCD * nb;
+    const uint32_t ncold = (uint32_t)((((uint64_t)n) + (1ull << bits) - 1) >> bits);
     const ag_plan_params *q = &p->params;
     const uint32_t piece = q->tile_nnz < AG_PIECE_NNZ ? q->tile_nnz : AG_PIECE_NNZ;
+    const uint32_t hot_nnz = q->tile_nnz < AG_HOT_NNZ ? q->tile_nnz : AG_HOT_NNZ;
     int oom = 0;
+
+    /* 0: source blocks. Segments of 2^kb device ids; the hot ones are blocks of their own,
+     * every other segment maps to its ordinary 2^bits block. */
+    const uint32_t kb = bits < AG_HOT_SEG_BITS ? bits : AG_HOT_SEG_BITS;
+    const uint32_t nseg = (uint32_t)((((uint64_t)n) + (1ull << kb) - 1) >> kb);
+    uint32_t *kmap = (uint32_t *)zalloc(nseg, sizeof(uint32_t));
+    if (!kmap) {
+        *err = "host allocation failed";
+        return AG_ERR_OOM;
+    }
+    uint32_t H = 0;
+    {
+        const uint32_t hmax = q->hot_segments < AG_MAX_SRC_BLOCKS - ncold ? q->hot_segments
+                                                                           : AG_MAX_SRC_BLOCKS - ncold;
+        int st = select_hot_segments(p, row0, kb, nseg, hmax, kmap, &H, &B->hot_edges);
+        if (st != AG_OK) {
+            free(kmap);
+            *err = "host allocation failed";
+            return st;
+        }
+    }
+    if (H) {
+        B->hot_seg = (uint32_t *)zalloc(H, sizeof(uint32_t));
+        if (!B->hot_seg) {
+            free(kmap);
+            *err = "host allocation failed";
+            return AG_ERR_OOM;
+        }
+        for (uint32_t sg = 0; sg < nseg; sg++)
+            if (kmap[sg] != UINT32_MAX)
+                B->hot_seg[kmap[sg]] = sg;
+    }
+    for (uint32_t sg = 0; sg < nseg; sg++)
+        if (kmap[sg] == UINT32_MAX)
+            kmap[sg] = H + (sg >> (bits - kb));
+    const uint32_t nb = H + ncold;
+    const uint32_t nK = AG_NXCD * nb;
+    const blk_key K = {kmap, kb, H, piece, hot_nnz};
 
     B->bits = bits;
     B->nblocks = nb;
     B->row0 = row0;
     B->mb = mb;
+    B->n_hot = H;
+    B->hot_bits = H ? kb : 0;
 
     uint32_t *ss = (uint32_t *)zalloc(mb, sizeof(uint32_t));
     float *sw = p->w ? (float *)zalloc(mb, sizeof(float)) : NULL;
@@ -365,6 +546,7 @@ static int build_block_plan(ag_plan *p, const char **err)
         free(ss);
         free(sw);
         free(nvr);
+        free(kmap);
         *err = "host allocation failed";
         return AG_ERR_OOM;
     }
@@ -387,9 +569,9 @@ static int build_block_plan(ag_plan *p, const char **err)
                     continue;
                 }
             }
-            sort_row_by_block(p->src + p->off[r], p->w ? p->w + p->off[r] : NULL, d, bits,
+            sort_row_by_block(p->src + p->off[r], p->w ? p->w + p->off[r] : NULL, d, &K,
                               ss + o, sw ? sw + o : NULL, keys);
-            nvr[r - row0] = row_pieces(ss + o, d, bits, piece, NULL, NULL, NULL);
+            nvr[r - row0] = row_pieces(ss + o, d, &K, NULL, NULL, NULL);
         }
         free(keys);
     }
@@ -397,6 +579,7 @@ static int build_block_plan(ag_plan *p, const char **err)
         free(ss);
         free(sw);
         free(nvr);
+        free(kmap);
         *err = "host allocation failed";
         return AG_ERR_OOM;
     }
@@ -459,7 +642,7 @@ static int build_block_plan(ag_plan *p, const char **err)
         uint32_t x = 0;
         while (x + 1 < AG_NXCD && r >= B->slice_row[x + 1])
             x++;
-        uint32_t cnt = row_pieces(ss + o, d, bits, piece, vst + c, vln + c, vkey + c);
+        uint32_t cnt = row_pieces(ss + o, d, &K, vst + c, vln + c, vkey + c);
         for (uint32_t j = 0; j < cnt; j++) {
             vkey[c + j] = x * nb + vkey[c + j];
             vst[c + j] += o;
@@ -533,14 +716,21 @@ static int build_block_plan(ag_plan *p, const char **err)
             B->r_slot_blk0[sl + 1] = B->r_slot_blk0[sl] + rb[ag_slot_bin[sl]];
     }
 
-    /* 4: pass-1 items, slice by slice, block-major */
-    B->n_items = emit_items(p, kstart, vlen, B->r_slot_blk0[AG_NSLOTS], NULL, NULL);
-    B->item = (uint32_t *)zalloc((size_t)B->n_items * 2, sizeof(uint32_t));
-    if (!B->item) {
-        oom = 1;
-        goto done;
+    /* 4: pass-1 items, slice by slice, block-major (hot segments first), and the groups of
+     * hot-segment tiles */
+    {
+        uint32_t ng = 0;
+        B->n_items = emit_items(p, kstart, vlen, B->r_slot_blk0[AG_NSLOTS], hot_nnz, NULL, NULL,
+                                NULL, &ng);
+        B->item = (uint32_t *)zalloc((size_t)B->n_items * 2, sizeof(uint32_t));
+        B->hgrp = (uint32_t *)zalloc((size_t)(ng ? ng : 1) * 3, sizeof(uint32_t));
+        if (!B->item || !B->hgrp) {
+            oom = 1;
+            goto done;
+        }
+        emit_items(p, kstart, vlen, B->r_slot_blk0[AG_NSLOTS], hot_nnz, B->item, B->item_off,
+                   B->hgrp, &B->n_hgrp);
     }
-    emit_items(p, kstart, vlen, B->r_slot_blk0[AG_NSLOTS], B->item, B->item_off);
 
     /* 5: combine tiles: per block, the tile's rows' virtual rows are one contiguous run of
      * the (slice, block) list; lpos places each partial at its row-major slot in the tile */
@@ -581,6 +771,7 @@ static int build_block_plan(ag_plan *p, const char **err)
     }
 
 done:
+    free(kmap);
     free(ss);
     free(sw);
     free(vkey);
@@ -611,6 +802,13 @@ int ag_plan_build(uint32_t n, uint32_t m, const uint32_t *out_degree,
     int st = ag_plan_params_check(&p->params, err);
     if (st != AG_OK)
         return st;
+    if (p->params.fp32_order == AG_ORDER_REFERENCE) {
+        /* sequential sums only: rows that fit a tile are tile rows (one lane, from LDS), every
+         * longer row is one hub segment (one lane over chunks staged in LDS); no blocking */
+        ag_plan_params *q = &p->params;
+        q->thread_max_deg = q->group16_max_deg = q->wave_max_deg = q->block_max_deg = q->tile_nnz;
+        q->src_block_bits = AG_SRC_BLOCK_OFF;
+    }
     /* keeps n + 1, word-rounded mark counts and grid sizes inside 32 bits */
     if (n > 0xFFFF0000u) {
         *err = "num_vertices exceeds 2^32 - 2^16";
@@ -750,7 +948,9 @@ int ag_plan_build(uint32_t n, uint32_t m, const uint32_t *out_degree,
     /* hub segments */
     {
         uint32_t lo = p->bin_row[AG_BIN_HUB], hi = p->bin_row[AG_BIN_HUB + 1];
-        uint32_t seg = p->params.segment_nnz;
+        /* reference order: one segment per hub row, added in sequence */
+        uint32_t seg = p->params.fp32_order == AG_ORDER_REFERENCE ? UINT32_MAX
+                                                                  : p->params.segment_nnz;
         uint64_t nsegs = 0;
         for (uint32_t r = lo; r < hi; r++)
             nsegs += ((uint64_t)(p->off[r + 1] - p->off[r]) + seg - 1) / seg;
@@ -768,10 +968,11 @@ int ag_plan_build(uint32_t n, uint32_t m, const uint32_t *out_degree,
         uint32_t s = 0;
         for (uint32_t r = lo; r < hi; r++) {
             p->hub_seg0[r - lo] = s;
-            for (uint32_t a = p->off[r]; a < p->off[r + 1]; a += seg) {
+            for (uint32_t a = p->off[r]; a < p->off[r + 1];) {
                 p->seg_row[s] = r;
                 p->seg_lo[s] = a;
                 p->seg_hi[s] = (p->off[r + 1] - a > seg) ? a + seg : p->off[r + 1];
+                a = p->seg_hi[s];
                 s++;
             }
         }

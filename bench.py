@@ -48,6 +48,11 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def _tmpdir() -> str:
+    import tempfile
+    return tempfile.mkdtemp(prefix="ag_bench_")
+
+
 def algorithmic_bytes(n: int, m: int, kind: str) -> int:
     # inv offsets 4(N+1), source ids 4M, one contribution per edge, out-degree 4N,
     # rank read 4N, rank write 4N, contribution write 4N
@@ -186,24 +191,36 @@ def time_to_l1(ag, dg, g, f64) -> dict:
     return out
 
 
-def warm_section(ag, dev, g, f64) -> dict:
-    """Learned warm start on the bench graph (held out: the network was trained on graphs of
-    scale 14-18), time-to-L1 <= 1e-6 including features and inference, vs the uniform start."""
+def warm_section(ag, dev, g, f64, cpu: bool) -> dict:
+    """Learned warm start on the bench graph and on the held-out RMAT-22 and RMAT-24 graphs
+    (the network was trained on graphs of scale 14-18), time-to-L1 <= 1e-6 including
+    features and inference, vs the uniform start, each with its error against the FP64
+    oracle (CPU leg)."""
     from accelgraph_amd import warmstart
     wpath = os.path.join(ROOT, "accelgraph_amd", "warmstart_weights.json")
     m = ag.MLP(dev)
     m.load(wpath)
     res = warmstart.evaluate(dev, g, m, f64)
+    for scale in (22, 24):
+        h = ag.Graph.rmat_gpu(dev, scale)
+        hf = None
+        if cpu:
+            import oracle
+            hf = oracle.pagerank_f64(h.out_degree, h.inv_idx, h.in_degree, h.inv_src,
+                                     tol=1e-13, max_iterations=150, threads=HOST_THREADS)
+        res[f"held_out_rmat{scale}"] = warmstart.evaluate(dev, h, m, hf)
+        h.close()
     with open(wpath) as f:
         res["training"] = json.load(f).get("training")
     m.close()
     return res
 
 
-def batched_section(ag, dev, count: int, lo: int, hi: int) -> dict:
+def batched_section(ag, dev, count: int, lo: int, hi: int, cpu: bool) -> dict:
     """count RMAT graphs with vertex counts 2^s log-uniform, i.e. scale s uniform over the
     integers [lo, hi] (seeded), to L1 <= 1e-6: one persistent hipGraph launch versus the plain
-    single-graph GPU path run graph by graph."""
+    single-graph GPU path run graph by graph; each graph's distance to its FP64 solution
+    (bounded through the FP64 residual, CPU leg)."""
     import numpy as np
     rng = np.random.default_rng(ag.DEFAULT_SEED)
     scales = rng.integers(lo, hi + 1, count)
@@ -213,8 +230,14 @@ def batched_section(ag, dev, count: int, lo: int, hi: int) -> dict:
         f"{time.perf_counter() - t0:.1f} s")
     b = ag.Batch(dev, graphs)
     kw = dict(max_iterations=300, stop_rule=ag.STOP_L1, l1_tolerance=1e-6, init=ag.INIT_UNIFORM)
-    b.pagerank(use_hip_graph=1, **kw)                       # capture + warm
-    ranks, its, err, ms = b.pagerank(use_hip_graph=1, **kw)
+    # part size and heavy-row threshold picked by the autotuner (timed on this solve)
+    from accelgraph_amd import autotune
+    tuned, rep = autotune.autotune_batch(b, repeats=2, passes=1, force=True,
+                                         table_path=os.path.join(_tmpdir(), "tuning.json"))
+    b.pagerank(use_hip_graph=1, **kw)                       # capture + warm (defaults)
+    _, _, _, default_ms = b.pagerank(use_hip_graph=1, **kw)
+    b.pagerank(use_hip_graph=1, **tuned, **kw)              # capture + warm (tuned)
+    ranks, its, err, ms = b.pagerank(use_hip_graph=1, **tuned, **kw)
     dgs = [dev.upload(g) for g in graphs]
     for dg in dgs[:4]:
         dg.pagerank(history=False, **kw)                    # warm
@@ -226,28 +249,61 @@ def batched_section(ag, dev, count: int, lo: int, hi: int) -> dict:
     for dg in dgs:
         dg.close()
     b.close()
-    return {"graphs": count, "scales": f"{lo}-{hi} (2^s log-uniform)", "edges": int(sum(g.m for g in graphs)),
-            "batched_ms": round(ms, 3), "plain_loop_ms": round(plain_ms, 3),
-            "speedup": round(plain_ms / max(ms, 1e-9), 2),
-            "max_iterations": int(its.max()), "max_final_l1": float(err.max()),
-            "max_abs_diff_vs_single_graph_path": worst}
+    out = {"graphs": count, "scales": f"{lo}-{hi} (2^s log-uniform)", "edges": int(sum(g.m for g in graphs)),
+           "batched_ms": round(ms, 3), "plain_loop_ms": round(plain_ms, 3),
+           "autotuned": {"options": tuned, "ms": round(ms, 3),
+                         "default_options": rep["default"], "default_ms": round(default_ms, 3),
+                         "trials": rep["trials"]},
+           "speedup": round(plain_ms / max(ms, 1e-9), 2),
+           "max_iterations": int(its.max()), "max_final_l1": float(err.max()),
+           "max_abs_diff_vs_single_graph_path": worst}
+    if cpu:
+        # per-graph L1 distance to the FP64 solution (normalised units), bounded by the FP64
+        # residual: ||r - r*||_1 <= ||p + 0.85 A r - r||_1 / 0.15
+        import oracle
+        rel = []
+        for g, r in zip(graphs, ranks):
+            res, nrm = oracle.ppr_residual(g.out_degree, g.inv_idx, g.in_degree, g.inv_src,
+                                           [ag.PPR_UNIFORM], r, threads=HOST_THREADS)
+            rel.append(float(res[0] / 0.15))
+        out["max_l1_err_vs_f64_bound"] = max(rel)
+        out["median_l1_err_vs_f64_bound"] = float(np.median(rel))
+    return out
 
 
-def ppr_section(ag, dev, scale: int, k: int) -> dict:
+def ppr_section(ag, dev, scale: int, k: int, cpu: bool) -> dict:
     """k-column personalised PageRank (column 0 uniform = plain PageRank, the others seeded
-    uniformly at random by seed) to L1 <= 1e-6, BF16 storage with FP32 refinement, versus k
-    plain single-vector runs (estimated as k x one plain run's time)."""
+    uniformly at random by seed) to L1 <= 1e-6 in BF16, FP16 and FP32 storage (16-bit storage
+    refined in FP32), one captured hipGraph solve each, versus k plain single-vector runs
+    (k x one plain run's time); every column's distance to its FP64 solution is bounded
+    through the FP64 residual (CPU leg)."""
     import numpy as np
     g = ag.Graph.rmat(scale)
     dg = dev.upload(g)
     rng = np.random.default_rng(ag.DEFAULT_SEED)
     seeds = np.concatenate([[ag.PPR_UNIFORM], rng.choice(g.n, k - 1, replace=False)])
     res = {}
+    from accelgraph_amd import autotune
+    tuned, rep = autotune.autotune_ppr(dg, seeds, iterations=6, repeats=2, force=True,
+                                       table_path=os.path.join(_tmpdir(), "tuning.json"))
+    res["autotuned"] = {"options": tuned, "bf16_ms_per_iteration": round(rep["ms_per_iteration"], 4),
+                        "default_options": rep["default"],
+                        "default_bf16_ms_per_iteration": round(rep["default_ms_per_iteration"], 4),
+                        "trials": rep["trials"]}
     for name, st in (("bf16", ag.STORE_BF16), ("f16", ag.STORE_F16), ("f32", ag.STORE_F32)):
+        dg.ppr(seeds, max_iterations=300, stop_rule=ag.STOP_L1, l1_tolerance=1e-6, storage=st,
+               refine_l1=1e-3, **tuned)                      # capture + warm
         ranks, colerr, r = dg.ppr(seeds, max_iterations=300, stop_rule=ag.STOP_L1,
-                                  l1_tolerance=1e-6, storage=st, refine_l1=1e-3)
+                                  l1_tolerance=1e-6, storage=st, refine_l1=1e-3, **tuned)
         res[name] = {"ms": round(r.loop_ms, 3), "iterations": r.executed,
+                     "ms_per_iteration": round(r.loop_ms / max(r.executed, 1), 4),
                      "max_col_l1": r.max_error, "refine_iteration": r.refine_iteration}
+        if cpu:
+            # every column's L1 distance to its FP64 solution, bounded by the FP64 residual
+            import oracle
+            rr, nrm = oracle.ppr_residual(g.out_degree, g.inv_idx, g.in_degree, g.inv_src,
+                                          seeds, ranks, threads=HOST_THREADS)
+            res[name]["max_col_l1_err_vs_f64_bound"] = float((rr / 0.15).max())
         if name == "bf16":
             col0 = ranks[:, 0].copy()
     plain = dg.pagerank(max_iterations=300, stop_rule=ag.STOP_L1, l1_tolerance=1e-6,
@@ -368,11 +424,14 @@ def config1_section(ag, dev, cpu: bool) -> dict:
     return out
 
 
-def er_section(ag, dev, scale: int, steps: int) -> dict:
-    """PageRank-pull FP32 iteration on an Erdos-Renyi graph with the bench graph's vertex and
-    edge counts (uniform sources: no hot set, every gather is a cold random read)."""
+def er_section(ag, dev, scale: int, steps: int, cpu: bool) -> dict:
+    """PageRank-pull FP32 iteration on an Erdos-Renyi graph with the bench graph's vertex
+    count and 16n edge draws (uniform sources: no hot set, every gather is a cold random
+    read), and the solve to L1 <= 1e-6 from the uniform start with its error against the FP64
+    oracle (CPU leg)."""
+    import numpy as np
     n = 1 << scale
-    g = ag.Graph.erdos_renyi(n, 16 * n)
+    g = ag.Graph.erdos_renyi_gpu(dev, n, 16 * n)
     dg = dev.upload(g)
     sess = dg.session(max_iterations=steps + 3, stop_rule=ag.STOP_NONE, init=ag.INIT_UNIFORM)
     sess.step(3)
@@ -382,13 +441,27 @@ def er_section(ag, dev, scale: int, steps: int) -> dict:
     ms, _ = sess.last_step_kernel_ms()
     sess.end()
     info = dg.info()
+    kw = dict(max_iterations=300, stop_rule=ag.STOP_L1, l1_tolerance=1e-6,
+              init=ag.INIT_UNIFORM, history=True)
+    run = dg.pagerank(**kw)
     dg.close()
     nb = algorithmic_bytes(g.n, g.m, "f32")
     gbs = nb / (ms * 1e-3) / 1e9
-    return {"graph": f"Erdos-Renyi n=2^{scale}, 16n draws, simplified (n={g.n}, m={g.m})",
-            "ms_per_iteration": round(ms, 4), "gteps": round(g.m / (ms * 1e-3) / 1e9, 2),
-            "gb_per_s": round(gbs, 1), "roofline_frac": round(gbs / HBM_PEAK_GBS, 4),
-            "src_block_bits": info.src_block_bits}
+    out = {"graph": f"Erdos-Renyi n=2^{scale}, 16n draws, simplified (n={g.n}, m={g.m})",
+           "ms_per_iteration": round(ms, 4), "gteps": round(g.m / (ms * 1e-3) / 1e9, 2),
+           "gb_per_s": round(gbs, 1), "roofline_frac": round(gbs / HBM_PEAK_GBS, 4),
+           "src_block_bits": info.src_block_bits, "hot_segments": info.hot_segments,
+           "time_to_l1_1e-6": {"ms": round(float(run.hist_ms.sum()), 3),
+                               "iterations": run.executed, "final_l1": run.error_total}}
+    if cpu:
+        import oracle
+        f64 = oracle.pagerank_f64(g.out_degree, g.inv_idx, g.in_degree, g.inv_src, tol=1e-13,
+                                  max_iterations=150, threads=HOST_THREADS)
+        d = np.abs(run.ranks.astype(np.float64) - f64)
+        out["time_to_l1_1e-6"]["max_abs_err_vs_f64"] = float(d.max())
+        out["time_to_l1_1e-6"]["l1_err_vs_f64"] = float(d.sum())
+    g.close()
+    return out
 
 
 def main() -> int:
@@ -413,6 +486,8 @@ def main() -> int:
     ap.add_argument("--no-spmv", action="store_true", help="skip the RMAT-22 SpMV")
     ap.add_argument("--spmv-scale", type=int, default=22)
     ap.add_argument("--no-er", action="store_true", help="skip the ER PageRank iteration")
+    ap.add_argument("--hot-segments", type=int, default=-1,
+                    help="plan parameter hot_segments for the bench graph (-1 = the default)")
     ap.add_argument("--no-rows", action="store_true",
                     help="skip the fixed-32, quantised, push and data-driven rows")
     args = ap.parse_args()
@@ -459,7 +534,8 @@ def main() -> int:
         log(f"graph construction: {construction}")
     log(f"device copy bandwidth {copy_gbs:.0f} GB/s (1 GiB, read + write)")
     t0 = time.perf_counter()
-    dg = dev.upload(g)
+    dg = dev.upload(g, params=(ag.plan_params(hot_segments=args.hot_segments)
+                               if args.hot_segments >= 0 else None))
     info = dg.info()
     log(f"uploaded + planned in {time.perf_counter() - t0:.1f} s; bins rows="
         f"{list(info.bin_rows)} edges={list(info.bin_edges)} tiles={info.num_tiles} "
@@ -502,13 +578,19 @@ def main() -> int:
         directions = directions_section(ag, dev, dg, m, n)
         log(f"push / data-driven: {directions}")
 
-    traffic = None
+    # HBM traffic per launch is not measured in this run (counters need rocprofv3 runs of
+    # their own): it is the figure of the committed profile of this workload and plan, used
+    # only while the profile's plan matches the one in effect
+    traffic = traffic_src = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     wl = f"pagerank_pull_{args.storage}_{args.graph}{args.scale}"
     if os.path.exists(tpath):
         t = json.load(open(tpath)).get(wl)
-        if t:
+        if t and t.get("hot_segments") == info.hot_segments and \
+                t.get("src_block_bits") == info.src_block_bits:
             traffic = t["traffic_bytes_per_launch"]
+            traffic_src = (f"profiled, not measured in this run: {t['profile']} "
+                           f"(rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a copy kernel)")
     out = {
         "metric": METRIC, "value": round(gteps, 3), "unit": "GTEPS", "n_gpus": 1,
         "steps": K, "warmup": W, "ms_per_step": round(ms_per_step, 4),
@@ -522,13 +604,19 @@ def main() -> int:
         "hbm_gb_per_s": round(nbytes / (ms_per_step * 1e-3) / 1e9, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "algorithmic_bytes": nbytes,
+                     "traffic": traffic, "traffic_source": traffic_src,
+                     "algorithmic_bytes": nbytes,
                      "kernel_ms": round(kern_ms, 4),
-                     "kernel": ("one iteration = pr_f32_blk_kernel (source-blocked gather) + "
+                     "kernel": ("one iteration = " +
+                                ("pr_f32_hot_kernel (hot source segments gathered from LDS) + "
+                                 if info.hot_segments else "") +
+                                "pr_f32_blk_kernel (source-blocked gather) + "
                                 "pr_f32_cmb_kernel (combine + update) + pr_f32_finish"
                                 if info.src_block_bits else
                                 "one iteration = pr_f32_kernel + pr_f32_finish"),
                      "src_block_bits": info.src_block_bits,
+                     "hot_segments": info.hot_segments,
+                     "hot_edge_fraction": round(info.hot_edges / max(m, 1), 4),
                      "measured_copy_gb_per_s": round(copy_gbs, 1)},
     }
 
@@ -556,18 +644,19 @@ def main() -> int:
         dg.close()
         if not args.no_warm:
             t0 = time.perf_counter()
-            out["time_to_l1_1e-6"]["learned_warm_start"] = warm_section(ag, dev, g, f64)
+            out["time_to_l1_1e-6"]["learned_warm_start"] = warm_section(ag, dev, g, f64,
+                                                                        not args.no_cpu)
             log(f"warm start: {out['time_to_l1_1e-6']['learned_warm_start']} "
                 f"({time.perf_counter() - t0:.1f} s)")
     else:
         dg.close()
     if not args.no_batch and args.batch_count > 0:
         t0 = time.perf_counter()
-        out["batched"] = batched_section(ag, dev, args.batch_count, 10, 16)
+        out["batched"] = batched_section(ag, dev, args.batch_count, 10, 16, not args.no_cpu)
         log(f"batched: {out['batched']} ({time.perf_counter() - t0:.1f} s)")
     if not args.no_ppr:
         t0 = time.perf_counter()
-        out["ppr_64"] = ppr_section(ag, dev, args.ppr_scale, 64)
+        out["ppr_64"] = ppr_section(ag, dev, args.ppr_scale, 64, not args.no_cpu)
         log(f"ppr: {out['ppr_64']} ({time.perf_counter() - t0:.1f} s)")
     if not args.no_spmv:
         t0 = time.perf_counter()
@@ -579,7 +668,7 @@ def main() -> int:
         log(f"config 1: {out['config1_rmat16']} ({time.perf_counter() - t0:.1f} s)")
     if not args.no_er and args.graph == "rmat":
         t0 = time.perf_counter()
-        out["er_pagerank"] = er_section(ag, dev, args.scale, 20)
+        out["er_pagerank"] = er_section(ag, dev, args.scale, 20, not args.no_cpu)
         log(f"er: {out['er_pagerank']} ({time.perf_counter() - t0:.1f} s)")
     dev.close()
     if not args.no_cpu:

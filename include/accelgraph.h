@@ -98,7 +98,24 @@ typedef struct ag_plan_params {
                                   Default AG_BIN_THREAD (every row): on a blocked graph the
                                   thread bin then loses its stream-order FP32 sums (FP32
                                   results stay within rounding; integer results are exact) */
+    uint32_t hot_segments;     /* with source blocking, at most this many "hot" source
+                                  segments (2^min(14, bits) consecutive device ids whose
+                                  in-edges average at least two per (row, segment) piece and
+                                  four per source) become blocks of their own, gathered by FP32
+                                  PageRank and FP32 SpMV from a copy of the segment in LDS
+                                  instead of through L2 requests. Default 128; 0 = none      */
+    uint32_t fp32_order;       /* AG_ORDER_FAST (default): rows above the thread bin are
+                                  summed in fixed tree / block orders. AG_ORDER_REFERENCE:
+                                  every row is summed in sequence in inverse-CSR order, as
+                                  the reference's CPU loop and float sum kernel do, so FP32
+                                  results (and the active-count stop) equal the sequential
+                                  oracle bit for bit. The plan then bins rows up to tile_nnz
+                                  in-edges as thread-bin rows (one lane each, from LDS), every
+                                  longer row is one hub segment added by one lane in chunks
+                                  staged through LDS, and source blocking is off             */
 } ag_plan_params;
+
+enum { AG_ORDER_FAST = 0, AG_ORDER_REFERENCE = 1 };
 
 #define AG_SRC_BLOCK_AUTO 0u
 #define AG_SRC_BLOCK_OFF 255u
@@ -120,6 +137,9 @@ typedef struct ag_graph_info {
     uint32_t src_blocks;       /* source blocks                                         */
     uint32_t virtual_rows;     /* (row, source block) pieces of the blocked rows        */
     uint32_t blocked_row0;     /* first blocked device row                              */
+    uint32_t hot_segments;     /* hot source segments in effect (blocks staged in LDS)      */
+    uint32_t hot_segment_bits; /* sources per hot segment = 2^this                          */
+    uint64_t hot_edges;        /* in-edges whose source lies in a hot segment               */
 } ag_graph_info;
 
 /* Upload one directed graph. Arrays are host arrays in the reference's layout:
@@ -278,7 +298,13 @@ typedef struct ag_ppr_options {
     double l1_tolerance;
     uint32_t storage;          /* AG_STORE_*                                                */
     double refine_l1;
-    uint32_t chunk;            /* iterations queued between convergence polls (0 = auto)   */
+    uint32_t chunk;            /* eager mode: iterations queued between polls (0 = auto)   */
+    uint32_t use_hip_graph;    /* 1 (default): the whole solve is one captured hipGraph,
+                                  replayed with one launch and one wait (captured once per
+                                  graph, max_iterations and grid; the buffers stay with the
+                                  graph)                                                   */
+    uint32_t wg_per_cu;        /* resident workgroups per CU of the iteration kernel (0 =
+                                  as many as fit, up to 8)                                  */
 } ag_ppr_options;
 
 typedef struct ag_ppr_result {
@@ -372,8 +398,10 @@ int ag_graph_csr_generate_er(ag_device *dev, uint32_t num_vertices, uint64_t num
 
 /* ---------------------------------------------------------------- many small graphs */
 
-/* A batch of independent graphs resident in HBM, solved together by one persistent launch
- * (one workgroup per graph, largest graphs first) that a hipGraph captures and replays.
+/* A batch of independent graphs resident in HBM, solved together: every graph is cut into
+ * parts (runs of rows with a bounded in-edge count), one launch per iteration runs every part
+ * of every graph still iterating and a second adds each graph's part partials and applies its
+ * stop rule; the whole solve is captured once into a hipGraph and replayed as one launch.
  * Inputs are the per-graph reference arrays concatenated: graph g owns vertices
  * [vert_off[g], vert_off[g+1]) and in-edges [edge_off[g], edge_off[g+1]); inv_edges_idx is
  * relative to the graph's first edge and inv_src holds graph-local vertex ids. */
@@ -391,12 +419,16 @@ typedef struct ag_batch_options {
     double epsilon;
     double l1_tolerance;
     uint32_t init;             /* AG_INIT_REFERENCE or AG_INIT_UNIFORM                   */
-    uint32_t use_hip_graph;    /* capture the launch once and replay it                  */
+    uint32_t use_hip_graph;    /* capture the solve once and replay it                   */
+    uint32_t part_edges;       /* in-edges per part (0 = AG_BATCH_PART_EDGES); a row with
+                                  more is a part by itself                                */
+    uint32_t heavy_deg;        /* rows with more in-edges are summed by a wavefront in a
+                                  fixed tree, the others by one lane each in the reference's
+                                  stream order (0 = AG_BATCH_HEAVY_DEG)                    */
 } ag_batch_options;
 
-/* Rows with more in-edges than this are summed by a whole wavefront; the others by one lane
- * each, in the reference's stream order. */
 #define AG_BATCH_HEAVY_DEG 32
+#define AG_BATCH_PART_EDGES 4096
 
 void ag_batch_options_default(ag_batch_options *o);
 

@@ -75,6 +75,10 @@ def lib() -> C.CDLL:
         L.or_ppr_f64.restype = C.c_int
         L.or_ppr_f64.argtypes = [u32, P(u32), P(u32), P(u32), P(u32), u32, f64, u32, C.c_int,
                                  P(f64)]
+        L.or_fixed_encode_n.argtypes = [u32, P(f32), u64, P(u64)]
+        L.or_ppr_residual.restype = C.c_int
+        L.or_ppr_residual.argtypes = [u32, P(u32), P(u32), P(u32), P(u32), u32, P(u32), P(f32),
+                                      C.c_int, P(f64), P(f64)]
         L.or_build_csr.restype = C.c_int
         L.or_build_csr.argtypes = [u32, u64, P(u32), P(u32), P(u32), P(u32), P(u32), P(u32),
                                    P(u32), P(u32), P(u64)]
@@ -95,6 +99,14 @@ def fixed_encode(bits: int, x) -> np.ndarray:
     L = lib()
     return np.array([L.or_fixed_encode(bits, float(v)) for v in np.asarray(x, np.float32)],
                     dtype=np.uint64)
+
+
+def fixed_encode_array(bits: int, x, threads: int = 0) -> np.ndarray:
+    """fixed_encode over a whole array in C (OpenMP): for arrays of millions of values."""
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros(max(x.size, 1), np.uint64)
+    lib().or_fixed_encode_n(bits, _p(x, f32), x.size, _p(out, u64))
+    return out[:x.size]
 
 
 def fixed_decode(bits: int, q) -> np.ndarray:
@@ -276,6 +288,24 @@ def ppr_f64(outdeg, idx, deg, src, seed: int, tol: float = 1e-14,
     lib().or_ppr_f64(idx.size, _p(outdeg, u32), _p(idx, u32), _p(deg, u32), _p(src, u32),
                      seed, tol, max_iterations, threads, _p(out, f64))
     return out
+
+
+def ppr_residual(outdeg, idx, deg, src, seeds, ranks, threads: int = 0):
+    """(residual L1, column L1) per column of normalised ranks [n, k]: the FP64 residual
+    p + 0.85 A r - r of the (personalised) PageRank equation. ||r - r*||_1 <= residual / 0.15."""
+    outdeg, idx, deg, src = _u32(outdeg), _u32(idx), _u32(deg), _u32(src)
+    seeds = _u32(np.atleast_1d(seeds))
+    r = np.ascontiguousarray(ranks, np.float32)
+    k = seeds.size
+    assert r.size == idx.size * k
+    res = np.zeros(k, np.float64)
+    nrm = np.zeros(k, np.float64)
+    st = lib().or_ppr_residual(idx.size, _p(outdeg, u32), _p(idx, u32), _p(deg, u32),
+                               _p(src, u32), k, _p(seeds, u32), _p(r, f32), threads,
+                               _p(res, f64), _p(nrm, f64))
+    if st != 0:
+        raise RuntimeError(f"or_ppr_residual failed ({st})")
+    return res, nrm
 
 
 def build_csr(n: int, src, dst):

@@ -29,6 +29,11 @@
  *                      or_pagerank_pull kinds 132/116/108 run those pulls end to end).
  *   or_fixed_*         the fixed-point formats, computed with integer arithmetic on the float's
  *                      mantissa/exponent rather than with floating-point scaling.
+ *   or_ppr_residual    the FP64 residual p + 0.85 * A r - r of given (personalised) rank
+ *                      columns: the defining equation of the fixed point of the update at
+ *                      pageRank.c:1055-1069 (dangling mass dropped, :1029-1032). Since
+ *                      ||0.85 A||_1 <= 0.85, ||r - r*||_1 <= ||residual||_1 / 0.15 bounds the
+ *                      distance of r to the FP64 solution without computing it.
  *
  * Threads: `threads` = 0 runs the sequential restatement; > 0 runs the same loops under
  * OpenMP with schedule(dynamic,1024) (pageRank.c:2890), the CPU baseline. Per-vertex values
@@ -95,6 +100,13 @@ double or_fixed_decode(uint32_t bits, uint64_t q)
     case 16: return (double)ldexpf((float)q, -8);
     default: return (double)ldexpf((float)q, -4);
     }
+}
+
+void or_fixed_encode_n(uint32_t bits, const float *x, uint64_t n, uint64_t *out)
+{
+    #pragma omp parallel for schedule(static)
+    for (uint64_t i = 0; i < n; i++)
+        out[i] = or_fixed_encode(bits, x[i]);
 }
 
 /* ------------------------------------------------------------------ gathers */
@@ -688,6 +700,53 @@ int or_ppr_f64(uint32_t n, const uint32_t *outdeg, const uint32_t *idx, const ui
     free(rank);
     free(contrib);
     return (int)it;
+}
+
+/* FP64 residual of k rank columns, vertex-major ranks[v * k + c], normalised (divided by n).
+ * Column c restarts on seeds[c] (UINT32_MAX: uniform restart, plain PageRank): the exact
+ * solution satisfies r_v = p_v + 0.85 * sum_{u -> v} r_u / outdeg_u with p_v = 0.15 / n
+ * (uniform) or 0.15 * [v == seed]. Writes, per column, the L1 norm of the residual and of
+ * the column. */
+int or_ppr_residual(uint32_t n, const uint32_t *outdeg, const uint32_t *idx,
+                    const uint32_t *deg, const uint32_t *src, uint32_t k, const uint32_t *seeds,
+                    const float *ranks, int threads, double *res_l1, double *rank_l1)
+{
+    if (k == 0 || k > 256)
+        return -1;
+    const double damp = 0.85, base = 1.0 - 0.85;
+    const int nt = threads > 0 ? threads : 1;
+    for (uint32_t c = 0; c < k; c++)
+        res_l1[c] = rank_l1[c] = 0.0;
+    #pragma omp parallel num_threads(nt)
+    {
+        double *acc = (double *)calloc(3 * (size_t)k, sizeof(double));
+        double *res = acc + k, *nrm = acc + 2 * (size_t)k;
+        #pragma omp for schedule(dynamic, 1024)
+        for (uint32_t v = 0; v < n; v++) {
+            for (uint32_t c = 0; c < k; c++)
+                acc[c] = 0.0;
+            for (uint32_t e = 0; e < deg[v]; e++) {
+                const uint32_t u = src[idx[v] + e];
+                const double inv = 1.0 / (double)outdeg[u];
+                const float *ru = ranks + (size_t)u * k;
+                for (uint32_t c = 0; c < k; c++)
+                    acc[c] += (double)ru[c] * inv;
+            }
+            const float *rv = ranks + (size_t)v * k;
+            for (uint32_t c = 0; c < k; c++) {
+                const double p = seeds[c] == 0xFFFFFFFFu ? base / n : (v == seeds[c] ? base : 0.0);
+                res[c] += fabs(p + damp * acc[c] - (double)rv[c]);
+                nrm[c] += fabs((double)rv[c]);
+            }
+        }
+        #pragma omp critical
+        for (uint32_t c = 0; c < k; c++) {
+            res_l1[c] += res[c];
+            rank_l1[c] += nrm[c];
+        }
+        free(acc);
+    }
+    return 0;
 }
 
 /* ------------------------------------------------------------------ CSR builder */

@@ -136,3 +136,70 @@ def test_spmv_entry_points(graph):
     yq = oracle.spmv_push_fixed(*fwd, None, oracle.fixed_encode(64, x), reps=2)
     np.testing.assert_array_equal(y, oracle.fixed_decode(64, yq).astype(np.float32))
     L.freeSPMVStats(st)
+
+
+def test_graph_rebuilt_in_the_same_buffers():
+    """The reference's vertex-order sweep frees a graph and builds the same graph relabelled,
+    with the same N and M, in a loop, and never releases anything
+    (02_capi_graph/src/tests/sweep_order-PR-performance-graph.c:205-233); freed arrays of
+    that size come back at the same addresses. Here the relabelled graph is written into the
+    very same buffers: the drop-in call must notice the new contents and return the new
+    graph's ranks (fixed-32 bit-exact with the oracle), twice in a row, then again after the
+    first graph is written back."""
+    say("drop-in cache: relabelled graph written into the same buffers")
+    L = ag.lib()
+    g = ag.Graph.rmat(12, seed=4242)
+    rng = np.random.default_rng(17)
+    perm = rng.permutation(g.n).astype(np.uint32)
+    fsrc = np.repeat(np.arange(g.n, dtype=np.uint32), g.out_degree.astype(np.int64))
+    h = ag.Graph.from_edges(perm[fsrc], perm[g.fwd_dest], n=g.n)
+    assert h.n == g.n and h.m == g.m
+    views = lambda x: (x.out_degree, x.edges_idx, x.fwd_src, x.fwd_dest, x.in_degree,
+                       x.inv_idx, x.inv_src)
+    first = [a.copy() for a in views(g)]
+
+    def run_fixed32():
+        st = L.pageRankPullFixedPoint32BitGraphCSR(C.byref(arguments(9)), g.ptr)
+        ranks = np.ctypeslib.as_array(st.contents.pageRanks, shape=(g.n,)).copy()
+        its = int(st.contents.iterations)
+        L.freePageRankStats(st)
+        return ranks, its
+
+    def expect(x):
+        return oracle.pagerank_pull(x.out_degree, x.inv_idx, x.in_degree, x.inv_src, kind=32,
+                                    max_iterations=100, epsilon=1e-4)
+
+    want_g = expect(g)
+    want_h = expect(h)
+    assert not np.array_equal(want_g.ranks, want_h.ranks)
+    for contents, want in ((first, want_g), (views(h), want_h), (views(h), want_h),
+                           (first, want_g)):
+        for dst, src in zip(views(g), contents):
+            dst[:] = src
+        ranks, its = run_fixed32()
+        assert its == want.iterations
+        np.testing.assert_array_equal(ranks.view(np.uint32), want.ranks.view(np.uint32))
+    L.accelgraphShutdown()
+    g.close()
+    h.close()
+
+
+def test_reference_order_through_the_drop_in(graph, monkeypatch):
+    """ACCELGRAPH_FP32_ORDER=reference: pageRankPullGraphCSR sums every row in the reference's
+    order, so its iteration count and ranks equal the sequential oracle's bit for bit (the
+    default fast order may stop a few iterations apart)."""
+    say("pageRankPullGraphCSR with ACCELGRAPH_FP32_ORDER=reference")
+    L = ag.lib()
+    g = graph
+    monkeypatch.setenv("ACCELGRAPH_FP32_ORDER", "reference")
+    ref = oracle.pagerank_pull(g.out_degree, g.inv_idx, g.in_degree, g.inv_src,
+                               max_iterations=100, epsilon=1e-4)
+    st = L.pageRankPullGraphCSR(C.byref(arguments(0)), g.ptr)
+    s = st.contents
+    ranks = np.ctypeslib.as_array(s.pageRanks, shape=(g.n,)).copy()
+    assert s.iterations == ref.iterations and s.activeVertices == ref.active
+    np.testing.assert_array_equal(ranks.view(np.uint32), ref.ranks.view(np.uint32))
+    assert abs(s.error_total - ref.error_total) <= 1e-12 + 1e-9 * ref.error_total
+    L.freePageRankStats(st)
+    monkeypatch.delenv("ACCELGRAPH_FP32_ORDER")
+    L.accelgraphReleaseGraph()

@@ -1,8 +1,10 @@
 """GPU parity: the HIP path through the C-ABI against the CPU oracle.
 
 Bars: integer / fixed-point results bit-exact; FP32 ranks within 1e-5 max-abs per vertex of
-the oracle's FP32 path (the reference's own FP32 arithmetic), plus the error against the FP64
-oracle, which is asserted to be of the same size as the oracle's own FP32 error.
+the oracle's FP32 path (the reference's own FP32 arithmetic), plus two per-row bars in scaled
+units that hold heavy and light rows alike (tests/fp32_bars.py): one iteration from the same
+start within the error bound of a reordered sum of the row's terms, and after a fixed number
+of iterations every row at most twice as far from FP64 as the oracle's row plus 64 ulp.
 """
 import json
 import os
@@ -13,6 +15,7 @@ import pytest
 
 import accelgraph_amd as ag
 import oracle
+from fp32_bars import anchor_check, f32_row_bars, one_step_check
 
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -30,8 +33,10 @@ PARAMS = {"default": None, "small": SMALL, "small-order1": dict(SMALL, order=1),
           # row class and every combine bin populated)
           "blocked": dict(src_block_bits=10, blocked_min_bin=1),
           "blocked-all": dict(src_block_bits=8, blocked_min_bin=0),
-          "blocked-small": dict(SMALL, src_block_bits=9, blocked_min_bin=1)}
-BLOCKED = ["blocked", "blocked-all", "blocked-small"]
+          "blocked-small": dict(SMALL, src_block_bits=9, blocked_min_bin=1),
+          # the same blocking without hot segments (every gather through L2)
+          "blocked-nohot": dict(src_block_bits=10, blocked_min_bin=1, hot_segments=0)}
+BLOCKED = ["blocked", "blocked-all", "blocked-small", "blocked-nohot"]
 
 
 def say(*a):
@@ -190,6 +195,11 @@ def test_pagerank_f32_reference_rule(device, rmat16, pname):
     say(f"pagerank f32 reference rule {pname}")
     g = rmat16
     dg = upload(device, g, pname)
+    info = dg.info()
+    if pname == "blocked":    # hot segments gathered from LDS, the rest through L2
+        assert info.hot_segments > 0 and 0 < info.hot_edges < g.m
+    if pname == "blocked-nohot":
+        assert info.hot_segments == 0 and info.src_block_bits == 10
     run = dg.pagerank(max_iterations=100, epsilon=1e-4)
     ref = ref_pr(g, max_iterations=100, epsilon=1e-4)
     # Rows outside the thread bin are summed in a fixed tree order rather than the reference's
@@ -212,6 +222,46 @@ def test_pagerank_f32_reference_rule(device, rmat16, pname):
     say(f"  iterations={run.iterations} maxabs_vs_cpu={err:.3e} vs_f64 gpu={gpu_err:.3e} "
         f"cpu={cpu_err:.3e}")
     assert gpu_err <= 2 * cpu_err + 1e-9
+    f32_row_bars(dg, g, 30)
+    dg.close()
+
+
+REF_ORDER_PLANS = {"default": {}, "small": SMALL, "small-order1": dict(SMALL, order=1),
+                   "tiles": dict(tile_rows=7, tile_nnz=64),
+                   "blocked": dict(src_block_bits=10, blocked_min_bin=1)}
+
+
+@pytest.mark.parametrize("pname", list(REF_ORDER_PLANS))
+def test_pagerank_f32_reference_order(device, rmat16, pname):
+    """fp32_order = AG_ORDER_REFERENCE: every row summed in sequence in inverse-CSR order (the
+    reference's CPU loop, pageRank.c:2903-2907, and float sum kernel,
+    pagerank_kernel_tb.sv:303-314) -- rows that fit a tile by one lane from LDS, longer rows
+    (here up to 6310 in-edges: two LDS chunks) by one lane over chunks. Under the reference's
+    active-count stop (pageRank.c:1065-1074) the run equals the sequential oracle bit for bit:
+    iteration count, active history, scaled and normalised ranks, whatever the other plan
+    parameters (source blocking is turned off by the mode)."""
+    say(f"pagerank f32 reference order {pname}")
+    g = rmat16
+    assert g.in_degree.max() > 4096
+    dg = device.upload(g, params=ag.plan_params(**REF_ORDER_PLANS[pname],
+                                                 fp32_order=ag.ORDER_REFERENCE))
+    info = dg.info()
+    assert info.src_block_bits == 0
+    assert all(info.bin_rows[b] == 0 for b in (1, 2, 3))   # tile rows and whole-row hubs only
+    run = dg.pagerank(max_iterations=100, epsilon=1e-4, scaled=True)
+    ref = ref_pr(g, max_iterations=100, epsilon=1e-4)
+    assert run.iterations == ref.iterations and run.executed == ref.executed
+    np.testing.assert_array_equal(run.hist_active, ref.hist_active)
+    np.testing.assert_array_equal(run.scaled.view(np.uint32), ref.scaled.view(np.uint32))
+    np.testing.assert_array_equal(run.ranks.view(np.uint32), ref.ranks.view(np.uint32))
+    np.testing.assert_allclose(run.hist_err, ref.hist_err, rtol=1e-9, atol=1e-15)
+    # the uniform start and the L1 rule too
+    run = dg.pagerank(max_iterations=100, stop_rule=ag.STOP_L1, l1_tolerance=1e-6,
+                      init=ag.INIT_UNIFORM)
+    ref = ref_pr(g, max_iterations=100, stop_rule=1, l1_tolerance=1e-6, init=1)
+    assert run.iterations == ref.iterations
+    np.testing.assert_array_equal(run.ranks.view(np.uint32), ref.ranks.view(np.uint32))
+    say(f"  iterations={ref.iterations} bit-exact")
     dg.close()
 
 
@@ -225,6 +275,10 @@ def test_pagerank_f32_config1_l1_rule(device, rmat16):
     ref = ref_pr(g, max_iterations=100, stop_rule=1, l1_tolerance=1e-4, init=1)
     assert run.iterations == ref.iterations and run.error_total <= 1e-4
     assert np.abs(run.ranks - ref.ranks).max() <= FP32_TOL
+    f64 = oracle.pagerank_f64(g.out_degree, g.inv_idx, g.in_degree, g.inv_src, tol=0.0,
+                              max_iterations=run.executed, threads=8)
+    anchor_check(run.ranks, ref.ranks, f64)
+    f32_row_bars(dg, g, run.executed)
     dg.close()
 
 
@@ -568,7 +622,8 @@ def test_pagerank_blocked_auto_rmat22(device):
     g = ag.Graph.rmat(22)
     dg = device.upload(g)
     info = dg.info()
-    assert info.src_block_bits == 19 and info.src_blocks == 8 and info.virtual_rows > 0
+    assert info.src_block_bits == 19 and info.virtual_rows > 0
+    assert info.hot_segments > 0 and info.src_blocks == 8 + info.hot_segments
     kw = dict(max_iterations=20, stop_rule=ag.STOP_NONE)
     fx = dg.pagerank(kind=32, **kw)
     ref = ref_pr(g, kind=32, max_iterations=20, stop_rule=2, threads=8)
@@ -578,11 +633,13 @@ def test_pagerank_blocked_auto_rmat22(device):
     err = np.abs(run.ranks - ref.ranks).max()
     say(f"  fixed32 bit-exact; f32 maxabs={err:.3e}")
     assert err <= FP32_TOL
+    f32_row_bars(dg, g, 20)
     dg.close()
     plain = device.upload(g, params=ag.plan_params(src_block_bits=ag.SRC_BLOCK_OFF))
     assert plain.info().src_block_bits == 0
     run2 = plain.pagerank(**kw)
     assert np.abs(run.ranks - run2.ranks).max() <= FP32_TOL
+    f32_row_bars(plain, g, 20)
     plain.close()
 
 
@@ -596,7 +653,8 @@ def test_rmat25_full_size(device):
     assert g.n == 1 << 25 and g.m == 528_693_006
     dg = device.upload(g)
     info = dg.info()
-    assert info.src_block_bits == 19 and info.src_blocks == 64
+    assert info.src_block_bits == 19 and info.src_blocks == 64 + info.hot_segments
+    assert info.hot_segments > 0 and info.hot_segment_bits == 14
     kw = dict(max_iterations=4, stop_rule=ag.STOP_NONE)
     say("rmat-25 full size: fixed-32 pull")
     fx = dg.pagerank(kind=32, **kw)
@@ -612,6 +670,8 @@ def test_rmat25_full_size(device):
     assert np.abs(run.ranks - ref.ranks).max() <= FP32_TOL
     light = g.in_degree <= 1024   # rows whose sequential FP32 sum stays within ~1e-4 relative
     np.testing.assert_allclose(run.ranks[light], ref.ranks[light], rtol=1e-4, atol=0)
+    say("rmat-25 full size: per-row FP32 bars (hub rows included)")
+    f32_row_bars(dg, g, 4, threads=16)
     dd = dg.pagerank(direction=ag.DIR_DATA_DRIVEN, epsilon=0.0, **kw)
     np.testing.assert_array_equal(dd.ranks.view(np.uint32), run.ranks.view(np.uint32))
     dg.close()
@@ -702,6 +762,37 @@ def test_spmv_unweighted(device):
 
 
 # ------------------------------------------------------------------ edge cases
+
+def test_watchdog_timeout_then_recovery(device):
+    """The bounded wait (the role of waitAFU's run deadline, capienv.c:526-638): with a 1 ms
+    deadline, a poll behind 100 queued RMAT-20 iterations (~10 ms of kernels) returns
+    AG_ERR_TIMEOUT from the host; the kernels themselves run to their normal end. Without the
+    deadline the same session then completes, its fixed-32 ranks bit-exact with the oracle,
+    and the device takes new work."""
+    say("watchdog: timeout, then the same session completes")
+    g = ag.Graph.rmat(20)
+    dg = device.upload(g)
+    iters = 100
+    sess = dg.session(kind=32, max_iterations=iters, stop_rule=ag.STOP_NONE)
+    try:
+        device.set_watchdog(1, 20)
+        sess.step(iters)
+        with pytest.raises(ag.AGError) as exc:
+            sess.poll()
+        assert exc.value.status == ag._abi.AG_ERR_TIMEOUT
+    finally:
+        device.set_watchdog(0)
+    device.sync()
+    r = sess.poll()
+    assert r.executed == iters
+    ranks, _ = sess.end()
+    ref = ref_pr(g, kind=32, max_iterations=iters, stop_rule=2, threads=8)
+    np.testing.assert_array_equal(ranks.view(np.uint32), ref.ranks.view(np.uint32))
+    run = dg.pagerank(kind=32, max_iterations=5, stop_rule=ag.STOP_NONE)
+    ref = ref_pr(g, kind=32, max_iterations=5, stop_rule=2, threads=8)
+    np.testing.assert_array_equal(run.ranks.view(np.uint32), ref.ranks.view(np.uint32))
+    dg.close()
+
 
 def test_edge_cases(device):
     say("edge cases")
@@ -839,10 +930,45 @@ def test_batch_hip_graph_replays(device):
         again, _, _, _ = b.pagerank(use_hip_graph=0, **kw)
         for i in range(len(gs)):
             np.testing.assert_array_equal(again[i], eager[i])
-    # options change between replays without re-capture
+    # the stop rule and the start change between replays of one capture; another iteration
+    # budget captures again
+    got, gits, _, _ = b.pagerank(use_hip_graph=1, max_iterations=200, stop_rule=ag.STOP_ACTIVE,
+                                 epsilon=1e-4, init=ag.INIT_REFERENCE)
+    eager, eits, _, _ = b.pagerank(use_hip_graph=0, max_iterations=200,
+                                   stop_rule=ag.STOP_ACTIVE, epsilon=1e-4,
+                                   init=ag.INIT_REFERENCE)
+    np.testing.assert_array_equal(gits, eits)
+    for i in range(len(gs)):
+        np.testing.assert_array_equal(got[i], eager[i])
     got, gits, _, _ = b.pagerank(use_hip_graph=1, max_iterations=3, stop_rule=ag.STOP_NONE,
                                  init=ag.INIT_UNIFORM)
     assert np.all(gits == 3)
+    b.close()
+
+
+@pytest.mark.parametrize("part_edges,heavy_deg", [(64, 32), (7, 4), (100000, 8)])
+def test_batch_part_layouts(device, part_edges, heavy_deg):
+    """The parts a graph is cut into (tiny parts: most graphs in many parts, single rows
+    bigger than a part alone in theirs; huge parts: one per graph) change only who computes
+    which rows: with the same heavy-row threshold every row's value is computed the same way,
+    so the ranks equal the default layout's bit for bit whenever the run stops at the same
+    iteration (the FP64 L1 delta is summed part by part, so it may differ in its last bits);
+    every threshold keeps the oracle's bar."""
+    say(f"batch part layout part_edges={part_edges} heavy_deg={heavy_deg}")
+    gs = _small_graphs(30, seed=21)
+    b = ag.Batch(device, gs)
+    kw = dict(max_iterations=300, stop_rule=ag.STOP_L1, l1_tolerance=1e-6, init=ag.INIT_UNIFORM)
+    base, bits, berr, _ = b.pagerank(heavy_deg=heavy_deg, **kw)
+    got, gits, gerr, _ = b.pagerank(part_edges=part_edges, heavy_deg=heavy_deg, **kw)
+    assert np.sum(gits != bits) <= 1
+    same = gits == bits
+    np.testing.assert_allclose(gerr[same], berr[same], rtol=1e-12, atol=1e-300)
+    for i, g in enumerate(gs):
+        if same[i]:
+            np.testing.assert_array_equal(got[i], base[i])
+        ref = ref_pr(g, max_iterations=300, stop_rule=1, l1_tolerance=1e-6, init=1)
+        assert abs(int(gits[i]) - ref.iterations) <= 3
+        assert np.abs(got[i] - ref.ranks).max() <= FP32_TOL
     b.close()
 
 
@@ -872,8 +998,49 @@ def test_ppr_64_columns(device, rmat12, storage, pname):
         e = np.abs(ranks[:, c] - f64).max()
         worst = max(worst, e / max(f64.max(), 1e-30))
         assert e <= FP32_TOL, (c, e)
+        # L1 distance to the FP64 solution within 1e-5 in normalised units (the stop rule
+        # leaves at most 0.85e-6 / 0.15 = 5.7e-6)
+        assert np.abs(ranks[:, c].astype(np.float64) - f64).sum() <= 1e-5
+    # every column: the FP64 residual bounds the L1 distance to the FP64 solution
+    resid, mass = oracle.ppr_residual(g.out_degree, g.inv_idx, g.in_degree, g.inv_src, seeds,
+                                      ranks, threads=8)
+    bound = resid / 0.15
     say(f"  iterations={res.iterations} refine@{res.refine_iteration} "
-        f"max relative err vs FP64 {worst:.2e}")
+        f"max relative err vs FP64 {worst:.2e}; residual bound max {bound.max():.2e}")
+    assert np.all(bound <= 1e-5)
+    dg.close()
+
+
+def test_ppr_hip_graph_replays(device, rmat12):
+    """The captured PPR solve replayed several times, with eager solves in between, gives the
+    eager result every time; seeds, column count and storage change between replays without a
+    new capture, and a different iteration budget captures again."""
+    say("ppr hipGraph replays")
+    g = rmat12
+    dg = upload(device, g, "small")
+    rng = np.random.default_rng(12)
+    seeds = np.concatenate([[ag.PPR_UNIFORM], rng.choice(g.n, 63, replace=False)]).astype(np.uint32)
+    kw = dict(max_iterations=200, stop_rule=ag.STOP_L1, l1_tolerance=1e-6, refine_l1=1e-3)
+    for storage in (ag.STORE_BF16, ag.STORE_F32):
+        eager, ecol, eres = dg.ppr(seeds, storage=storage, use_hip_graph=0, **kw)
+        for rep in range(3):
+            got, gcol, gres = dg.ppr(seeds, storage=storage, use_hip_graph=1, **kw)
+            np.testing.assert_array_equal(got, eager)
+            np.testing.assert_array_equal(gcol, ecol)
+            assert gres.iterations == eres.iterations and gres.executed == eres.executed
+            assert gres.refine_iteration == eres.refine_iteration
+            again, _, _ = dg.ppr(seeds, storage=storage, use_hip_graph=0, **kw)
+            np.testing.assert_array_equal(again, eager)
+    few = seeds[5:9].copy()
+    eager, _, _ = dg.ppr(few, storage=ag.STORE_F16, use_hip_graph=0, **kw)
+    got, _, _ = dg.ppr(few, storage=ag.STORE_F16, use_hip_graph=1, **kw)
+    np.testing.assert_array_equal(got, eager)
+    got, _, r = dg.ppr(few, storage=ag.STORE_F32, use_hip_graph=1, max_iterations=7,
+                       stop_rule=ag.STOP_NONE)
+    eager, _, _ = dg.ppr(few, storage=ag.STORE_F32, use_hip_graph=0, max_iterations=7,
+                         stop_rule=ag.STOP_NONE)
+    assert r.executed == 7
+    np.testing.assert_array_equal(got, eager)
     dg.close()
 
 

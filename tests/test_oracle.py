@@ -208,6 +208,57 @@ def test_ppr_f64_matches_linear_solve(seed):
                                                    max_iterations=20000), rtol=1e-12)
 
 
+@pytest.mark.parametrize("seed", range(4))
+def test_ppr_residual_against_linear_solve(seed):
+    """The residual p + 0.85 P r - r (normalised units) of the exact solution (linear solve)
+    is at the rounding floor; of r = 0 it is p exactly (L1 = 0.15 for every restart); of a
+    perturbed solution r* + delta it is (I - 0.85 P) delta, so it lies within
+    [0.15, 1.85] * ||delta||_1 and bounds ||r - r*||_1 <= residual / 0.15. Several columns at
+    once, vertex-major."""
+    rng = np.random.default_rng(300 + seed)
+    n = int(rng.integers(3, 60))
+    m = int(rng.integers(n, 5 * n))
+    edges = sorted({(int(rng.integers(n)), int(rng.integers(n))) for _ in range(m)})
+    edges = [(s, d) for s, d in edges if s != d]
+    A = np.zeros((n, n))
+    od = np.zeros(n)
+    for s, d in edges:
+        A[d, s] += 1.0
+        od[s] += 1
+    P = A * np.where(od > 0, 1.0 / np.maximum(od, 1), 0.0)[None, :]
+    M = np.eye(n) - 0.85 * P
+    odeg, idx, deg, src = _csr(n, edges)
+    seeds = np.array([0xFFFFFFFF, 0, n - 1, int(rng.integers(n))], np.uint32)
+    cols = []
+    for sv in seeds:
+        p = np.full(n, 0.15 / n) if sv == 0xFFFFFFFF else 0.15 * (np.arange(n) == sv)
+        cols.append(np.linalg.solve(M, p))
+    exact = np.stack(cols, 1)
+    res, nrm = oracle.ppr_residual(odeg, idx, deg, src, seeds, exact.astype(np.float32))
+    # FP32 storage of the exact solution leaves a residual of a few FP32 ulps of the column
+    assert np.all(res <= 8 * 2.0 ** -24 * nrm)
+    np.testing.assert_allclose(nrm, np.abs(exact).sum(0), rtol=1e-6)
+    res0, nrm0 = oracle.ppr_residual(odeg, idx, deg, src, seeds, np.zeros((n, 4), np.float32))
+    np.testing.assert_allclose(res0, 0.15, rtol=1e-12)
+    assert np.all(nrm0 == 0)
+    delta = (rng.normal(size=(n, 4)) * 1e-3 / n).astype(np.float32)
+    r = (exact + delta).astype(np.float32)
+    res, _ = oracle.ppr_residual(odeg, idx, deg, src, seeds, r)
+    d1 = np.abs(r.astype(np.float64) - exact).sum(0)
+    want = np.abs(M @ (r.astype(np.float64) - exact)).sum(0)
+    np.testing.assert_allclose(res, want, rtol=1e-6, atol=1e-9 / n)
+    assert np.all(d1 <= res / 0.15 * (1 + 1e-6) + 1e-9 / n)
+
+
+def test_fixed_encode_array_equals_scalar():
+    rng = np.random.default_rng(8)
+    x = np.concatenate([np.array([0.0, -1.0, np.nan, np.inf, 0.5, 65535.996, 1e30], np.float32),
+                        (rng.random(5000) * 2.0 ** rng.integers(-20, 20, 5000)).astype(np.float32)])
+    for bits in (64, 32, 16, 8):
+        np.testing.assert_array_equal(oracle.fixed_encode_array(bits, x),
+                                      oracle.fixed_encode(bits, x))
+
+
 def test_openmp_matches_sequential():
     rng = np.random.default_rng(3)
     n = 3000

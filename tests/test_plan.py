@@ -20,7 +20,10 @@ class BlockPlan(C.Structure):
                 ("coff", u32p), ("cidx", u32p), ("lpos", C.POINTER(C.c_uint16)),
                 ("c_n_tiles", C.c_uint32), ("ct_row", u32p), ("ct_start", u32p),
                 ("ct_pre", u32p), ("c_n_items", C.c_uint32), ("c_item", u32p),
-                ("c_item_off", C.c_uint32 * 9), ("r_slot_blk0", C.c_uint32 * 6)]
+                ("c_item_off", C.c_uint32 * 9), ("r_slot_blk0", C.c_uint32 * 6),
+                ("hot_bits", C.c_uint32), ("n_hot", C.c_uint32), ("hot_edges", C.c_uint64),
+                ("hot_seg", u32p), ("hot_end", C.c_uint32 * 8), ("n_hgrp", C.c_uint32),
+                ("hgrp", u32p), ("hgrp_off", C.c_uint32 * 9)]
 
 
 class Plan(C.Structure):
@@ -97,7 +100,8 @@ def check_plan(g, p):
         assert len(segs) >= 1
         assert lo[segs[0]] == off[r] and hi[segs[-1]] == off[r + 1]
         for s in segs:
-            assert rows[s] == r and 0 < hi[s] - lo[s] <= prm.segment_nnz
+            assert rows[s] == r and 0 < hi[s] - lo[s]
+            assert hi[s] - lo[s] <= prm.segment_nnz or prm.fp32_order == 1
         for a, b in zip(segs[:-1], segs[1:]):
             assert hi[a] == lo[b]
     # launch slots: HUB, BLOCK, WAVE, GROUP16, THREAD
@@ -126,6 +130,28 @@ def test_plan_invariants_rmat(params):
         check_plan(g, p)
         if params:
             assert all(p.bin_row[b + 1] > p.bin_row[b] for b in range(5)), "every bin used"
+    finally:
+        ag.lib().ag_plan_free(C.byref(p))
+
+
+@pytest.mark.parametrize("params", [{}, SMALL, dict(src_block_bits=8, tile_nnz=64)],
+                         ids=["default", "small", "blocked-request"])
+def test_plan_reference_order(params):
+    """fp32_order = reference: rows up to tile_nnz in-edges are thread-bin rows (one lane
+    each), every longer row is a hub row of exactly one segment, the middle bins are empty and
+    source blocking is off, whatever the other parameters ask."""
+    g = ag.Graph.rmat(12)
+    st, p, err = build(g, ag.plan_params(**dict(params, fp32_order=1)))
+    assert st == 0, err
+    try:
+        prm = p.params
+        assert prm.fp32_order == 1 and prm.thread_max_deg == prm.tile_nnz
+        check_plan(g, p)
+        assert all(p.bin_row[b + 1] == p.bin_row[b] for b in (1, 2, 3))
+        assert p.n_segs == p.n_hubs and p.blk.bits == 0
+        deg = np.diff(view(p.off, g.n + 1).astype(np.int64))
+        assert np.all(deg[p.bin_row[4]:] > prm.tile_nnz)
+        assert np.all(deg[:p.bin_row[1]] <= prm.tile_nnz)
     finally:
         ag.lib().ag_plan_free(C.byref(p))
 
@@ -168,6 +194,23 @@ NXCD = 8
 ITEM_REG, ITEM_VTILE, ITEM_CTILE, ITEM_CBIG = 0, 1, 2, 3
 
 
+HOT_NNZ = 2048
+
+
+def block_keys(B, n):
+    """Source block of every device id: hot segment b for the ids of B.hot_seg[b], else the
+    ordinary 2^bits block after the hot ones."""
+    ids = np.arange(n, dtype=np.int64)
+    keys = B.n_hot + (ids >> B.bits)
+    if B.n_hot:
+        hs = view(B.hot_seg, B.n_hot).astype(np.int64)
+        seg = ids >> B.hot_bits
+        pos = np.searchsorted(hs, seg)
+        hit = (pos < hs.size) & (hs[np.minimum(pos, hs.size - 1)] == seg)
+        keys[hit] = pos[hit]
+    return keys
+
+
 def check_block_plan(g, p):
     """Every blocked row's in-edges appear exactly once among its virtual rows, grouped by
     source block in stream order; virtual rows are ordered (slice, block, row, piece); the
@@ -177,11 +220,32 @@ def check_block_plan(g, p):
     B, prm, n = p.blk, p.params, g.n
     bits, nb, row0, nv = B.bits, B.nblocks, B.row0, B.nv
     piece = min(PIECE, prm.tile_nnz)
+    hot_piece = min(HOT_NNZ, prm.tile_nnz)
     off = view(p.off, n + 1)
     srcd = view(p.src, p.m)
     br = list(p.bin_row)
     assert row0 == br[prm.blocked_min_bin] and B.mb == off[n] - off[row0]
-    assert nb == -(-n // (1 << bits)) and nb <= 1024
+    H = B.n_hot
+    assert nb == H + -(-n // (1 << bits)) and nb <= 256
+    blk_of = block_keys(B, n)   # source block of every device id
+    hot_seg = view(B.hot_seg, H)
+    if H:
+        assert B.hot_bits == min(14, bits) and np.all(np.diff(hot_seg.astype(np.int64)) > 0)
+        # the hot segments are the qualifying ones with the most in-edges
+        kb = B.hot_bits
+        rows = np.repeat(np.arange(n), np.diff(off.astype(np.int64)))[off[row0]:]
+        seg = srcd[off[row0]:].astype(np.int64) >> kb
+        E = np.bincount(seg, minlength=-(-n >> kb) or 1)
+        P = np.bincount(np.unique(rows * (1 << 20) + seg) % (1 << 20), minlength=E.size)
+        ok = (E >= (4 << kb)) & (2 * P <= E)
+        assert set(hot_seg.tolist()) <= set(np.nonzero(ok)[0].tolist())
+        assert B.hot_edges == E[hot_seg].sum()
+        if H < min(prm.hot_segments, 256 - (nb - H)):
+            assert ok.sum() == H
+        else:
+            assert E[hot_seg].min() >= E[ok & ~np.isin(np.arange(E.size), hot_seg)].max(initial=0)
+    else:
+        assert B.hot_bits == 0 and B.hot_edges == 0 and B.n_hgrp == 0
     sl = list(B.slice_row)
     assert sl[0] == row0 and sl[-1] == n and sl == sorted(sl)
     voff, vsrc = view(B.voff, nv + 1), view(B.vsrc, B.mb)
@@ -200,19 +264,21 @@ def check_block_plan(g, p):
         vrow[vs] = r
         seq = [vsrc[voff[v]:voff[v + 1]] for v in vs]
         for v, sq in zip(vs, seq):
-            blks = np.unique(sq >> bits)
+            blks = np.unique(blk_of[sq])
             assert blks.size == 1
             vblk[v] = blks[0]
         got = np.concatenate(seq) if seq else np.zeros(0, np.uint32)
         row = srcd[off[r]:off[r + 1]]
-        want = row[np.argsort(row >> bits, kind="stable")]
+        want = row[np.argsort(blk_of[row], kind="stable")]
         np.testing.assert_array_equal(got, want)
         # pieces: consecutive virtual rows of one block are full pieces except the last
         bl = vblk[vs]
         assert np.all(np.diff(bl) >= 0)
         for k in range(len(vs) - 1):
             if bl[k] == bl[k + 1]:
-                assert vlen[vs[k]] == piece and vs[k + 1] == vs[k] + 1
+                assert vlen[vs[k]] == (hot_piece if bl[k] < H else piece)
+                assert vs[k + 1] == vs[k] + 1
+        assert all(vlen[v] <= hot_piece for v in vs if vblk[v] < H)
     xs = np.searchsorted(np.array(sl[1:]), vrow, side="right")
     key = xs * nb + vblk
     assert np.all(np.diff(key) >= 0), "virtual rows ordered by (slice, block)"
@@ -225,12 +291,17 @@ def check_block_plan(g, p):
     covered = np.zeros(nv, np.int64)
     reg_total = B.r_slot_blk0[5]
     reg_seen = []
+    hg = view(B.hgrp, 3 * B.n_hgrp).reshape(-1, 3).astype(np.int64)
+    ho = list(B.hgrp_off)
+    assert ho[0] == 0 and ho[-1] == B.n_hgrp and ho == sorted(ho)
     for x in range(NXCD):
         last_blk = -1
-        for first, kc in it[io[x]:io[x + 1]]:
+        he = B.hot_end[x]
+        assert io[x] <= he <= io[x + 1]
+        for i, (first, kc) in enumerate(it[io[x]:io[x + 1]], io[x]):
             kind, cnt = kc & 15, kc >> 4
             if kind == ITEM_REG:
-                assert cnt == 0 and first % NXCD == x
+                assert cnt == 0 and first % NXCD == x and i >= he
                 reg_seen.append(first)
                 continue
             assert kind == ITEM_VTILE
@@ -240,7 +311,19 @@ def check_block_plan(g, p):
             assert np.all(xs[vs] == x) and np.unique(vblk[vs]).size == 1
             assert vblk[vs[0]] >= last_blk
             last_blk = vblk[vs[0]]
-            assert vlen[vs].sum() <= prm.tile_nnz
+            assert (vblk[vs[0]] < H) == (i < he)
+            assert vlen[vs].sum() <= (hot_piece if vblk[vs[0]] < H else prm.tile_nnz)
+        # the groups of slice x cover its hot items [io[x], he) once, in order, one segment
+        # each, at most 65536 edges
+        nxt = io[x]
+        for first, cnt, sg in hg[ho[x]:ho[x + 1]]:
+            assert first == nxt and cnt >= 1
+            nxt += cnt
+            vs = np.concatenate([np.arange(it[i, 0], it[i, 0] + (it[i, 1] >> 4))
+                                 for i in range(first, first + cnt)])
+            assert np.all(vblk[vs] == np.searchsorted(hot_seg, sg)) and hot_seg[vblk[vs[0]]] == sg
+            assert vlen[vs].sum() <= 65536
+        assert nxt == he
     assert np.all(covered == 1)
     assert sorted(reg_seen) == list(range(reg_total))
     # regular schedule of the unblocked rows: bins below blocked_min_bin only
@@ -298,7 +381,7 @@ def test_block_plan_invariants(min_bin, small):
         if row0 == g.n:   # no row in the blocked bins: nothing to block
             assert p.blk.bits == 0 and p.blk.nv == 0
             return
-        assert p.blk.bits == 8 and p.blk.nblocks == 16 and p.blk.nv > 0
+        assert p.blk.bits == 8 and p.blk.nblocks == 16 + p.blk.n_hot and p.blk.nv > 0
         check_plan(g, p)
         check_block_plan(g, p)
         if small:   # tiny tiles: some rows have more partials than a combine tile holds
@@ -370,7 +453,7 @@ def test_block_plan_weights_follow_sources():
                 continue
             osrc, ow = g.inv_src[lo:hi], w[lo:hi]
             dsrc = view(p.o2d, n)[osrc]
-            order = np.argsort(dsrc >> B.bits, kind="stable")
+            order = np.argsort(block_keys(B, n)[dsrc], kind="stable")
             got_s = np.concatenate([vsrc[voff[q]:voff[q + 1]] for q in cidx[coff[r]:coff[r + 1]]])
             got_w = np.concatenate([vw[voff[q]:voff[q + 1]] for q in cidx[coff[r]:coff[r + 1]]])
             np.testing.assert_array_equal(got_s, dsrc[order])
