@@ -167,6 +167,7 @@ SIGNATURES = {
     "ag_pr_last_step_kernel_ms": (C.c_int, [VP, P(f64), P(u32)]),
     "ag_spmv_pull": (C.c_int, [VP, P(f32), P(f32), u32, P(f64)]),
     "ag_spmv_pull_fixed": (C.c_int, [VP, P(u32), P(u64), u32, P(f64)]),
+    "ag_spmv_pull_noalloc": (C.c_int, [VP, P(f32), P(f32)]),
     "ag_spmv_push": (C.c_int, [VP, P(f32), P(f32), u32, P(f64)]),
     "ag_spmv_push_fixed": (C.c_int, [VP, P(u64), P(u64), u32, P(f64)]),
     "ag_pull_gather": (C.c_int, [VP, u32, VP, VP]),

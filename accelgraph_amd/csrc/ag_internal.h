@@ -27,6 +27,7 @@ struct ag_device {
     uint32_t poll_us;
     uint32_t live_graphs;   // graphs uploaded on this device and not yet freed
     uint32_t num_cus;       // compute units (persistent-grid sizing)
+    void *scratch;          // this device's copy of the ag_spmv_pull_noalloc scratch array
 };
 
 // Device-resident graph: the plan arrays in HBM plus launch geometry.

@@ -77,6 +77,15 @@ extern "C" int ag_device_count(int *count)
     return AG_OK;
 }
 
+// Scratch of the allocation-free entry point ag_spmv_pull_noalloc (permuted x and y), a
+// file-scope device array: every GPU the library runs on holds its own copy at its own
+// address until the process ends. Host code never uses the array's name as an address: the
+// address is taken once per device with hipGetSymbolAddress (that device current) when the
+// device is first opened, and kept per ordinal.
+static __device__ __align__(256) unsigned char scratch[AG_NOALLOC_SCRATCH_BYTES];
+#define AG_MAX_ORDINALS 64
+static void *g_scratch_ptr[AG_MAX_ORDINALS];
+
 extern "C" int ag_device_open(int ordinal, ag_device **out)
 {
     if (!out)
@@ -103,6 +112,16 @@ extern "C" int ag_device_open(int ordinal, ag_device **out)
         return ag_hip_fail(e, "hipDeviceGetAttribute(CU count)", __FILE__, __LINE__);
     }
     d->num_cus = (uint32_t)cus;
+    if (ordinal < AG_MAX_ORDINALS && !g_scratch_ptr[ordinal]) {
+        void *p = NULL;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            free(d);
+            return ag_hip_fail(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+        }
+        g_scratch_ptr[ordinal] = p;
+    }
+    d->scratch = ordinal < AG_MAX_ORDINALS ? g_scratch_ptr[ordinal] : NULL;
     e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         free(d);

@@ -310,6 +310,30 @@ extern "C" int ag_spmv_pull(ag_dgraph *g, const float *x, float *y, uint32_t rep
     return spmv_f32_device(g, x, y, xo, xd, yd, reps, avg_ms);
 }
 
+extern "C" int ag_spmv_pull_noalloc(ag_dgraph *g, const float *x, float *y)
+{
+    if (!g || (g->n && (!x || !y)))
+        return ag_set_error(AG_ERR_INVALID, "null argument");
+    const uint32_t n = g->n;
+    if (n == 0)
+        return AG_OK;
+    if ((uint64_t)n * 2u * sizeof(float) > (uint64_t)AG_NOALLOC_SCRATCH_BYTES)
+        return ag_set_error(AG_ERR_INVALID, "ag_spmv_pull_noalloc: %u vertices need %llu bytes "
+                            "of scratch, more than its fixed %u (at most %u vertices)", n,
+                            (unsigned long long)n * 2u * sizeof(float),
+                            (unsigned)AG_NOALLOC_SCRATCH_BYTES,
+                            (unsigned)(AG_NOALLOC_SCRATCH_BYTES / (2u * sizeof(float))));
+    if (!g->dev->scratch)
+        return ag_set_error(AG_ERR_UNSUPPORTED, "ag_spmv_pull_noalloc: no scratch on device %d",
+                            g->dev->ordinal);
+    AG_HIP(hipSetDevice(g->dev->ordinal));
+    // two halves of the scratch: x in original order, then (once permuted) y in device order;
+    // x in device order, then y in original order
+    float *a = (float *)g->dev->scratch;
+    float *b = a + n;
+    return spmv_f32_device(g, x, y, a, b, a, 1, NULL);
+}
+
 static uint32_t host_fixed32(float x)
 {
     double s = (double)x * 65536.0 + 0.5;

@@ -265,6 +265,17 @@ int ag_pr_last_step_kernel_ms(ag_pr_session *s, double *avg_ms, uint32_t *launch
  * the graph has no weights. Repeats the product `repetitions` times (each overwrites y). */
 int ag_spmv_pull(ag_dgraph *g, const float *x, float *y, uint32_t repetitions,
                  double *avg_kernel_ms);
+/* ag_spmv_pull without any allocation (no counterpart in the reference; for callers that
+ * cannot allocate device memory: serving loops, code under stream capture, harnesses that
+ * forbid it). One product y = A x, inputs and outputs in host memory as for ag_spmv_pull;
+ * the permuted copies of x and y live in a fixed device array of AG_NOALLOC_SCRATCH_BYTES
+ * per GPU (taken at ag_device_open, held until the process ends), so the graph's vertex
+ * count must satisfy 8 * num_vertices <= AG_NOALLOC_SCRATCH_BYTES (4,194,304 vertices);
+ * larger graphs return AG_ERR_INVALID. Every call on one GPU shares that array: calls that
+ * overlap on one GPU (several host threads) race. Nothing in the array needs clearing (x and
+ * y are written whole on every call). Waits like every other call (the watchdog applies). */
+#define AG_NOALLOC_SCRATCH_BYTES (32u << 20)
+int ag_spmv_pull_noalloc(ag_dgraph *g, const float *x, float *y);
 /* Fixed point: y_q[v] = sum of (u32)(((u64)x_q[u] * w_q) >> 16) modulo 2^64, with
  * w_q = FloatToFixed32(w) per edge, or the codes given to ag_graph_set_fixed_weights. */
 int ag_spmv_pull_fixed(ag_dgraph *g, const uint32_t *x_q, uint64_t *y_q,
