@@ -57,7 +57,7 @@ extern const uint32_t ag_slot_bin[AG_NSLOTS];
 #define AG_HOT_SEG_BITS 14u
 #define AG_HOT_NNZ 2048u
 #define AG_HOT_GROUP_NNZ 65536u
-#define AG_HOT_DEFAULT 128u
+#define AG_HOT_DEFAULT 0u
 
 /* work items of the blocked passes: item = {first, kind | count << 4} */
 enum { AG_ITEM_REG = 0,     /* pass 1: first = item of the unblocked rows' regular schedule */

@@ -103,7 +103,10 @@ typedef struct ag_plan_params {
                                   in-edges average at least two per (row, segment) piece and
                                   four per source) become blocks of their own, gathered by FP32
                                   PageRank and FP32 SpMV from a copy of the segment in LDS
-                                  instead of through L2 requests. Default 128; 0 = none      */
+                                  instead of through L2 requests. Default 0 (none): on
+                                  RMAT-25 128 of them made the FP32 iteration slower (the LDS
+                                  pass is latency-bound at 2 workgroups per CU and doubles the
+                                  partials the combine pass reads)                          */
     uint32_t fp32_order;       /* AG_ORDER_FAST (default): rows above the thread bin are
                                   summed in fixed tree / block orders. AG_ORDER_REFERENCE:
                                   every row is summed in sequence in inverse-CSR order, as

@@ -30,8 +30,9 @@ PARAMS = {"default": None, "small": SMALL, "small-order1": dict(SMALL, order=1),
           "tiles": dict(thread_max_deg=16, tile_rows=1024, tile_nnz=4096),
           # source blocking forced on small graphs: 2^10-source blocks, rows of in-degree > 16
           # blocked; every row blocked with 2^8-source blocks; small thresholds (every virtual
-          # row class and every combine bin populated)
-          "blocked": dict(src_block_bits=10, blocked_min_bin=1),
+          # row class and every combine bin populated); "blocked" also stages up to 128 hot
+          # source segments in LDS (the default plan stages none)
+          "blocked": dict(src_block_bits=10, blocked_min_bin=1, hot_segments=128),
           "blocked-all": dict(src_block_bits=8, blocked_min_bin=0),
           "blocked-small": dict(SMALL, src_block_bits=9, blocked_min_bin=1),
           # the same blocking without hot segments (every gather through L2)
@@ -616,14 +617,14 @@ def test_pagerank_directions_corners(device):
 
 def test_pagerank_blocked_auto_rmat22(device):
     """Above 2^21 vertices the default plan blocks sources (2^19 per block). RMAT-22: fixed32
-    bit-exact and FP32 within tolerance of the oracle for 20 iterations, and the blocked and
-    unblocked FP32 runs agree."""
+    bit-exact and FP32 within tolerance of the oracle for 20 iterations; the blocked, unblocked
+    and hot-segment (LDS-staged) FP32 runs all within the per-row FP32 bars."""
     say("pagerank blocked auto rmat-22")
     g = ag.Graph.rmat(22)
     dg = device.upload(g)
     info = dg.info()
     assert info.src_block_bits == 19 and info.virtual_rows > 0
-    assert info.hot_segments > 0 and info.src_blocks == 8 + info.hot_segments
+    assert info.hot_segments == 0 and info.src_blocks == 8
     kw = dict(max_iterations=20, stop_rule=ag.STOP_NONE)
     fx = dg.pagerank(kind=32, **kw)
     ref = ref_pr(g, kind=32, max_iterations=20, stop_rule=2, threads=8)
@@ -641,6 +642,16 @@ def test_pagerank_blocked_auto_rmat22(device):
     assert np.abs(run.ranks - run2.ranks).max() <= FP32_TOL
     f32_row_bars(plain, g, 20)
     plain.close()
+    hot = device.upload(g, params=ag.plan_params(hot_segments=128))
+    hinfo = hot.info()
+    assert hinfo.hot_segments > 0 and hinfo.src_blocks == 8 + hinfo.hot_segments
+    assert hinfo.hot_segment_bits == 14 and 0 < hinfo.hot_edges < g.m
+    fx3 = hot.pagerank(kind=32, **kw)
+    np.testing.assert_array_equal(fx3.ranks.view(np.uint32), fx.ranks.view(np.uint32))
+    run3 = hot.pagerank(**kw)
+    assert np.abs(run.ranks - run3.ranks).max() <= FP32_TOL
+    f32_row_bars(hot, g, 20)
+    hot.close()
 
 
 def test_rmat25_full_size(device):
@@ -653,8 +664,7 @@ def test_rmat25_full_size(device):
     assert g.n == 1 << 25 and g.m == 528_693_006
     dg = device.upload(g)
     info = dg.info()
-    assert info.src_block_bits == 19 and info.src_blocks == 64 + info.hot_segments
-    assert info.hot_segments > 0 and info.hot_segment_bits == 14
+    assert info.src_block_bits == 19 and info.src_blocks == 64 and info.hot_segments == 0
     kw = dict(max_iterations=4, stop_rule=ag.STOP_NONE)
     say("rmat-25 full size: fixed-32 pull")
     fx = dg.pagerank(kind=32, **kw)

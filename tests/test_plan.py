@@ -368,12 +368,14 @@ def check_block_plan(g, p):
     assert np.all(rows_seen[row0:] == 1) and np.all(rows_seen[:row0] == 0)
 
 
+@pytest.mark.parametrize("hot", [0, 128], ids=["nohot", "hot"])
 @pytest.mark.parametrize("min_bin", [0, 1, 3])
 @pytest.mark.parametrize("small", [False, True], ids=["default", "small"])
-def test_block_plan_invariants(min_bin, small):
+def test_block_plan_invariants(min_bin, small, hot):
     g = ag.Graph.rmat(12)
     base = dict(SMALL) if small else {}
-    prm = ag.plan_params(**dict(base, src_block_bits=8, blocked_min_bin=min_bin))
+    prm = ag.plan_params(**dict(base, src_block_bits=8, blocked_min_bin=min_bin,
+                                hot_segments=hot))
     st, p, err = build(g, prm)
     assert st == 0, err
     try:
@@ -382,6 +384,7 @@ def test_block_plan_invariants(min_bin, small):
             assert p.blk.bits == 0 and p.blk.nv == 0
             return
         assert p.blk.bits == 8 and p.blk.nblocks == 16 + p.blk.n_hot and p.blk.nv > 0
+        assert (p.blk.n_hot > 0) == (hot > 0)
         check_plan(g, p)
         check_block_plan(g, p)
         if small:   # tiny tiles: some rows have more partials than a combine tile holds
