@@ -24,6 +24,7 @@ AG_ERR_INVALID, AG_ERR_NO_DEVICE, AG_ERR_HIP, AG_ERR_OOM, AG_ERR_TIMEOUT, AG_ERR
 AG_NBINS = 5
 SRC_BLOCK_AUTO, SRC_BLOCK_OFF = 0, 255
 ORDER_FAST, ORDER_REFERENCE = 0, 1
+TILES_AUTO, TILES_ROUND_ROBIN, TILES_TICKET = 0, 1, 2
 # graphCSR.h GRAPH_FILE_*: the reference driver's -z codes
 FILE_TEXT_EDGE_LIST, FILE_BINARY_EDGE_LIST, FILE_CSR_SERIALIZED = 0, 1, 2
 BIN_NAMES = ["thread", "group16", "wave", "block", "hub"]
@@ -42,7 +43,8 @@ class PlanParams(C.Structure):
                 ("block_max_deg", u32), ("tile_rows", u32), ("tile_nnz", u32),
                 ("segment_nnz", u32), ("order", u32), ("src_block_bits", u32),
                 ("wg_per_cu", u32), ("blocked_min_bin", u32), ("hot_segments", u32),
-                ("fp32_order", u32)]
+                ("fp32_order", u32), ("tile_order", u32),
+                ("piece_nnz", u32)]
 
 
 class GraphInfo(C.Structure):

@@ -59,6 +59,8 @@ struct ag_dgraph {
     // schedule scalars and HBM arrays of the block plan
     ag_block_plan blk;   // pointer members are device pointers here
     uint64_t *vpart;     // [blk.nv] partial sum of every virtual row (float or u64 bits)
+    uint32_t *btick;     // [2 * AG_NXCD] pass-1 ticket counters (BlockGeom::tick), zeroed
+    uint32_t bdyn;       // pass-1 tiles handed out by ticket (BlockGeom::dyn)
     uint32_t *vwq;       // [blk.mb] FloatToFixed32(vw), built on the first fixed SpMV
     // forward CSR in device row ids (out-edges), built on the first push / data-driven run
     uint32_t *fwd_off;   // [n+1]
@@ -123,6 +125,11 @@ struct BlockGeom {
     uint32_t hgrp_off[AG_NXCD + 1];
     uint32_t hot_bits;
     uint32_t n_src;          // source (vertex) count: the last segment may be short
+    // pass 1 hands out its tiles through per-slice tickets (no AG_ITEM_REG item in pass 1):
+    // tick[x] = next ticket of slice x, tick[AG_NXCD + x] = its workgroups that have left;
+    // both zero between launches
+    uint32_t dyn;
+    uint32_t *tick;
 };
 
 // Points pass 1 of B past the hot-segment tiles (when the hot pass runs before it).

@@ -421,6 +421,13 @@ __device__ __forceinline__ void mixed_reduce(const uint32_t *bnd, uint32_t base,
 // vector, which that XCD's L2 holds. `pol` gathers through virtual-row edge positions (its
 // per-edge arrays, if any, are in virtual-row order); AG_ITEM_REG items run the unblocked
 // rows' regular schedule U with `upol`.
+// With B.dyn (never with an AG_ITEM_REG item in pass 1) the tiles are handed out in order
+// through a per-slice ticket counter, so the tiles in flight on an XCD stay consecutive -- a fixed
+// round-robin lets the workgroups drift apart by several blocks over an iteration, and the
+// XCD's gathers then spread over more of the vector than its L2 holds. A tile's result does
+// not depend on the workgroup that runs it. Each workgroup takes its next ticket while it
+// runs the current tile; the last workgroup of the slice to leave resets the counters for
+// the next launch.
 // LDS of the blocked passes beyond the tile values, declared once per kernel (a kernel that
 // instantiates a pass for several policies shares it).
 struct BlkLds {
@@ -428,7 +435,49 @@ struct BlkLds {
     uint32_t start[AG_MAX_SRC_BLOCKS];     // combine: the tile's first virtual row per block
     uint32_t lng[AG_LONG_MAX];             // rows of a tile summed by a wavefront
     uint32_t nlong;
+    uint32_t ticket[2];                    // pass 1 with tickets: the workgroup's next tile
 };
+
+// Partials of virtual rows [v0, v0 + nnz) that are single in-edges [e0, e0 + nnz): each is
+// its edge's gathered term, written straight out (no LDS, no barrier).
+template <class PP>
+__device__ __forceinline__ void gather_singletons(const uint32_t *src, uint32_t e0, uint32_t v0,
+                                                  uint32_t nnz, const PP &pp, Red &red)
+{
+    for (uint32_t k = threadIdx.x; k < nnz; k += AG_TILE_UNROLL * AG_WG) {
+        uint32_t sidx[AG_TILE_UNROLL];
+#pragma unroll
+        for (int u = 0; u < AG_TILE_UNROLL; u++) {
+            const uint32_t kk = k + u * AG_WG;
+            sidx[u] = kk < nnz ? ld_stream(&src[e0 + kk]) : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < AG_TILE_UNROLL; u++) {
+            const uint32_t kk = k + u * AG_WG;
+            if (kk < nnz)
+                pp.finish(v0 + kk, pp.edge(e0 + kk, sidx[u]), red);
+        }
+    }
+}
+
+// One pass-1 tile, virtual rows [first, last). When they are all single edges (as many
+// rows as edges) the tile is a plain gather; otherwise its edges are staged in LDS and
+// reduced row by row, ending with a barrier. Either way each partial is its piece's terms
+// added in element order, so the two forms agree bit for bit.
+template <class PP>
+__device__ __forceinline__ void blocked_tile(const BlockGeom &B, const PP &pp, Red &red,
+                                             typename PP::Acc *s_vals, BlkLds &L,
+                                             uint32_t first, uint32_t last)
+{
+    const uint32_t e0 = B.V.off[first], nnz = B.V.off[last] - e0;
+    if (nnz == last - first) {
+        gather_singletons(B.V.src, e0, first, nnz, pp, red);
+        return;
+    }
+    stage_edges(B.V.src, e0, nnz, pp, s_vals);
+    __syncthreads();
+    mixed_reduce(B.V.off, e0, first, last, pp, red, s_vals, L.lng, &L.nlong);
+}
 
 template <class P, class PU>
 __device__ __forceinline__ void pull_blocked_items(const PullGeom &U, const BlockGeom &B,
@@ -441,16 +490,45 @@ __device__ __forceinline__ void pull_blocked_items(const PullGeom &U, const Bloc
     const uint32_t x = blockIdx.x % AG_NXCD;
     const uint32_t step = gridDim.x / AG_NXCD;   // the grid is a multiple of AG_NXCD
     const uint32_t end = B.item_off[x + 1];
+    if (B.dyn) {
+        const uint32_t lo = B.item_lo[x], count = end - lo, tid = threadIdx.x;
+        uint32_t *tick = B.tick + x, *left = B.tick + AG_NXCD + x;
+        // the ticket of round k is read from L.ticket[k & 1] after the round's barrier, and
+        // the next one written to the other word, which no lane reads before the next barrier
+        uint32_t k = 0;
+        if (tid == 0)
+            L.ticket[0] = atomicAdd(tick, 1u);
+        __syncthreads();
+        uint32_t t = L.ticket[0];
+        while (t < count) {
+            uint32_t next = 0;
+            if (tid == 0)
+                next = atomicAdd(tick, 1u);   // in flight while this tile runs
+            const uint32_t i = lo + t;
+            const uint32_t first = B.item[2 * i], last = first + (B.item[2 * i + 1] >> 4);
+            blocked_tile(B, pp, red, s_vals, L, first, last);
+            k ^= 1u;
+            if (tid == 0)
+                L.ticket[k] = next;
+            __syncthreads();
+            t = L.ticket[k];
+        }
+        if (tid == 0) {
+            __threadfence();
+            // every workgroup of the slice has taken its last ticket once `left` reaches step
+            if (atomicAdd(left, 1u) == step - 1u) {
+                atomicExch(tick, 0u);
+                atomicExch(left, 0u);
+            }
+        }
+        return;
+    }
     for (uint32_t i = B.item_lo[x] + blockIdx.x / AG_NXCD; i < end; i += step) {
         const uint32_t first = B.item[2 * i], kc = B.item[2 * i + 1];
         if ((kc & 15u) == AG_ITEM_REG) {
             pull_one_item(U, upol, first, red, s_vals, s_wave);
         } else {
-            const uint32_t last = first + (kc >> 4);
-            const uint32_t e0 = B.V.off[first];
-            stage_edges(B.V.src, e0, B.V.off[last] - e0, pp, s_vals);
-            __syncthreads();
-            mixed_reduce(B.V.off, e0, first, last, pp, red, s_vals, L.lng, &L.nlong);
+            blocked_tile(B, pp, red, s_vals, L, first, first + (kc >> 4));
         }
     }
 }

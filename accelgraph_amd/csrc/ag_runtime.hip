@@ -229,7 +229,7 @@ extern "C" int ag_graph_free(ag_dgraph *g)
                     g->blk.voff, g->blk.vsrc, g->blk.vw, g->blk.item, g->blk.coff, g->blk.cidx,
                     g->blk.lpos, g->blk.ct_row, g->blk.ct_start, g->blk.ct_pre, g->blk.c_item,
                     g->blk.hgrp,
-                    g->vpart, g->vwq, g->fwd_off, g->fwd_src, g->fwd_dst,
+                    g->vpart, g->btick, g->vwq, g->fwd_off, g->fwd_src, g->fwd_dst,
                     g->fwd_w, g->pb_row, g->pb_pos, g->pb_dst};
     for (void *p : ptrs)
         if (p)
@@ -301,6 +301,8 @@ BlockGeom ag_make_block_geom(const ag_dgraph *g)
     K.hgrp = B.hgrp;
     K.hot_bits = B.hot_bits;
     K.n_src = g->n;
+    K.dyn = g->bdyn;
+    K.tick = g->btick;
     return K;
 }
 
@@ -450,6 +452,26 @@ extern "C" int ag_graph_upload(ag_device *dev, uint32_t n, uint32_t m, const uin
             if (he != hipSuccess)
                 st = ag_hip_fail(he, "partial-sum allocation", __FILE__, __LINE__);
             g->device_bytes += pb;
+        }
+        if (st == AG_OK) {
+            // tickets only when pass 1 holds tiles alone (no AG_ITEM_REG item, whose FP64
+            // error terms go to the running workgroup's partial); by default when the pieces
+            // are at least half the in-edges (sources spread evenly over the blocks: there a
+            // fixed round-robin lets the XCD's workgroups drift over more blocks than its L2
+            // holds; on skewed graphs it measured faster)
+            uint32_t reg = 0;
+            for (uint32_t i = 0; i < B.n_items; i++)
+                reg |= (B.item[2 * i + 1] & 15u) == AG_ITEM_REG;
+            const uint32_t order = g->params.tile_order;
+            g->bdyn = !reg && (order == AG_TILES_TICKET ||
+                               (order == AG_TILES_AUTO && 2ull * B.nv >= B.mb));
+            const size_t tb = 2 * AG_NXCD * sizeof(uint32_t);
+            he = hipMalloc((void **)&g->btick, tb);
+            if (he == hipSuccess)
+                he = hipMemsetAsync(g->btick, 0, tb, dev->stream);
+            if (he != hipSuccess)
+                st = ag_hip_fail(he, "ticket counter allocation", __FILE__, __LINE__);
+            g->device_bytes += tb;
         }
     }
     // the host plan arrays must outlive the asynchronous copies

@@ -102,6 +102,7 @@ typedef struct ag_block_plan {
     uint32_t n_hgrp;
     uint32_t *hgrp;         /* [3*n_hgrp]                                                */
     uint32_t hgrp_off[AG_NXCD + 1];
+    uint32_t piece;         /* most in-edges of one piece of an ordinary block            */
 } ag_block_plan;
 
 typedef struct ag_plan {

@@ -29,6 +29,8 @@ void ag_plan_params_default(ag_plan_params *p)
     p->blocked_min_bin = AG_BIN_THREAD;
     p->hot_segments = AG_HOT_DEFAULT;
     p->fp32_order = AG_ORDER_FAST;
+    p->tile_order = AG_TILES_AUTO;
+    p->piece_nnz = 0;
 }
 
 int ag_plan_params_check(const ag_plan_params *p, const char **err)
@@ -73,6 +75,14 @@ int ag_plan_params_check(const ag_plan_params *p, const char **err)
     }
     if (p->hot_segments > AG_MAX_SRC_BLOCKS) {
         *err = "hot_segments must be in [0, 256]";
+        return AG_ERR_INVALID;
+    }
+    if (p->tile_order > AG_TILES_TICKET) {
+        *err = "tile_order must be AG_TILES_AUTO, AG_TILES_ROUND_ROBIN or AG_TILES_TICKET";
+        return AG_ERR_INVALID;
+    }
+    if (p->piece_nnz > p->tile_nnz) {
+        *err = "piece_nnz must be 0 (auto) or in [1, tile_nnz]";
         return AG_ERR_INVALID;
     }
     if (p->fp32_order > AG_ORDER_REFERENCE) {
@@ -489,7 +499,8 @@ static int build_block_plan(ag_plan *p, const char **err)
     const uint32_t e0 = p->off[row0], mb = p->off[n] - e0;
     const uint32_t ncold = (uint32_t)((((uint64_t)n) + (1ull << bits) - 1) >> bits);
     const ag_plan_params *q = &p->params;
-    const uint32_t piece = q->tile_nnz < AG_PIECE_NNZ ? q->tile_nnz : AG_PIECE_NNZ;
+    const uint32_t piece = q->piece_nnz ? q->piece_nnz
+                                        : (q->tile_nnz < AG_PIECE_NNZ ? q->tile_nnz : AG_PIECE_NNZ);
     const uint32_t hot_nnz = q->tile_nnz < AG_HOT_NNZ ? q->tile_nnz : AG_HOT_NNZ;
     int oom = 0;
 
@@ -529,7 +540,7 @@ static int build_block_plan(ag_plan *p, const char **err)
             kmap[sg] = H + (sg >> (bits - kb));
     const uint32_t nb = H + ncold;
     const uint32_t nK = AG_NXCD * nb;
-    const blk_key K = {kmap, kb, H, piece, hot_nnz};
+    blk_key K = {kmap, kb, H, piece, hot_nnz};
 
     B->bits = bits;
     B->nblocks = nb;
@@ -585,6 +596,19 @@ static int build_block_plan(ag_plan *p, const char **err)
     }
     {
         uint64_t acc = 0;
+        for (uint32_t r = row0; r < n; r++)
+            acc += nvr[r - row0];
+        /* automatic piece limit: when the pieces are nearly as many as the in-edges (sources
+         * spread evenly over the blocks), every in-edge becomes a piece of its own, so that
+         * the first pass is a plain gather with no per-piece offsets to read */
+        if (!q->piece_nnz && K.piece > 1 && acc * 4 >= (uint64_t)mb * 3) {
+            K.piece = 1;
+            #pragma omp parallel for schedule(dynamic, 256)
+            for (uint32_t r = row0; r < n; r++)
+                nvr[r - row0] = row_pieces(ss + (p->off[r] - e0), p->off[r + 1] - p->off[r],
+                                           &K, NULL, NULL, NULL);
+        }
+        acc = 0;
         for (uint32_t r = 0; r <= row0; r++)
             B->coff[r] = 0;
         for (uint32_t r = row0; r < n; r++) {
@@ -592,6 +616,7 @@ static int build_block_plan(ag_plan *p, const char **err)
             B->coff[r + 1] = (uint32_t)acc;
         }
         B->nv = (uint32_t)acc;
+        B->piece = K.piece;
     }
     free(nvr);
     const uint32_t nv = B->nv;

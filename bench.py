@@ -424,7 +424,7 @@ def config1_section(ag, dev, cpu: bool) -> dict:
     return out
 
 
-def er_section(ag, dev, scale: int, steps: int, cpu: bool) -> dict:
+def er_section(ag, dev, scale: int, steps: int, cpu: bool, pkw: dict) -> dict:
     """PageRank-pull FP32 iteration on an Erdos-Renyi graph with the bench graph's vertex
     count and 16n edge draws (uniform sources: no hot set, every gather is a cold random
     read), and the solve to L1 <= 1e-6 from the uniform start with its error against the FP64
@@ -432,7 +432,7 @@ def er_section(ag, dev, scale: int, steps: int, cpu: bool) -> dict:
     import numpy as np
     n = 1 << scale
     g = ag.Graph.erdos_renyi_gpu(dev, n, 16 * n)
-    dg = dev.upload(g)
+    dg = dev.upload(g, params=ag.plan_params(**pkw) if pkw else None)
     sess = dg.session(max_iterations=steps + 3, stop_rule=ag.STOP_NONE, init=ag.INIT_UNIFORM)
     sess.step(3)
     dev.sync()
@@ -488,6 +488,11 @@ def main() -> int:
     ap.add_argument("--no-er", action="store_true", help="skip the ER PageRank iteration")
     ap.add_argument("--hot-segments", type=int, default=-1,
                     help="plan parameter hot_segments for the bench graph (-1 = the default)")
+    ap.add_argument("--tile-order", type=int, default=0,
+                    help="plan parameter tile_order for the bench graphs (0 auto, 1 round-robin, "
+                         "2 ticket)")
+    ap.add_argument("--piece-nnz", type=int, default=0,
+                    help="plan parameter piece_nnz for the bench graphs (0 = auto)")
     ap.add_argument("--no-rows", action="store_true",
                     help="skip the fixed-32, quantised, push and data-driven rows")
     args = ap.parse_args()
@@ -534,8 +539,14 @@ def main() -> int:
         log(f"graph construction: {construction}")
     log(f"device copy bandwidth {copy_gbs:.0f} GB/s (1 GiB, read + write)")
     t0 = time.perf_counter()
-    dg = dev.upload(g, params=(ag.plan_params(hot_segments=args.hot_segments)
-                               if args.hot_segments >= 0 else None))
+    pkw = {}
+    if args.hot_segments >= 0:
+        pkw["hot_segments"] = args.hot_segments
+    if args.tile_order:
+        pkw["tile_order"] = args.tile_order
+    if args.piece_nnz:
+        pkw["piece_nnz"] = args.piece_nnz
+    dg = dev.upload(g, params=ag.plan_params(**pkw) if pkw else None)
     info = dg.info()
     log(f"uploaded + planned in {time.perf_counter() - t0:.1f} s; bins rows="
         f"{list(info.bin_rows)} edges={list(info.bin_edges)} tiles={info.num_tiles} "
@@ -668,7 +679,7 @@ def main() -> int:
         log(f"config 1: {out['config1_rmat16']} ({time.perf_counter() - t0:.1f} s)")
     if not args.no_er and args.graph == "rmat":
         t0 = time.perf_counter()
-        out["er_pagerank"] = er_section(ag, dev, args.scale, 20, not args.no_cpu)
+        out["er_pagerank"] = er_section(ag, dev, args.scale, 20, not args.no_cpu, pkw)
         log(f"er: {out['er_pagerank']} ({time.perf_counter() - t0:.1f} s)")
     dev.close()
     if not args.no_cpu:

@@ -116,9 +116,25 @@ typedef struct ag_plan_params {
                                   in-edges as thread-bin rows (one lane each, from LDS), every
                                   longer row is one hub segment added by one lane in chunks
                                   staged through LDS, and source blocking is off             */
+    uint32_t tile_order;       /* order in which the workgroups of an XCD take the first-pass
+                                  tiles of a blocked iteration (every row blocked).
+                                  AG_TILES_AUTO (default): by ticket when the pieces number
+                                  at least half the in-edges (sources spread evenly, as in an
+                                  Erdos-Renyi graph), else round-robin. AG_TILES_ROUND_ROBIN:
+                                  workgroup j takes tiles j, j + G, ... AG_TILES_TICKET: in
+                                  order through a counter, so the tiles in flight stay
+                                  consecutive (inside about one source block). The results
+                                  are the same bit for bit in every order                  */
+    uint32_t piece_nnz;        /* with source blocking, the most in-edges of one piece (the
+                                  (row, source block) run a first-pass partial sums). 0 =
+                                  auto: min(tile_nnz, 4096), or 1 when that leaves at least
+                                  3/4 as many pieces as in-edges (then every in-edge is its
+                                  own piece and the first pass is a plain gather); else
+                                  1..tile_nnz                                                */
 } ag_plan_params;
 
 enum { AG_ORDER_FAST = 0, AG_ORDER_REFERENCE = 1 };
+enum { AG_TILES_AUTO = 0, AG_TILES_ROUND_ROBIN = 1, AG_TILES_TICKET = 2 };
 
 #define AG_SRC_BLOCK_AUTO 0u
 #define AG_SRC_BLOCK_OFF 255u

@@ -36,8 +36,11 @@ PARAMS = {"default": None, "small": SMALL, "small-order1": dict(SMALL, order=1),
           "blocked-all": dict(src_block_bits=8, blocked_min_bin=0),
           "blocked-small": dict(SMALL, src_block_bits=9, blocked_min_bin=1),
           # the same blocking without hot segments (every gather through L2)
-          "blocked-nohot": dict(src_block_bits=10, blocked_min_bin=1, hot_segments=0)}
-BLOCKED = ["blocked", "blocked-all", "blocked-small", "blocked-nohot"]
+          "blocked-nohot": dict(src_block_bits=10, blocked_min_bin=1, hot_segments=0),
+          # every in-edge its own piece (the first pass a plain gather), tiles by ticket
+          "blocked-edge": dict(src_block_bits=9, blocked_min_bin=0, piece_nnz=1,
+                               tile_order=ag.TILES_TICKET)}
+BLOCKED = ["blocked", "blocked-all", "blocked-small", "blocked-nohot", "blocked-edge"]
 
 
 def say(*a):
@@ -636,6 +639,11 @@ def test_pagerank_blocked_auto_rmat22(device):
     assert err <= FP32_TOL
     f32_row_bars(dg, g, 20)
     dg.close()
+    # pass-1 tiles by ticket instead of round-robin: the same ranks bit for bit
+    fixed = device.upload(g, params=ag.plan_params(tile_order=ag.TILES_TICKET))
+    np.testing.assert_array_equal(fixed.pagerank(**kw).ranks.view(np.uint32),
+                                  run.ranks.view(np.uint32))
+    fixed.close()
     plain = device.upload(g, params=ag.plan_params(src_block_bits=ag.SRC_BLOCK_OFF))
     assert plain.info().src_block_bits == 0
     run2 = plain.pagerank(**kw)

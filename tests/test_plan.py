@@ -23,7 +23,7 @@ class BlockPlan(C.Structure):
                 ("c_item_off", C.c_uint32 * 9), ("r_slot_blk0", C.c_uint32 * 6),
                 ("hot_bits", C.c_uint32), ("n_hot", C.c_uint32), ("hot_edges", C.c_uint64),
                 ("hot_seg", u32p), ("hot_end", C.c_uint32 * 8), ("n_hgrp", C.c_uint32),
-                ("hgrp", u32p), ("hgrp_off", C.c_uint32 * 9)]
+                ("hgrp", u32p), ("hgrp_off", C.c_uint32 * 9), ("piece", C.c_uint32)]
 
 
 class Plan(C.Structure):
@@ -219,7 +219,13 @@ def check_block_plan(g, p):
     tile's per-block runs hold exactly its rows' partials and lpos places them row-major."""
     B, prm, n = p.blk, p.params, g.n
     bits, nb, row0, nv = B.bits, B.nblocks, B.row0, B.nv
-    piece = min(PIECE, prm.tile_nnz)
+    piece = B.piece
+    if prm.piece_nnz:
+        assert piece == prm.piece_nnz
+    else:   # the default limit, or single in-edges when pieces are nearly as many as edges
+        assert piece in (min(PIECE, prm.tile_nnz), 1)
+        if piece == 1 and min(PIECE, prm.tile_nnz) > 1:
+            assert B.n_hot or B.nv == B.mb
     hot_piece = min(HOT_NNZ, prm.tile_nnz)
     off = view(p.off, n + 1)
     srcd = view(p.src, p.m)
@@ -390,6 +396,42 @@ def test_block_plan_invariants(min_bin, small, hot):
         if small:   # tiny tiles: some rows have more partials than a combine tile holds
             kinds = set(view(p.blk.c_item, 2 * p.blk.c_n_items)[1::2] & 15)
             assert {ITEM_CTILE, ITEM_CBIG} <= kinds
+    finally:
+        ag.lib().ag_plan_free(C.byref(p))
+
+
+@pytest.mark.parametrize("piece", [1, 5])
+def test_block_plan_piece_limit(piece):
+    g = ag.Graph.rmat(12)
+    st, p, err = build(g, ag.plan_params(src_block_bits=8, blocked_min_bin=0, piece_nnz=piece))
+    assert st == 0, err
+    try:
+        assert p.blk.piece == piece
+        if piece == 1:
+            assert p.blk.nv == p.blk.mb
+        check_plan(g, p)
+        check_block_plan(g, p)
+    finally:
+        ag.lib().ag_plan_free(C.byref(p))
+
+
+def test_block_plan_auto_piece_on_even_sources():
+    """Sources spread evenly over the blocks (Erdos-Renyi, ~16 in-edges over 64 blocks):
+    nearly one piece per in-edge, so the automatic limit makes every in-edge a piece; an RMAT
+    graph of the same size keeps the default limit."""
+    n = 1 << 14
+    g = ag.Graph.erdos_renyi(n, 16 * n)
+    st, p, err = build(g, ag.plan_params(src_block_bits=8, blocked_min_bin=0))
+    assert st == 0, err
+    try:
+        assert p.blk.piece == 1 and p.blk.nv == p.blk.mb
+        check_block_plan(g, p)
+    finally:
+        ag.lib().ag_plan_free(C.byref(p))
+    st, p, err = build(ag.Graph.rmat(14), ag.plan_params(src_block_bits=8, blocked_min_bin=0))
+    assert st == 0, err
+    try:
+        assert p.blk.piece == PIECE and p.blk.nv * 4 < p.blk.mb * 3
     finally:
         ag.lib().ag_plan_free(C.byref(p))
 
