@@ -329,13 +329,34 @@ __global__ __launch_bounds__(AG_FIN_WG) void ppr_finish_kernel(PullGeom G, PprAr
         ppr_finish(A, m, G.bin_row[AG_BIN_HUB] + hub, q, t, seed, err);
     }
     ppr_column_errors<PPR_FIN_GROUPS>(err, s_err, s_tot);
-    if (tid < PPR_W) {
-        double t = s_tot[tid];
-        for (uint32_t b = 0; b < A.nparts; b++)
-            t += A.part[(size_t)b * PPR_W + tid];
-        s_tot[tid] = t;
+    // the iteration kernel's per-workgroup column errors: lane (chunk j, column c) adds the
+    // workgroups of chunk j in order (8 loads in flight), then each column adds its hub term
+    // and the chunks in chunk order -- a fixed order for any grid
+    {
+        constexpr uint32_t NCH = AG_FIN_WG / PPR_W, U = 8;
+        const uint32_t c = tid % PPR_W, j = tid / PPR_W;
+        const uint32_t len = (A.nparts + NCH - 1) / NCH;
+        const uint32_t b0 = j * len, b1 = b0 + len < A.nparts ? b0 + len : A.nparts;
+        double t = 0.0;
+        for (uint32_t b = b0; b < b1; b += U) {
+            double v[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++)
+                v[u] = b + u < b1 ? A.part[(size_t)(b + u) * PPR_W + c] : 0.0;
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++)
+                t += v[u];
+        }
+        s_err[j * PPR_W + c] = t;   // s_err is free again (ppr_column_errors ends synced)
+        __syncthreads();
+        if (tid < PPR_W) {
+            double tot = s_tot[tid];
+            for (uint32_t q2 = 0; q2 < NCH; q2++)
+                tot += s_err[q2 * PPR_W + tid];
+            s_tot[tid] = tot;
+        }
+        __syncthreads();
     }
-    __syncthreads();
     if (tid != 0)
         return;
     double E = 0.0;
