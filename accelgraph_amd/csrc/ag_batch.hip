@@ -204,7 +204,24 @@ __global__ void batch_fin_kernel(BatchArgs A, uint32_t it)
     double E = 0.0;
     uint32_t act = 0;
     const size_t pb = (size_t)(it & 1u) * A.nparts;
-    for (uint32_t p = A.gpart[2 * g]; p < A.gpart[2 * g + 1]; p++) {
+    const uint32_t p1 = A.gpart[2 * g + 1];
+    uint32_t p = A.gpart[2 * g];
+    // part order, eight loads in flight
+    for (; p + 8u <= p1; p += 8u) {
+        double e[8];
+        uint32_t a[8];
+#pragma unroll
+        for (uint32_t u = 0; u < 8u; u++) {
+            e[u] = A.perr[pb + p + u];
+            a[u] = A.pact[pb + p + u];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 8u; u++) {
+            E += e[u];
+            act += a[u];
+        }
+    }
+    for (; p < p1; p++) {
         E += A.perr[pb + p];
         act += A.pact[pb + p];
     }

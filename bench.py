@@ -493,6 +493,8 @@ def main() -> int:
                          "2 ticket)")
     ap.add_argument("--piece-nnz", type=int, default=0,
                     help="plan parameter piece_nnz for the bench graphs (0 = auto)")
+    ap.add_argument("--src-block-bits", type=int, default=0,
+                    help="plan parameter src_block_bits for the bench graphs (0 = auto)")
     ap.add_argument("--no-rows", action="store_true",
                     help="skip the fixed-32, quantised, push and data-driven rows")
     args = ap.parse_args()
@@ -546,6 +548,8 @@ def main() -> int:
         pkw["tile_order"] = args.tile_order
     if args.piece_nnz:
         pkw["piece_nnz"] = args.piece_nnz
+    if args.src_block_bits:
+        pkw["src_block_bits"] = args.src_block_bits
     dg = dev.upload(g, params=ag.plan_params(**pkw) if pkw else None)
     info = dg.info()
     log(f"uploaded + planned in {time.perf_counter() - t0:.1f} s; bins rows="
