@@ -274,6 +274,29 @@ static uint32_t row_pieces(const uint32_t *ss, uint32_t d, const blk_key *K, uin
         (k)++;                                                              \
     } while (0)
 
+/* L2 lines of 32 sources (128 bytes of FP32 contributions) that a row's pieces gather: per
+ * piece one plus the changes of line along it (a piece's sources are nearly in order, so this
+ * is about its distinct lines). A wavefront's gather issues one L2 request per distinct line,
+ * and on skewed graphs this, not the edge count, sets a tile's cost. */
+static uint32_t row_lines(const uint32_t *ss, uint32_t d, const blk_key *K)
+{
+    uint32_t cnt = 0, i = 0;
+    while (i < d) {
+        uint32_t b = key_of(K, ss[i]), j = i + 1;
+        while (j < d && key_of(K, ss[j]) == b)
+            j++;
+        const uint32_t piece = b < K->n_hot ? K->hot_piece : K->piece;
+        for (uint32_t a = i; a < j; a += piece) {
+            const uint32_t e = j - a > piece ? a + piece : j;
+            cnt++;
+            for (uint32_t k = a + 1; k < e; k++)
+                cnt += (ss[k] >> 5) != (ss[k - 1] >> 5);
+        }
+        i = j;
+    }
+    return cnt;
+}
+
 /* Pass-1 items of every slice: per block, greedy tiles of consecutive virtual rows (at most
  * tile_nnz edges, so at most that many virtual rows; at most hot_nnz in a hot segment), then
  * the slice's share of the unblocked rows' items. The hot segments' tiles come first; runs
@@ -553,10 +576,12 @@ static int build_block_plan(ag_plan *p, const char **err)
     float *sw = p->w ? (float *)zalloc(mb, sizeof(float)) : NULL;
     B->coff = (uint32_t *)zalloc((size_t)n + 1, sizeof(uint32_t));
     uint32_t *nvr = (uint32_t *)zalloc((size_t)(n - row0), sizeof(uint32_t));
-    if (!ss || (p->w && !sw) || !B->coff || !nvr) {
+    uint32_t *lnr = (uint32_t *)zalloc((size_t)(n - row0), sizeof(uint32_t));
+    if (!ss || (p->w && !sw) || !B->coff || !nvr || !lnr) {
         free(ss);
         free(sw);
         free(nvr);
+        free(lnr);
         free(kmap);
         *err = "host allocation failed";
         return AG_ERR_OOM;
@@ -583,6 +608,7 @@ static int build_block_plan(ag_plan *p, const char **err)
             sort_row_by_block(p->src + p->off[r], p->w ? p->w + p->off[r] : NULL, d, &K,
                               ss + o, sw ? sw + o : NULL, keys);
             nvr[r - row0] = row_pieces(ss + o, d, &K, NULL, NULL, NULL);
+            lnr[r - row0] = row_lines(ss + o, d, &K);
         }
         free(keys);
     }
@@ -590,6 +616,7 @@ static int build_block_plan(ag_plan *p, const char **err)
         free(ss);
         free(sw);
         free(nvr);
+        free(lnr);
         free(kmap);
         *err = "host allocation failed";
         return AG_ERR_OOM;
@@ -604,9 +631,11 @@ static int build_block_plan(ag_plan *p, const char **err)
         if (!q->piece_nnz && K.piece > 1 && acc * 4 >= (uint64_t)mb * 3) {
             K.piece = 1;
             #pragma omp parallel for schedule(dynamic, 256)
-            for (uint32_t r = row0; r < n; r++)
-                nvr[r - row0] = row_pieces(ss + (p->off[r] - e0), p->off[r + 1] - p->off[r],
-                                           &K, NULL, NULL, NULL);
+            for (uint32_t r = row0; r < n; r++) {
+                const uint32_t *rs = ss + (p->off[r] - e0), d = p->off[r + 1] - p->off[r];
+                nvr[r - row0] = row_pieces(rs, d, &K, NULL, NULL, NULL);
+                lnr[r - row0] = row_lines(rs, d, &K);
+            }
         }
         acc = 0;
         for (uint32_t r = 0; r <= row0; r++)
@@ -620,17 +649,33 @@ static int build_block_plan(ag_plan *p, const char **err)
     }
     free(nvr);
     const uint32_t nv = B->nv;
-    /* slices: contiguous blocked rows of about equal pass-1 cost, in-edges + 2 x virtual
-     * rows (a virtual row costs its offset read and partial write, about two gathers) */
+    /* slices: contiguous blocked rows of about equal pass-1 cost, 10 per in-edge (its source
+     * id and LDS traffic), 17 per gathered line (an L2 request) and 11 per virtual row (its
+     * offset reads and partial write) -- weights fitted to the measured pass-1 time of each
+     * slice of RMAT-25 (within 3%); an edge count alone left the slice of light rows 30%
+     * faster than the others */
     {
-        const uint64_t total = (uint64_t)mb + 2ull * nv;
+        uint64_t *cpre = (uint64_t *)malloc(((size_t)(n - row0) + 1) * sizeof(uint64_t));
+        if (!cpre) {
+            free(lnr);
+            free(ss);
+            free(sw);
+            free(kmap);
+            *err = "host allocation failed";
+            return AG_ERR_OOM;
+        }
+        cpre[0] = 0;
+        for (uint32_t r = row0; r < n; r++)
+            cpre[r - row0 + 1] = cpre[r - row0] + 10ull * (p->off[r + 1] - p->off[r]) +
+                                 17ull * lnr[r - row0] + 11ull * (B->coff[r + 1] - B->coff[r]);
+        const uint64_t total = cpre[n - row0];
         B->slice_row[0] = row0;
         for (uint32_t x = 1; x < AG_NXCD; x++) {
             const uint64_t target = total * x / AG_NXCD;
             uint32_t lo = B->slice_row[x - 1], hi = n;
             while (lo < hi) {
                 uint32_t mid = lo + (hi - lo) / 2;
-                if ((uint64_t)(p->off[mid] - e0) + 2ull * B->coff[mid] < target)
+                if (cpre[mid - row0] < target)
                     lo = mid + 1;
                 else
                     hi = mid;
@@ -638,7 +683,9 @@ static int build_block_plan(ag_plan *p, const char **err)
             B->slice_row[x] = lo;
         }
         B->slice_row[AG_NXCD] = n;
+        free(cpre);
     }
+    free(lnr);
 
     /* 2: every virtual row in (row, block, piece) slot order: key, start in ss, length, row */
     uint32_t *vkey = (uint32_t *)zalloc(nv, sizeof(uint32_t));
