@@ -627,7 +627,8 @@ __device__ __forceinline__ void pull_hot_items(const BlockGeom &B, const P &pol,
 
 // Second pass. A combine tile (rows [r0, r1) of one slice) streams its partials block by
 // block -- for each source block they are one contiguous run of virtual rows -- into LDS at
-// their row-major slots (lpos), then each row adds its partials in block order and applies
+// their row-major slots (lpos, stored in this stream order, so read contiguously), then each
+// row adds its partials in block order and applies
 // the update. A row with more partials than a tile holds is summed by a workgroup through
 // cidx. The items are spread over the whole grid: a slice's cost here is per row, and the
 // slices are balanced for the first pass, not for this one.
@@ -667,7 +668,7 @@ __device__ __forceinline__ void pull_combine_items(const BlockGeom &B, const P &
                     hi = mid;
             }
             const uint32_t v = s_start[lo] + (k - s_pre[lo]);
-            s_vals[B.lpos[v]] = cp.part[v];
+            s_vals[B.lpos[p0 + k]] = cp.part[v];
         }
         __syncthreads();
         mixed_reduce(B.C.off, p0, r0, r1, cp, red, s_vals, L.lng, &L.nlong);

@@ -37,7 +37,8 @@ extern const uint32_t ag_slot_bin[AG_NSLOTS];
  * order, so at any time the gathers of one XCD fall in one block of the contribution vector.
  * The combine pass cuts each slice's rows into tiles; a tile streams its partials block by
  * block (rows are sorted inside a block, so they are contiguous), drops each into LDS at its
- * row-major position lpos, and adds every row's partials in block order. */
+ * row-major position (lpos, stored in the order the tile streams its partials, so that it is
+ * read contiguously), and adds every row's partials in block order. */
 #define AG_NXCD 8u
 #define AG_PIECE_NNZ 4096u
 /* source blocks of one plan: the combine keeps two per-block words of its tile in LDS, so this
@@ -81,7 +82,9 @@ typedef struct ag_block_plan {
     uint32_t item_off[AG_NXCD + 1]; /* items of slice x: [item_off[x], item_off[x+1])   */
     uint32_t *coff;         /* [n+1]  partial slots of each row (empty below row0)      */
     uint32_t *cidx;         /* [nv]   virtual row of each slot, block order within a row */
-    uint16_t *lpos;         /* [nv]   slot of each virtual row relative to its combine tile */
+    uint16_t *lpos;         /* [nv]   lpos[p0 + k]: row-major slot, relative to the tile, of
+                               the k-th partial a combine tile streams (p0 = coff of its first
+                               row); slots of rows combined through cidx are unused         */
     uint32_t c_n_tiles;
     uint32_t *ct_row;       /* [2*c_n_tiles] first and end row of each combine tile     */
     uint32_t *ct_start;     /* [c_n_tiles*nblocks] first virtual row of the tile in a block */

@@ -805,7 +805,8 @@ static int build_block_plan(ag_plan *p, const char **err)
     }
 
     /* 5: combine tiles: per block, the tile's rows' virtual rows are one contiguous run of
-     * the (slice, block) list; lpos places each partial at its row-major slot in the tile */
+     * the (slice, block) list; the tile streams its partials block by block, and the k-th of
+     * them goes to row-major slot lpos[p0 + k] of the tile (p0 = its first slot) */
     {
         uint32_t tiles = 0;
         B->c_n_items = emit_combine(p, NULL, &tiles, NULL, NULL);
@@ -819,6 +820,12 @@ static int build_block_plan(ag_plan *p, const char **err)
             goto done;
         }
         emit_combine(p, B->ct_row, &tiles, B->c_item, B->c_item_off);
+        /* row-major slot of each virtual row inside its tile (each tile fills its own) */
+        uint16_t *vslot = (uint16_t *)zalloc(nv, sizeof(uint16_t));
+        if (!vslot) {
+            oom = 1;
+            goto done;
+        }
         #pragma omp parallel for schedule(dynamic, 64)
         for (uint32_t t = 0; t < tiles; t++) {
             const uint32_t r0 = B->ct_row[2 * t], r1 = B->ct_row[2 * t + 1];
@@ -838,8 +845,16 @@ static int build_block_plan(ag_plan *p, const char **err)
             const uint32_t p0 = B->coff[r0];
             for (uint32_t r = r0; r < r1; r++)
                 for (uint32_t j = B->coff[r]; j < B->coff[r + 1]; j++)
-                    B->lpos[B->cidx[j]] = (uint16_t)(j - p0);
+                    vslot[B->cidx[j]] = (uint16_t)(j - p0);
+            for (uint32_t b = 0; b < nb; b++) {
+                const uint32_t s = B->ct_start[(size_t)t * nb + b];
+                const uint32_t k0 = B->ct_pre[(size_t)t * (nb + 1) + b];
+                const uint32_t k1 = B->ct_pre[(size_t)t * (nb + 1) + b + 1];
+                for (uint32_t k = k0; k < k1; k++)
+                    B->lpos[p0 + k] = vslot[s + (k - k0)];
+            }
         }
+        free(vslot);
     }
 
 done:

@@ -367,9 +367,10 @@ def check_block_plan(g, p):
                 got.extend(vs.tolist())
             want = cidx[coff[r0]:coff[r1]]
             assert sorted(got) == sorted(want.tolist())
-            # lpos: the partial's slot relative to the tile, so that row r's partials are
-            # [coff[r] - p0, coff[r + 1] - p0) in block order
-            np.testing.assert_array_equal(lpos[want], np.arange(npart))
+            # lpos[p0 + k]: slot relative to the tile of the k-th partial the tile streams
+            # (block order), so that row r's partials land in [coff[r] - p0, coff[r + 1] - p0)
+            # in block order
+            np.testing.assert_array_equal(want[lpos[p0:p0 + npart]], np.array(got))
     assert sorted(tiles_seen) == list(range(B.c_n_tiles))
     assert np.all(rows_seen[row0:] == 1) and np.all(rows_seen[:row0] == 0)
 
