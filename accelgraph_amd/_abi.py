@@ -25,6 +25,7 @@ AG_NBINS = 5
 SRC_BLOCK_AUTO, SRC_BLOCK_OFF = 0, 255
 ORDER_FAST, ORDER_REFERENCE = 0, 1
 TILES_AUTO, TILES_ROUND_ROBIN, TILES_TICKET = 0, 1, 2
+TOP_AUTO, TOP_OFF, TOP_ON = 0, 1, 2
 # graphCSR.h GRAPH_FILE_*: the reference driver's -z codes
 FILE_TEXT_EDGE_LIST, FILE_BINARY_EDGE_LIST, FILE_CSR_SERIALIZED = 0, 1, 2
 BIN_NAMES = ["thread", "group16", "wave", "block", "hub"]
@@ -44,7 +45,7 @@ class PlanParams(C.Structure):
                 ("segment_nnz", u32), ("order", u32), ("src_block_bits", u32),
                 ("wg_per_cu", u32), ("blocked_min_bin", u32), ("hot_segments", u32),
                 ("fp32_order", u32), ("tile_order", u32),
-                ("piece_nnz", u32)]
+                ("piece_nnz", u32), ("top_table", u32)]
 
 
 class GraphInfo(C.Structure):
@@ -53,7 +54,8 @@ class GraphInfo(C.Structure):
                 ("num_tiles", u32), ("num_segments", u32), ("has_weights", u32),
                 ("device_bytes", u64), ("params", PlanParams), ("src_block_bits", u32),
                 ("src_blocks", u32), ("virtual_rows", u32), ("blocked_row0", u32),
-                ("hot_segments", u32), ("hot_segment_bits", u32), ("hot_edges", u64)]
+                ("hot_segments", u32), ("hot_segment_bits", u32), ("hot_edges", u64),
+                ("top_sources", u32), ("top_edges", u64)]
 
 
 class PrOptions(C.Structure):

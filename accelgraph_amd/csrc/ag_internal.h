@@ -130,6 +130,7 @@ struct BlockGeom {
     // both zero between launches
     uint32_t dyn;
     uint32_t *tick;
+    uint32_t top0;           // first source of the first-pass LDS table (n_src: none)
 };
 
 // Points pass 1 of B past the hot-segment tiles (when the hot pass runs before it).

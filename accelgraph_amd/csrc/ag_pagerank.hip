@@ -29,6 +29,8 @@
 #include <string.h>
 #include <time.h>
 
+#include <type_traits>
+
 #include "ag_internal.h"
 #include "ag_fixed_dev.h"
 #include "ag_pull_kernel.h"
@@ -272,6 +274,9 @@ __device__ __forceinline__ void pr_f32_fin(const PullGeom &G, const PrCtl &C,
 
 // Source-blocked iteration: pass 1 (partials of the blocked rows, full update of the others),
 // then pass 2 (each blocked row's partials in block order, then the update).
+// When the plan keeps an LDS table of the last sources (B.top0 < B.n_src), every workgroup
+// first copies their contributions, in the storage this iteration gathers, and pass 1 reads
+// those sources from it.
 template <typename HT>
 __device__ __forceinline__ void pr_f32_blk(const PullGeom &U, const BlockGeom &B,
                                            const PrCtl &C, const PrF32Args &A, bool mixed)
@@ -279,13 +284,25 @@ __device__ __forceinline__ void pr_f32_blk(const PullGeom &U, const BlockGeom &B
     __shared__ float s_vals[AG_TILE_NNZ_MAX];
     __shared__ float s_wave[AG_WG / AG_WAVE];
     __shared__ BlkLds L;
+    __shared__ float s_top[AG_TOP_IDS];
     const PrState *st = C.st;
     if (*(volatile const uint32_t *)&st->done)
         return;
     const uint32_t iter = *(volatile const uint32_t *)&st->iter;
     Red red{};
     pr_f32_dispatch<HT>(st, iter, A, mixed, [&](const auto &p) {
-        pull_blocked_items(U, B, p, p, red, s_vals, s_wave, L);
+        using RT = std::remove_cv_t<std::remove_pointer_t<decltype(p.cin)>>;
+        static_assert(sizeof(RT) <= sizeof(float), "the table holds 4-byte contributions");
+        if (B.top0 < B.n_src) {
+            RT *tab = (RT *)s_top;
+            const uint32_t cnt = B.n_src - B.top0;   // <= AG_TOP_IDS (the plan's table)
+            for (uint32_t i = threadIdx.x; i < cnt; i += AG_WG)
+                tab[i] = p.cin[B.top0 + i];
+            __syncthreads();
+            pull_blocked_items(U, B, p, p, red, s_vals, s_wave, L, TopTab<RT>{tab, B.top0});
+        } else {
+            pull_blocked_items(U, B, p, p, red, s_vals, s_wave, L);
+        }
     });
     pr_store_partial(C, red);
 }

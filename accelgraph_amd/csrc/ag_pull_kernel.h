@@ -331,6 +331,8 @@ struct PartPol {
     P base;
     Acc *part;
     __device__ __forceinline__ Acc edge(uint32_t j, uint32_t s) const { return base.edge(j, s); }
+    template <typename V>
+    __device__ __forceinline__ Acc edge_val(uint32_t j, V v) const { return base.edge_val(j, v); }
     __device__ __forceinline__ void finish(uint32_t v, Acc acc, Red &) const { part[v] = acc; }
 };
 
@@ -347,11 +349,33 @@ struct CombPol {
     }
 };
 
+// Where a first-pass gather reads edge j's term from (source s): the policy's own gather
+// (NoTop), or, for the sources [base, base + n) of an LDS copy of that part of the
+// contribution vector, the copy (TopTab). Both give the same value.
+struct NoTop {
+    template <class P>
+    __device__ __forceinline__ typename P::Acc term(const P &pol, uint32_t j, uint32_t s) const
+    {
+        return pol.edge(j, s);
+    }
+};
+template <typename RT>
+struct TopTab {
+    const RT *tab;
+    uint32_t base;
+    template <class P>
+    __device__ __forceinline__ typename P::Acc term(const P &pol, uint32_t j, uint32_t s) const
+    {
+        return s >= base ? pol.edge_val(j, tab[s - base]) : pol.edge(j, s);
+    }
+};
+
 // Gathers in-edges [e0, e0 + nnz) (nnz <= AG_TILE_NNZ_MAX) into s_vals with coalesced
 // source-id reads, AG_TILE_UNROLL loads then gathers in flight per lane.
-template <class P>
+template <class P, class TT = NoTop>
 __device__ __forceinline__ void stage_edges(const uint32_t *src, uint32_t e0, uint32_t nnz,
-                                            const P &pol, typename P::Acc *s_vals)
+                                            const P &pol, typename P::Acc *s_vals,
+                                            const TT &tt = TT())
 {
     const uint32_t tid = threadIdx.x;
     for (uint32_t k = tid; k < nnz; k += AG_TILE_UNROLL * AG_WG) {
@@ -365,7 +389,7 @@ __device__ __forceinline__ void stage_edges(const uint32_t *src, uint32_t e0, ui
         for (int u = 0; u < AG_TILE_UNROLL; u++) {
             const uint32_t kk = k + u * AG_WG;
             if (kk < nnz)
-                s_vals[kk] = pol.edge(e0 + kk, sidx[u]);
+                s_vals[kk] = tt.term(pol, e0 + kk, sidx[u]);
         }
     }
 }
@@ -440,9 +464,10 @@ struct BlkLds {
 
 // Partials of virtual rows [v0, v0 + nnz) that are single in-edges [e0, e0 + nnz): each is
 // its edge's gathered term, written straight out (no LDS, no barrier).
-template <class PP>
+template <class PP, class TT>
 __device__ __forceinline__ void gather_singletons(const uint32_t *src, uint32_t e0, uint32_t v0,
-                                                  uint32_t nnz, const PP &pp, Red &red)
+                                                  uint32_t nnz, const PP &pp, Red &red,
+                                                  const TT &tt)
 {
     for (uint32_t k = threadIdx.x; k < nnz; k += AG_TILE_UNROLL * AG_WG) {
         uint32_t sidx[AG_TILE_UNROLL];
@@ -455,7 +480,7 @@ __device__ __forceinline__ void gather_singletons(const uint32_t *src, uint32_t 
         for (int u = 0; u < AG_TILE_UNROLL; u++) {
             const uint32_t kk = k + u * AG_WG;
             if (kk < nnz)
-                pp.finish(v0 + kk, pp.edge(e0 + kk, sidx[u]), red);
+                pp.finish(v0 + kk, tt.term(pp, e0 + kk, sidx[u]), red);
         }
     }
 }
@@ -464,26 +489,27 @@ __device__ __forceinline__ void gather_singletons(const uint32_t *src, uint32_t 
 // rows as edges) the tile is a plain gather; otherwise its edges are staged in LDS and
 // reduced row by row, ending with a barrier. Either way each partial is its piece's terms
 // added in element order, so the two forms agree bit for bit.
-template <class PP>
+template <class PP, class TT>
 __device__ __forceinline__ void blocked_tile(const BlockGeom &B, const PP &pp, Red &red,
                                              typename PP::Acc *s_vals, BlkLds &L,
-                                             uint32_t first, uint32_t last)
+                                             uint32_t first, uint32_t last, const TT &tt)
 {
     const uint32_t e0 = B.V.off[first], nnz = B.V.off[last] - e0;
     if (nnz == last - first) {
-        gather_singletons(B.V.src, e0, first, nnz, pp, red);
+        gather_singletons(B.V.src, e0, first, nnz, pp, red, tt);
         return;
     }
-    stage_edges(B.V.src, e0, nnz, pp, s_vals);
+    stage_edges(B.V.src, e0, nnz, pp, s_vals, tt);
     __syncthreads();
     mixed_reduce(B.V.off, e0, first, last, pp, red, s_vals, L.lng, &L.nlong);
 }
 
-template <class P, class PU>
+template <class P, class PU, class TT = NoTop>
 __device__ __forceinline__ void pull_blocked_items(const PullGeom &U, const BlockGeom &B,
                                                    const P &pol, const PU &upol, Red &red,
                                                    typename P::Acc *s_vals,
-                                                   typename P::Acc *s_wave, BlkLds &L)
+                                                   typename P::Acc *s_wave, BlkLds &L,
+                                                   const TT &tt = TT())
 {
     using Acc = typename P::Acc;
     const PartPol<P> pp{pol, (Acc *)B.part};
@@ -506,7 +532,7 @@ __device__ __forceinline__ void pull_blocked_items(const PullGeom &U, const Bloc
                 next = atomicAdd(tick, 1u);   // in flight while this tile runs
             const uint32_t i = lo + t;
             const uint32_t first = B.item[2 * i], last = first + (B.item[2 * i + 1] >> 4);
-            blocked_tile(B, pp, red, s_vals, L, first, last);
+            blocked_tile(B, pp, red, s_vals, L, first, last, tt);
             k ^= 1u;
             if (tid == 0)
                 L.ticket[k] = next;
@@ -528,7 +554,7 @@ __device__ __forceinline__ void pull_blocked_items(const PullGeom &U, const Bloc
         if ((kc & 15u) == AG_ITEM_REG) {
             pull_one_item(U, upol, first, red, s_vals, s_wave);
         } else {
-            blocked_tile(B, pp, red, s_vals, L, first, first + (kc >> 4));
+            blocked_tile(B, pp, red, s_vals, L, first, first + (kc >> 4), tt);
         }
     }
 }

@@ -303,6 +303,7 @@ BlockGeom ag_make_block_geom(const ag_dgraph *g)
     K.n_src = g->n;
     K.dyn = g->bdyn;
     K.tick = g->btick;
+    K.top0 = B.bits ? B.top0 : g->n;
     return K;
 }
 
@@ -546,6 +547,8 @@ extern "C" int ag_graph_get_info(const ag_dgraph *g, ag_graph_info *info)
     info->hot_segments = g->blk.n_hot;
     info->hot_segment_bits = g->blk.hot_bits;
     info->hot_edges = g->blk.hot_edges;
+    info->top_sources = g->blk.bits && g->blk.top0 < g->n ? g->n - g->blk.top0 : 0u;
+    info->top_edges = info->top_sources ? g->blk.top_edges : 0u;
     return AG_OK;
 }
 

@@ -59,6 +59,8 @@ extern const uint32_t ag_slot_bin[AG_NSLOTS];
 #define AG_HOT_NNZ 2048u
 #define AG_HOT_GROUP_NNZ 65536u
 #define AG_HOT_DEFAULT 0u
+/* sources of the first-pass LDS table (the last ids of the device order): 16 KB of FP32 */
+#define AG_TOP_IDS 4096u
 
 /* work items of the blocked passes: item = {first, kind | count << 4} */
 enum { AG_ITEM_REG = 0,     /* pass 1: first = item of the unblocked rows' regular schedule */
@@ -106,6 +108,9 @@ typedef struct ag_block_plan {
     uint32_t *hgrp;         /* [3*n_hgrp]                                                */
     uint32_t hgrp_off[AG_NXCD + 1];
     uint32_t piece;         /* most in-edges of one piece of an ordinary block            */
+    /* first-pass LDS table of the sources [top0, n) (top0 == n: none)                     */
+    uint32_t top0;
+    uint64_t top_edges;     /* blocked in-edges whose source is >= top0                   */
 } ag_block_plan;
 
 typedef struct ag_plan {

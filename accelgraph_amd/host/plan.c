@@ -31,6 +31,7 @@ void ag_plan_params_default(ag_plan_params *p)
     p->fp32_order = AG_ORDER_FAST;
     p->tile_order = AG_TILES_AUTO;
     p->piece_nnz = 0;
+    p->top_table = AG_TOP_AUTO;
 }
 
 int ag_plan_params_check(const ag_plan_params *p, const char **err)
@@ -79,6 +80,10 @@ int ag_plan_params_check(const ag_plan_params *p, const char **err)
     }
     if (p->tile_order > AG_TILES_TICKET) {
         *err = "tile_order must be AG_TILES_AUTO, AG_TILES_ROUND_ROBIN or AG_TILES_TICKET";
+        return AG_ERR_INVALID;
+    }
+    if (p->top_table > AG_TOP_ON) {
+        *err = "top_table must be AG_TOP_AUTO, AG_TOP_OFF or AG_TOP_ON";
         return AG_ERR_INVALID;
     }
     if (p->piece_nnz > p->tile_nnz) {
@@ -686,6 +691,19 @@ static int build_block_plan(ag_plan *p, const char **err)
         free(cpre);
     }
     free(lnr);
+    /* first-pass LDS table: the last AG_TOP_IDS device ids, when they are gathered often */
+    {
+        const uint32_t t0 = n > AG_TOP_IDS ? n - AG_TOP_IDS : 0;
+        uint64_t te = 0;
+        #pragma omp parallel for reduction(+ : te) schedule(static)
+        for (uint32_t i = 0; i < mb; i++)
+            te += p->src[e0 + i] >= t0;
+        B->top_edges = te;
+        const uint32_t mode = q->top_table;
+        B->top0 = (mode == AG_TOP_ON || (mode == AG_TOP_AUTO && te * 50 >= mb)) ? t0 : n;
+        if (B->top0 == n)
+            B->top_edges = 0;
+    }
 
     /* 2: every virtual row in (row, block, piece) slot order: key, start in ss, length, row */
     uint32_t *vkey = (uint32_t *)zalloc(nv, sizeof(uint32_t));

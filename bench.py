@@ -495,6 +495,8 @@ def main() -> int:
                     help="plan parameter piece_nnz for the bench graphs (0 = auto)")
     ap.add_argument("--src-block-bits", type=int, default=0,
                     help="plan parameter src_block_bits for the bench graphs (0 = auto)")
+    ap.add_argument("--top-table", type=int, default=0,
+                    help="plan parameter top_table for the bench graphs (0 auto, 1 off, 2 on)")
     ap.add_argument("--no-rows", action="store_true",
                     help="skip the fixed-32, quantised, push and data-driven rows")
     args = ap.parse_args()
@@ -550,6 +552,8 @@ def main() -> int:
         pkw["piece_nnz"] = args.piece_nnz
     if args.src_block_bits:
         pkw["src_block_bits"] = args.src_block_bits
+    if args.top_table:
+        pkw["top_table"] = args.top_table
     dg = dev.upload(g, params=ag.plan_params(**pkw) if pkw else None)
     info = dg.info()
     log(f"uploaded + planned in {time.perf_counter() - t0:.1f} s; bins rows="
@@ -632,6 +636,8 @@ def main() -> int:
                      "src_block_bits": info.src_block_bits,
                      "hot_segments": info.hot_segments,
                      "hot_edge_fraction": round(info.hot_edges / max(m, 1), 4),
+                     "lds_top_sources": info.top_sources,
+                     "lds_top_edge_fraction": round(info.top_edges / max(m, 1), 4),
                      "measured_copy_gb_per_s": round(copy_gbs, 1)},
     }
 

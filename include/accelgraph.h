@@ -131,10 +131,19 @@ typedef struct ag_plan_params {
                                   3/4 as many pieces as in-edges (then every in-edge is its
                                   own piece and the first pass is a plain gather); else
                                   1..tile_nnz                                                */
+    uint32_t top_table;        /* with source blocking, an LDS copy of the contributions of
+                                  the last 4096 device ids (the heaviest rows of the device
+                                  order, which on skewed graphs are also the most gathered
+                                  sources) in every first-pass workgroup of the FP32 PageRank
+                                  kinds, read instead of L2 for those sources (RMAT-25: 12% of
+                                  the edges, 2.9% faster; FP32 SpMV measured no gain). AG_TOP_AUTO (default): when they
+                                  receive at least 2% of the blocked in-edges; AG_TOP_OFF;
+                                  AG_TOP_ON                                                  */
 } ag_plan_params;
 
 enum { AG_ORDER_FAST = 0, AG_ORDER_REFERENCE = 1 };
 enum { AG_TILES_AUTO = 0, AG_TILES_ROUND_ROBIN = 1, AG_TILES_TICKET = 2 };
+enum { AG_TOP_AUTO = 0, AG_TOP_OFF = 1, AG_TOP_ON = 2 };
 
 #define AG_SRC_BLOCK_AUTO 0u
 #define AG_SRC_BLOCK_OFF 255u
@@ -159,6 +168,8 @@ typedef struct ag_graph_info {
     uint32_t hot_segments;     /* hot source segments in effect (blocks staged in LDS)      */
     uint32_t hot_segment_bits; /* sources per hot segment = 2^this                          */
     uint64_t hot_edges;        /* in-edges whose source lies in a hot segment               */
+    uint32_t top_sources;      /* sources of the first-pass LDS table (top_table; 0: none)  */
+    uint64_t top_edges;        /* blocked in-edges whose source lies in that table          */
 } ag_graph_info;
 
 /* Upload one directed graph. Arrays are host arrays in the reference's layout:

@@ -39,8 +39,12 @@ PARAMS = {"default": None, "small": SMALL, "small-order1": dict(SMALL, order=1),
           "blocked-nohot": dict(src_block_bits=10, blocked_min_bin=1, hot_segments=0),
           # every in-edge its own piece (the first pass a plain gather), tiles by ticket
           "blocked-edge": dict(src_block_bits=9, blocked_min_bin=0, piece_nnz=1,
-                               tile_order=ag.TILES_TICKET)}
-BLOCKED = ["blocked", "blocked-all", "blocked-small", "blocked-nohot", "blocked-edge"]
+                               tile_order=ag.TILES_TICKET),
+          # the last 4096 sources read from an LDS table in the FP32 first pass, and not
+          "blocked-top": dict(src_block_bits=10, blocked_min_bin=0, top_table=ag.TOP_ON),
+          "blocked-notop": dict(src_block_bits=10, blocked_min_bin=0, top_table=ag.TOP_OFF)}
+BLOCKED = ["blocked", "blocked-all", "blocked-small", "blocked-nohot", "blocked-edge",
+           "blocked-top", "blocked-notop"]
 
 
 def say(*a):
@@ -639,11 +643,16 @@ def test_pagerank_blocked_auto_rmat22(device):
     assert err <= FP32_TOL
     f32_row_bars(dg, g, 20)
     dg.close()
-    # pass-1 tiles by ticket instead of round-robin: the same ranks bit for bit
-    fixed = device.upload(g, params=ag.plan_params(tile_order=ag.TILES_TICKET))
-    np.testing.assert_array_equal(fixed.pagerank(**kw).ranks.view(np.uint32),
-                                  run.ranks.view(np.uint32))
-    fixed.close()
+    # pass-1 tiles by ticket instead of round-robin, and without the LDS table of the last
+    # sources (on by default here): the same ranks bit for bit
+    assert info.top_sources == 4096 and info.top_edges * 50 >= g.m
+    for prm in (dict(tile_order=ag.TILES_TICKET), dict(top_table=ag.TOP_OFF)):
+        other = device.upload(g, params=ag.plan_params(**prm))
+        np.testing.assert_array_equal(other.pagerank(**kw).ranks.view(np.uint32),
+                                      run.ranks.view(np.uint32))
+        if "top_table" in prm:
+            assert other.info().top_sources == 0
+        other.close()
     plain = device.upload(g, params=ag.plan_params(src_block_bits=ag.SRC_BLOCK_OFF))
     assert plain.info().src_block_bits == 0
     run2 = plain.pagerank(**kw)

@@ -23,7 +23,8 @@ class BlockPlan(C.Structure):
                 ("c_item_off", C.c_uint32 * 9), ("r_slot_blk0", C.c_uint32 * 6),
                 ("hot_bits", C.c_uint32), ("n_hot", C.c_uint32), ("hot_edges", C.c_uint64),
                 ("hot_seg", u32p), ("hot_end", C.c_uint32 * 8), ("n_hgrp", C.c_uint32),
-                ("hgrp", u32p), ("hgrp_off", C.c_uint32 * 9), ("piece", C.c_uint32)]
+                ("hgrp", u32p), ("hgrp_off", C.c_uint32 * 9), ("piece", C.c_uint32),
+                ("top0", C.c_uint32), ("top_edges", C.c_uint64)]
 
 
 class Plan(C.Structure):
@@ -227,6 +228,12 @@ def check_block_plan(g, p):
         if piece == 1 and min(PIECE, prm.tile_nnz) > 1:
             assert B.n_hot or B.nv == B.mb
     hot_piece = min(HOT_NNZ, prm.tile_nnz)
+    # first-pass LDS table: the last 4096 device ids, kept when asked or when they receive at
+    # least 2% of the blocked in-edges
+    t0 = max(n - 4096, 0)
+    te = int(np.count_nonzero(view(p.src, p.m)[view(p.off, n + 1)[B.row0]:] >= t0))
+    on = prm.top_table == ag.TOP_ON or (prm.top_table == ag.TOP_AUTO and te * 50 >= B.mb)
+    assert (B.top0, B.top_edges) == ((t0, te) if on else (n, 0))
     off = view(p.off, n + 1)
     srcd = view(p.src, p.m)
     br = list(p.bin_row)
