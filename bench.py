@@ -606,7 +606,9 @@ def main() -> int:
     if os.path.exists(tpath):
         t = json.load(open(tpath)).get(wl)
         if t and t.get("hot_segments") == info.hot_segments and \
-                t.get("src_block_bits") == info.src_block_bits:
+                t.get("src_block_bits") == info.src_block_bits and \
+                t.get("top_sources") == info.top_sources and \
+                t.get("virtual_rows") == info.virtual_rows:
             traffic = t["traffic_bytes_per_launch"]
             traffic_src = (f"profiled, not measured in this run: {t['profile']} "
                            f"(rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a copy kernel)")
