@@ -1073,9 +1073,22 @@ def test_ppr_hip_graph_replays(device, rmat12):
 
 # ------------------------------------------------------------------ autotuner
 
+def _iteration_ms(device, dg, steps=20):
+    sess = dg.session(max_iterations=steps + 3, stop_rule=ag.STOP_NONE, init=ag.INIT_UNIFORM)
+    sess.step(3)
+    device.sync()
+    sess.step(steps)
+    device.sync()
+    ms, _ = sess.last_step_kernel_ms()
+    sess.end()
+    return ms
+
+
 def test_autotuner_choice_keeps_results(device, tmp_path):
     """The tuned plan runs the same computation: fixed-point bit-exact and FP32 within
-    tolerance of the oracle; the table answers the second call."""
+    tolerance of the oracle; the table answers the second call; and timed again here, apart
+    from the tuner's own trials, the tuned plan is no slower than the default one (15% for
+    noise between runs of ~20 us iterations)."""
     from accelgraph_amd import autotune
     say("autotuner")
     g = ag.Graph.rmat(14)
@@ -1088,6 +1101,12 @@ def test_autotuner_choice_keeps_results(device, tmp_path):
     say(f"  tuned {autotune._params_dict(params)}: {rep['ms_per_iteration']:.4f} ms vs default "
         f"{rep['default_ms_per_iteration']:.4f} ms over {rep['trials']} trials")
     dg = device.upload(g, params=params)
+    base = device.upload(g)
+    t_tuned = min(_iteration_ms(device, dg) for _ in range(5))
+    t_default = min(_iteration_ms(device, base) for _ in range(5))
+    base.close()
+    say(f"  re-timed: tuned {t_tuned:.4f} ms, default {t_default:.4f} ms per iteration")
+    assert t_tuned <= 1.15 * t_default
     fx = dg.pagerank(kind=32, max_iterations=60, epsilon=1e-4)
     ref = ref_pr(g, kind=32, max_iterations=60, epsilon=1e-4)
     np.testing.assert_array_equal(fx.ranks.view(np.uint32), ref.ranks.view(np.uint32))
