@@ -113,22 +113,36 @@ __device__ __forceinline__ void add4(float4 &a, const float4 &b)
     a.w += b.w;
 }
 
+// This lane's source id in the first 16-edge chunk of group gi over in-edges [lo, hi)
+// (0 past hi).
+__device__ __forceinline__ uint32_t ppr_first_ids(const PullGeom &G, uint32_t lo, uint32_t hi,
+                                                  uint32_t gi, uint32_t q)
+{
+    const uint32_t j = lo + gi * PPR_GL + q;
+    return j < hi ? __builtin_nontemporal_load(&G.src[j]) : 0u;
+}
+
 // Sum of columns 4q..4q+3 over the in-edges [lo, hi) owned by group gi of the ng groups that
 // share the row: the edges are cut into 16-edge chunks, chunk c belongs to group c % ng. A
-// chunk's source ids arrive in one coalesced load (one per lane) and are passed lane to lane,
-// so every contribution line of the chunk is in flight at once: 16 loads of 8 bytes per lane
-// (16-bit storage) or two rounds of 8 loads of 16 bytes (FP32). Per lane the terms are added in
-// edge order into two accumulators (even and odd edges), combined at the end.
+// chunk's source ids arrive in one coalesced load (one per lane; `sl` holds the first chunk's,
+// loaded by the caller) and are passed lane to lane, so every contribution line of the chunk
+// is in flight at once: 16 loads of 8 bytes per lane (16-bit storage) or two rounds of 8 loads
+// of 16 bytes (FP32); the next chunk's ids are loaded before them. Per lane the terms are added
+// in edge order into two accumulators (even and odd edges), combined at the end.
 __device__ __forceinline__ float4 ppr_group_sum(const PullGeom &G, const PprArgs &A,
                                                 const Mode &m, uint32_t lo, uint32_t hi,
-                                                uint32_t gi, uint32_t ng, uint32_t q)
+                                                uint32_t gi, uint32_t ng, uint32_t q,
+                                                uint32_t sl)
 {
     float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = a0;
     const uint2 *c16 = (const uint2 *)A.c16[m.rd];
     const float4 *c32 = (const float4 *)A.c32[m.rd];
-    for (uint32_t base = lo + gi * PPR_GL; base < hi; base += ng * PPR_GL) {
+    const uint32_t stride = ng * PPR_GL;
+    for (uint32_t base = lo + gi * PPR_GL; base < hi; base += stride) {
         const uint32_t cnt = hi - base < PPR_GL ? hi - base : PPR_GL;
-        const uint32_t sl = q < cnt ? __builtin_nontemporal_load(&G.src[base + q]) : 0u;
+        const uint32_t nb = base + stride;
+        const uint32_t sl_next =
+            (nb < hi && nb + q < hi) ? __builtin_nontemporal_load(&G.src[nb + q]) : 0u;
         if (m.r16) {
             uint2 v[PPR_GL];
 #pragma unroll
@@ -156,6 +170,7 @@ __device__ __forceinline__ float4 ppr_group_sum(const PullGeom &G, const PprArgs
                     add4((t & 1u) ? a1 : a0, v[t]);
             }
         }
+        sl = sl_next;
     }
     add4(a0, a1);
     return a0;
@@ -251,7 +266,9 @@ __global__ __launch_bounds__(AG_WG) void ppr_kernel(PullGeom G, PprArgs A)
     const uint32_t nseg = G.slot_blk0[1] - G.slot_blk0[0];
     const uint32_t heavy = nseg + nblock;
     const uint32_t total = heavy + nwave + nlite;
-    for (uint32_t it = blockIdx.x; it < total; it += gridDim.x) {
+    const uint32_t lite0 = heavy + nwave;
+    uint32_t it = blockIdx.x;
+    for (; it < lite0; it += gridDim.x) {
         if (it < heavy) {
             uint32_t lo, hi, row = 0;
             const bool hub = it < nseg;
@@ -263,7 +280,8 @@ __global__ __launch_bounds__(AG_WG) void ppr_kernel(PullGeom G, PprArgs A)
                 lo = G.off[row];
                 hi = G.off[row + 1];
             }
-            s_red[gi * PPR_GL + q] = ppr_group_sum(G, A, m, lo, hi, gi, PPR_GROUPS, q);
+            s_red[gi * PPR_GL + q] = ppr_group_sum(G, A, m, lo, hi, gi, PPR_GROUPS, q,
+                                                   ppr_first_ids(G, lo, hi, gi, q));
             __syncthreads();
             if (gi == 0) {
                 float4 t = s_red[q];
@@ -279,8 +297,11 @@ __global__ __launch_bounds__(AG_WG) void ppr_kernel(PullGeom G, PprArgs A)
             const uint32_t row = wave0 + (it - heavy) * (AG_WG / AG_WAVE) + tid / AG_WAVE;
             const bool valid = row < wave1;
             float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (valid)
-                a = ppr_group_sum(G, A, m, G.off[row], G.off[row + 1], wl, AG_WAVE / PPR_GL, q);
+            if (valid) {
+                const uint32_t lo = G.off[row], hi = G.off[row + 1];
+                a = ppr_group_sum(G, A, m, lo, hi, wl, AG_WAVE / PPR_GL, q,
+                                  ppr_first_ids(G, lo, hi, wl, q));
+            }
             // (g0 + g1) + (g2 + g3), the same value in every group (addition commutes)
 #pragma unroll
             for (int msk = PPR_GL; msk < AG_WAVE; msk <<= 1) {
@@ -291,12 +312,37 @@ __global__ __launch_bounds__(AG_WG) void ppr_kernel(PullGeom G, PprArgs A)
             }
             if (valid && wl == 0)
                 ppr_finish(A, m, row, q, a, seed, err);
-        } else {
-            const uint32_t row = (it - heavy - nwave) * PPR_GROUPS + gi;
-            if (row < lite_rows) {
-                const float4 a = ppr_group_sum(G, A, m, G.off[row], G.off[row + 1], 0, 1, q);
-                ppr_finish(A, m, row, q, a, seed, err);
+        }
+    }
+    // THREAD and GROUP16 rows, a group each, two rows ahead: while row k is gathered, the
+    // first source ids of row k + 1 and the offsets of row k + 2 are in flight
+    {
+        const uint32_t step = gridDim.x * PPR_GROUPS;
+        uint32_t row = it < total ? (it - lite0) * PPR_GROUPS + gi : lite_rows;
+        uint32_t lo = 0, hi = 0, lo1 = 0, hi1 = 0, sl = 0;
+        if (row < lite_rows) {
+            lo = G.off[row];
+            hi = G.off[row + 1];
+            sl = ppr_first_ids(G, lo, hi, 0, q);
+        }
+        if (row < lite_rows && row + step < lite_rows) {
+            lo1 = G.off[row + step];
+            hi1 = G.off[row + step + 1];
+        }
+        for (; row < lite_rows; row += step) {
+            uint32_t lo2 = 0, hi2 = 0;
+            if (row + 2 * step < lite_rows) {
+                lo2 = G.off[row + 2 * step];
+                hi2 = G.off[row + 2 * step + 1];
             }
+            const uint32_t sl1 = ppr_first_ids(G, lo1, hi1, 0, q);
+            const float4 a = ppr_group_sum(G, A, m, lo, hi, 0, 1, q, sl);
+            ppr_finish(A, m, row, q, a, seed, err);
+            lo = lo1;
+            hi = hi1;
+            sl = sl1;
+            lo1 = lo2;
+            hi1 = hi2;
         }
     }
     ppr_column_errors<PPR_GROUPS>(err, s_err, s_tot);
