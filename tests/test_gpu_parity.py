@@ -622,6 +622,28 @@ def test_pagerank_directions_corners(device):
     dg.close()
 
 
+@pytest.mark.parametrize("scale", [9, 12])
+def test_pagerank_lds_table_whole_vector(device, scale):
+    """A graph of at most 4096 vertices: the first pass's LDS table holds every source, so
+    every gather reads LDS. FP32 ranks equal the table-less plan's bit for bit (also with
+    16-bit storage), fixed-32 equals the oracle."""
+    say(f"lds table holding the whole vector, rmat-{scale}")
+    g = ag.Graph.rmat(scale)
+    base = dict(src_block_bits=8, blocked_min_bin=0)
+    on = device.upload(g, params=ag.plan_params(**base, top_table=ag.TOP_ON))
+    off = device.upload(g, params=ag.plan_params(**base, top_table=ag.TOP_OFF))
+    assert on.info().top_sources == g.n and off.info().top_sources == 0
+    kw = dict(max_iterations=15, stop_rule=ag.STOP_NONE)
+    for extra in (dict(), dict(storage=ag.STORE_BF16, refine_l1=1e-5)):
+        a, b = on.pagerank(**kw, **extra), off.pagerank(**kw, **extra)
+        np.testing.assert_array_equal(a.ranks.view(np.uint32), b.ranks.view(np.uint32))
+    fx = on.pagerank(kind=32, **kw)
+    ref = ref_pr(g, kind=32, max_iterations=15, stop_rule=2)
+    np.testing.assert_array_equal(fx.ranks.view(np.uint32), ref.ranks.view(np.uint32))
+    on.close()
+    off.close()
+
+
 def test_pagerank_blocked_auto_rmat22(device):
     """Above 2^21 vertices the default plan blocks sources (2^19 per block). RMAT-22: fixed32
     bit-exact and FP32 within tolerance of the oracle for 20 iterations; the blocked, unblocked
