@@ -424,14 +424,18 @@ def config1_section(ag, dev, cpu: bool) -> dict:
     return out
 
 
-def er_section(ag, dev, scale: int, steps: int, cpu: bool, pkw: dict) -> dict:
-    """PageRank-pull FP32 iteration on an Erdos-Renyi graph with the bench graph's vertex
-    count and 16n edge draws (uniform sources: no hot set, every gather is a cold random
-    read), and the solve to L1 <= 1e-6 from the uniform start with its error against the FP64
-    oracle (CPU leg)."""
+def er_section(ag, dev, scale: int, steps: int, cpu: bool, pkw: dict, m_target: int) -> dict:
+    """PageRank-pull FP32 iteration on an Erdos-Renyi graph with the bench graph's vertex and
+    edge counts (uniform sources: no hot set, every gather is a cold random read), and the
+    solve to L1 <= 1e-6 from the uniform start with its error against the FP64 oracle (CPU
+    leg). The draws are the RMAT graph's edge count plus the expected self-loops and repeated
+    pairs that simplification removes (draws / n + draws^2 / 2n^2, about m / 2^22 at
+    scale 25), so both graphs of the configuration have the same N and, within a few dozen
+    edges, the same M."""
     import numpy as np
     n = 1 << scale
-    g = ag.Graph.erdos_renyi_gpu(dev, n, 16 * n)
+    extra = int(m_target / n + m_target * m_target / (2.0 * n * n)) + 1
+    g = ag.Graph.erdos_renyi_gpu(dev, n, m_target + extra)
     dg = dev.upload(g, params=ag.plan_params(**pkw) if pkw else None)
     sess = dg.session(max_iterations=steps + 3, stop_rule=ag.STOP_NONE, init=ag.INIT_UNIFORM)
     sess.step(3)
@@ -447,7 +451,8 @@ def er_section(ag, dev, scale: int, steps: int, cpu: bool, pkw: dict) -> dict:
     dg.close()
     nb = algorithmic_bytes(g.n, g.m, "f32")
     gbs = nb / (ms * 1e-3) / 1e9
-    out = {"graph": f"Erdos-Renyi n=2^{scale}, 16n draws, simplified (n={g.n}, m={g.m})",
+    out = {"graph": f"Erdos-Renyi n=2^{scale}, {m_target + extra} draws, simplified "
+                    f"(n={g.n}, m={g.m}; the RMAT graph's m={m_target})",
            "ms_per_iteration": round(ms, 4), "gteps": round(g.m / (ms * 1e-3) / 1e9, 2),
            "gb_per_s": round(gbs, 1), "roofline_frac": round(gbs / HBM_PEAK_GBS, 4),
            "src_block_bits": info.src_block_bits, "hot_segments": info.hot_segments,
@@ -691,7 +696,7 @@ def main() -> int:
         log(f"config 1: {out['config1_rmat16']} ({time.perf_counter() - t0:.1f} s)")
     if not args.no_er and args.graph == "rmat":
         t0 = time.perf_counter()
-        out["er_pagerank"] = er_section(ag, dev, args.scale, 20, not args.no_cpu, pkw)
+        out["er_pagerank"] = er_section(ag, dev, args.scale, 20, not args.no_cpu, pkw, m)
         log(f"er: {out['er_pagerank']} ({time.perf_counter() - t0:.1f} s)")
     dev.close()
     if not args.no_cpu:
