@@ -34,6 +34,8 @@ GRID = {
     "wg_per_cu": [3, 4, 5, 6, 8],
     "order": [0, 1],
     "hot_segments": [0, 32, 128],
+    "top_table": [_abi.TOP_AUTO, _abi.TOP_OFF, _abi.TOP_ON],
+    "tile_order": [_abi.TILES_AUTO, _abi.TILES_ROUND_ROBIN, _abi.TILES_TICKET],
 }
 
 # the batched forms: ag_batch_options fields and ag_ppr_options.wg_per_cu
