@@ -319,8 +319,8 @@ __device__ __forceinline__ void pull_items(const PullGeom &G, const P &pol, Red 
 
 // ---------------------------------------------------------------- source-blocked pull
 
-#define AG_LANE_MAX 32u    // rows of a tile with at most this many elements: a lane each
-#define AG_LONG_MAX 128u   // longer rows of one tile: at most AG_TILE_NNZ_MAX / (AG_LANE_MAX + 1)
+#define AG_LANE_MAX 64u    // rows of a tile with at most this many elements: a lane each
+#define AG_LONG_MAX 64u    // longer rows of one tile: at most AG_TILE_NNZ_MAX / (AG_LANE_MAX + 1)
 static_assert(AG_TILE_NNZ_MAX / (AG_LANE_MAX + 1u) < AG_LONG_MAX,
               "the long rows of a full tile must fit the tile's LDS list");
 
