@@ -130,8 +130,10 @@ __device__ __forceinline__ uint32_t ld_stream(const uint32_t *p)
 }
 
 #define AG_UNROLL 4
-// source-id loads (then gathers) in flight per lane in an LDS tile
-#define AG_TILE_UNROLL 4
+// source-id loads (then gathers) in flight per lane in an LDS tile: 16 covers a full tile
+// (AG_TILE_NNZ_MAX / AG_WG) in one round. Measured on RMAT-25 / ER-25 pass 1: 2 / 4 / 8 / 16
+// gave 2.96 / 2.76 / 2.63 / 2.57 ms and 5.11 / 4.63 / 4.52 / 4.43 ms per iteration
+#define AG_TILE_UNROLL 16
 
 // Lane-strided partial sum of in-edges lo, lo+S, ... < hi. Each round issues the source-id
 // loads of AG_UNROLL elements, then their gathers, all predicated, so a lane has up to
