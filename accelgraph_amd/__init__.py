@@ -21,6 +21,7 @@ from ._abi import (BIN_NAMES, GATHER_F32, GATHER_U8, GATHER_U16, GATHER_U32, GAT
                    KIND_QUANT8, DIR_PULL, DIR_PUSH, DIR_DATA_DRIVEN, STOP_ACTIVE, STOP_L1, STOP_NONE,
                    STORE_BF16, STORE_F16, STORE_F32, PPR_UNIFORM, SRC_BLOCK_AUTO, ORDER_FAST, ORDER_REFERENCE,
                    TILES_AUTO, TILES_ROUND_ROBIN, TILES_TICKET, TOP_AUTO, TOP_OFF, TOP_ON,
+                   SORT_AUTO, SORT_OFF, SORT_ON,
                    SRC_BLOCK_OFF, FILE_TEXT_EDGE_LIST, FILE_BINARY_EDGE_LIST,
                    FILE_CSR_SERIALIZED, GraphInfo, PlanParams, PrOptions, PrResult)
 

@@ -26,6 +26,7 @@ SRC_BLOCK_AUTO, SRC_BLOCK_OFF = 0, 255
 ORDER_FAST, ORDER_REFERENCE = 0, 1
 TILES_AUTO, TILES_ROUND_ROBIN, TILES_TICKET = 0, 1, 2
 TOP_AUTO, TOP_OFF, TOP_ON = 0, 1, 2
+SORT_AUTO, SORT_OFF, SORT_ON = 0, 1, 2
 # graphCSR.h GRAPH_FILE_*: the reference driver's -z codes
 FILE_TEXT_EDGE_LIST, FILE_BINARY_EDGE_LIST, FILE_CSR_SERIALIZED = 0, 1, 2
 BIN_NAMES = ["thread", "group16", "wave", "block", "hub"]
@@ -45,7 +46,7 @@ class PlanParams(C.Structure):
                 ("segment_nnz", u32), ("order", u32), ("src_block_bits", u32),
                 ("wg_per_cu", u32), ("blocked_min_bin", u32), ("hot_segments", u32),
                 ("fp32_order", u32), ("tile_order", u32),
-                ("piece_nnz", u32), ("top_table", u32)]
+                ("piece_nnz", u32), ("top_table", u32), ("tile_sort", u32)]
 
 
 class GraphInfo(C.Structure):
@@ -55,7 +56,7 @@ class GraphInfo(C.Structure):
                 ("device_bytes", u64), ("params", PlanParams), ("src_block_bits", u32),
                 ("src_blocks", u32), ("virtual_rows", u32), ("blocked_row0", u32),
                 ("hot_segments", u32), ("hot_segment_bits", u32), ("hot_edges", u64),
-                ("top_sources", u32), ("top_edges", u64)]
+                ("top_sources", u32), ("top_edges", u64), ("sorted_tiles", u32)]
 
 
 class PrOptions(C.Structure):

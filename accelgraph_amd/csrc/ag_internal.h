@@ -131,6 +131,11 @@ struct BlockGeom {
     uint32_t dyn;
     uint32_t *tick;
     uint32_t top0;           // first source of the first-pass LDS table (n_src: none)
+    // pass-1 tiles sorted by source (sbits != 0): vsp[e] = position in the tile << sbits |
+    // (source - tbase[item]); the per-edge weights are then in the same order
+    const uint32_t *vsp;
+    const uint32_t *tbase;
+    uint32_t sbits;
 };
 
 // Points pass 1 of B past the hot-segment tiles (when the hot pass runs before it).

@@ -487,16 +487,59 @@ __device__ __forceinline__ void gather_singletons(const uint32_t *src, uint32_t 
     }
 }
 
-// One pass-1 tile, virtual rows [first, last). When they are all single edges (as many
-// rows as edges) the tile is a plain gather; otherwise its edges are staged in LDS and
-// reduced row by row, ending with a barrier. Either way each partial is its piece's terms
-// added in element order, so the two forms agree bit for bit.
+// stage_edges over a tile whose ids are stored sorted by source: word w holds the source's
+// offset from `base` in its low `bits` bits and the edge's position in the tile above them.
+// Each term lands at its position, so s_vals ends as stage_edges leaves it; the lanes of a
+// wavefront gather neighbouring sources, which share more L2 lines than the in-edges of
+// consecutive rows do. Per-edge data (weights) is stored in the same sorted order and read
+// at the stream index, coalesced.
+template <class P, class TT>
+__device__ __forceinline__ void stage_sorted(const uint32_t *vsp, uint32_t base, uint32_t bits,
+                                             uint32_t e0, uint32_t nnz, const P &pol,
+                                             typename P::Acc *s_vals, const TT &tt)
+{
+    const uint32_t tid = threadIdx.x, mask = (1u << bits) - 1u;
+    for (uint32_t k = tid; k < nnz; k += AG_TILE_UNROLL * AG_WG) {
+        uint32_t w[AG_TILE_UNROLL];
+#pragma unroll
+        for (int u = 0; u < AG_TILE_UNROLL; u++) {
+            const uint32_t kk = k + u * AG_WG;
+            w[u] = kk < nnz ? ld_stream(&vsp[e0 + kk]) : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < AG_TILE_UNROLL; u++) {
+            const uint32_t kk = k + u * AG_WG;
+            if (kk < nnz) {
+                const uint32_t pos = w[u] >> bits;   // < nnz: a permutation of the tile
+                s_vals[pos] = tt.term(pol, e0 + kk, base + (w[u] & mask));
+            }
+        }
+    }
+}
+
+// One pass-1 tile (item `it`), virtual rows [first, last). When they are all single edges
+// (as many rows as edges) the tile is a plain gather; otherwise its edges are staged in LDS
+// and reduced row by row, ending with a barrier. Either way each partial is its piece's terms
+// added in element order, so the forms agree bit for bit. With sorted ids (B.sbits) the
+// terms are staged in LDS either way, and single edges are then written out in order.
 template <class PP, class TT>
 __device__ __forceinline__ void blocked_tile(const BlockGeom &B, const PP &pp, Red &red,
-                                             typename PP::Acc *s_vals, BlkLds &L,
+                                             typename PP::Acc *s_vals, BlkLds &L, uint32_t it,
                                              uint32_t first, uint32_t last, const TT &tt)
 {
     const uint32_t e0 = B.V.off[first], nnz = B.V.off[last] - e0;
+    if (B.sbits) {
+        stage_sorted(B.vsp, B.tbase[it], B.sbits, e0, nnz, pp, s_vals, tt);
+        __syncthreads();
+        if (nnz == last - first) {
+            for (uint32_t k = threadIdx.x; k < nnz; k += AG_WG)
+                pp.finish(first + k, s_vals[k], red);
+            __syncthreads();   // s_vals is reused by the next tile
+            return;
+        }
+        mixed_reduce(B.V.off, e0, first, last, pp, red, s_vals, L.lng, &L.nlong);
+        return;
+    }
     if (nnz == last - first) {
         gather_singletons(B.V.src, e0, first, nnz, pp, red, tt);
         return;
@@ -534,7 +577,7 @@ __device__ __forceinline__ void pull_blocked_items(const PullGeom &U, const Bloc
                 next = atomicAdd(tick, 1u);   // in flight while this tile runs
             const uint32_t i = lo + t;
             const uint32_t first = B.item[2 * i], last = first + (B.item[2 * i + 1] >> 4);
-            blocked_tile(B, pp, red, s_vals, L, first, last, tt);
+            blocked_tile(B, pp, red, s_vals, L, i, first, last, tt);
             k ^= 1u;
             if (tid == 0)
                 L.ticket[k] = next;
@@ -556,7 +599,7 @@ __device__ __forceinline__ void pull_blocked_items(const PullGeom &U, const Bloc
         if ((kc & 15u) == AG_ITEM_REG) {
             pull_one_item(U, upol, first, red, s_vals, s_wave);
         } else {
-            blocked_tile(B, pp, red, s_vals, L, first, first + (kc >> 4), tt);
+            blocked_tile(B, pp, red, s_vals, L, i, first, first + (kc >> 4), tt);
         }
     }
 }

@@ -228,7 +228,7 @@ extern "C" int ag_graph_free(ag_dgraph *g)
                     g->seg_lo, g->seg_hi, g->hub_seg0, g->hub_part, g->vec_a, g->vec_b,
                     g->blk.voff, g->blk.vsrc, g->blk.vw, g->blk.item, g->blk.coff, g->blk.cidx,
                     g->blk.lpos, g->blk.ct_row, g->blk.ct_start, g->blk.ct_pre, g->blk.c_item,
-                    g->blk.hgrp,
+                    g->blk.hgrp, g->blk.vsp, g->blk.tbase,
                     g->vpart, g->btick, g->vwq, g->fwd_off, g->fwd_src, g->fwd_dst,
                     g->fwd_w, g->pb_row, g->pb_pos, g->pb_dst};
     for (void *p : ptrs)
@@ -304,6 +304,9 @@ BlockGeom ag_make_block_geom(const ag_dgraph *g)
     K.dyn = g->bdyn;
     K.tick = g->btick;
     K.top0 = B.bits ? B.top0 : g->n;
+    K.vsp = B.vsp;
+    K.tbase = B.tbase;
+    K.sbits = B.bits && B.sorted ? B.bits : 0u;
     return K;
 }
 
@@ -433,6 +436,7 @@ extern "C" int ag_graph_upload(ag_device *dev, uint32_t n, uint32_t m, const uin
         g->blk.lpos = NULL;
         g->blk.vw = NULL;
         g->blk.hgrp = NULL;
+        g->blk.vsp = g->blk.tbase = NULL;
         g->blk.hot_seg = NULL;   // host-only array of the plan, freed with it
         const size_t nt = B.c_n_tiles, nb = B.nblocks;
         st = dev_upload(g, &g->blk.voff, B.voff, (size_t)B.nv + 1);
@@ -447,6 +451,8 @@ extern "C" int ag_graph_upload(ag_device *dev, uint32_t n, uint32_t m, const uin
         if (st == AG_OK) st = dev_upload(g, &g->blk.ct_pre, B.ct_pre, nt * (nb + 1));
         if (st == AG_OK) st = dev_upload(g, &g->blk.c_item, B.c_item, (size_t)B.c_n_items * 2);
         if (st == AG_OK && B.n_hgrp) st = dev_upload(g, &g->blk.hgrp, B.hgrp, (size_t)B.n_hgrp * 3);
+        if (st == AG_OK && B.sorted) st = dev_upload(g, &g->blk.vsp, B.vsp, B.mb);
+        if (st == AG_OK && B.sorted) st = dev_upload(g, &g->blk.tbase, B.tbase, B.n_items);
         if (st == AG_OK) {
             size_t pb = (size_t)(B.nv ? B.nv : 1) * sizeof(uint64_t);
             he = hipMalloc((void **)&g->vpart, pb);
@@ -549,6 +555,7 @@ extern "C" int ag_graph_get_info(const ag_dgraph *g, ag_graph_info *info)
     info->hot_edges = g->blk.hot_edges;
     info->top_sources = g->blk.bits && g->blk.top0 < g->n ? g->n - g->blk.top0 : 0u;
     info->top_edges = info->top_sources ? g->blk.top_edges : 0u;
+    info->sorted_tiles = g->blk.bits && g->blk.sorted ? 1u : 0u;
     return AG_OK;
 }
 

@@ -78,7 +78,8 @@ typedef struct ag_block_plan {
     uint32_t slice_row[AG_NXCD + 1];
     uint32_t *voff;         /* [nv+1] offsets of the virtual rows in vsrc               */
     uint32_t *vsrc;         /* [mb]   device source ids, virtual-row order              */
-    float *vw;              /* [mb]   weights in the same order, or NULL                */
+    float *vw;              /* [mb]   weights in pass-1 stream order (beside vsrc, or beside
+                               vsp when the tiles are sorted), or NULL                    */
     uint32_t n_items;
     uint32_t *item;         /* [2*n_items] pass-1 items                                 */
     uint32_t item_off[AG_NXCD + 1]; /* items of slice x: [item_off[x], item_off[x+1])   */
@@ -111,6 +112,12 @@ typedef struct ag_block_plan {
     /* first-pass LDS table of the sources [top0, n) (top0 == n: none)                     */
     uint32_t top0;
     uint64_t top_edges;     /* blocked in-edges whose source is >= top0                   */
+    /* tile_sort (no hot segments): the ids of every pass-1 tile sorted by source; vsp[e] =
+     * position of the edge in its tile << bits | (source - tbase[item]) (sorted == 0: vsp,
+     * tbase NULL)                                                                         */
+    uint32_t sorted;
+    uint32_t *vsp;          /* [mb]      packed sorted ids, tile by tile                  */
+    uint32_t *tbase;        /* [n_items] first source of the block of each pass-1 tile    */
 } ag_block_plan;
 
 typedef struct ag_plan {

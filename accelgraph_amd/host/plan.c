@@ -32,6 +32,7 @@ void ag_plan_params_default(ag_plan_params *p)
     p->tile_order = AG_TILES_AUTO;
     p->piece_nnz = 0;
     p->top_table = AG_TOP_AUTO;
+    p->tile_sort = AG_SORT_AUTO;
 }
 
 int ag_plan_params_check(const ag_plan_params *p, const char **err)
@@ -86,6 +87,10 @@ int ag_plan_params_check(const ag_plan_params *p, const char **err)
         *err = "top_table must be AG_TOP_AUTO, AG_TOP_OFF or AG_TOP_ON";
         return AG_ERR_INVALID;
     }
+    if (p->tile_sort > AG_SORT_ON) {
+        *err = "tile_sort must be AG_SORT_AUTO, AG_SORT_OFF or AG_SORT_ON";
+        return AG_ERR_INVALID;
+    }
     if (p->piece_nnz > p->tile_nnz) {
         *err = "piece_nnz must be 0 (auto) or in [1, tile_nnz]";
         return AG_ERR_INVALID;
@@ -125,6 +130,8 @@ void ag_plan_free(ag_plan *p)
     free(p->blk.c_item);
     free(p->blk.hot_seg);
     free(p->blk.hgrp);
+    free(p->blk.vsp);
+    free(p->blk.tbase);
     memset(p, 0, sizeof(*p));
 }
 
@@ -300,6 +307,95 @@ static uint32_t row_lines(const uint32_t *ss, uint32_t d, const blk_key *K)
         i = j;
     }
     return cnt;
+}
+
+/* The ids of every pass-1 tile sorted by source, stably (equal sources keep their order),
+ * each packed with its position in the tile: vsp[e0 + k] = pos << bits | (src - base), base
+ * = the tile's block start, kept in tbase. A tile never crosses a block key, so its sources
+ * lie in one aligned block of 2^bits (checked; a tile that did not would leave the plan
+ * unsorted); bits <= 20 and pos < AG_TILE_NNZ_MAX = 2^12 fill at most 32 bits. Per tile a
+ * two-digit LSD counting sort of pos << 20 | (src - base) on its low 20 bits. The weights
+ * (vw) are put in the same order, so pass 1 reads them beside the packed ids. */
+static int sort_tile_ids(ag_block_plan *B)
+{
+    const uint32_t bits = B->bits, mask = (1u << bits) - 1u;
+    const uint32_t n_items = B->n_items;
+    B->vsp = (uint32_t *)zalloc(B->mb ? B->mb : 1, sizeof(uint32_t));
+    B->tbase = (uint32_t *)zalloc(n_items ? n_items : 1, sizeof(uint32_t));
+    if (!B->vsp || !B->tbase)
+        return AG_ERR_OOM;
+    int bad = 0, oom = 0;
+    #pragma omp parallel reduction(|:bad, oom)
+    {
+        uint32_t *a = (uint32_t *)malloc(2 * AG_TILE_NNZ_MAX * sizeof(uint32_t));
+        uint32_t *cnt = (uint32_t *)malloc(1024 * sizeof(uint32_t));
+        if (!a || !cnt)
+            oom = 1;
+        #pragma omp for schedule(dynamic, 64)
+        for (uint32_t i = 0; i < n_items; i++) {
+            if (oom || (B->item[2 * i + 1] & 15u) != AG_ITEM_VTILE)
+                continue;
+            const uint32_t first = B->item[2 * i], last = first + (B->item[2 * i + 1] >> 4);
+            const uint32_t e0 = B->voff[first], nnz = B->voff[last] - e0;
+            const uint32_t base = nnz ? B->vsrc[e0] & ~mask : 0;
+            uint32_t *x = a, *y = a + AG_TILE_NNZ_MAX;
+            for (uint32_t k = 0; k < nnz; k++) {
+                const uint32_t s = B->vsrc[e0 + k];
+                bad |= (s & ~mask) != base;
+                x[k] = k << 20 | (s & mask);
+            }
+            for (uint32_t sh = 0; sh < 20; sh += 10) {
+                memset(cnt, 0, 1024 * sizeof(uint32_t));
+                for (uint32_t k = 0; k < nnz; k++)
+                    cnt[(x[k] >> sh) & 1023u]++;
+                for (uint32_t d = 0, run = 0; d < 1024; d++) {
+                    const uint32_t c = cnt[d];
+                    cnt[d] = run;
+                    run += c;
+                }
+                for (uint32_t k = 0; k < nnz; k++)
+                    y[cnt[(x[k] >> sh) & 1023u]++] = x[k];
+                uint32_t *t = x;
+                x = y;
+                y = t;
+            }
+            for (uint32_t k = 0; k < nnz; k++)
+                B->vsp[e0 + k] = (x[k] >> 20) << bits | (x[k] & 0xFFFFFu);
+            B->tbase[i] = base;
+        }
+        free(a);
+        free(cnt);
+    }
+    /* the weights in the same order, once every tile is known to be sorted */
+    if (!oom && !bad && B->vw) {
+        #pragma omp parallel reduction(|:oom)
+        {
+            float *tw = (float *)malloc(AG_TILE_NNZ_MAX * sizeof(float));
+            if (!tw)
+                oom = 1;
+            #pragma omp for schedule(dynamic, 64)
+            for (uint32_t i = 0; i < n_items; i++) {
+                if (oom || (B->item[2 * i + 1] & 15u) != AG_ITEM_VTILE)
+                    continue;
+                const uint32_t first = B->item[2 * i], last = first + (B->item[2 * i + 1] >> 4);
+                const uint32_t e0 = B->voff[first], nnz = B->voff[last] - e0;
+                for (uint32_t k = 0; k < nnz; k++)
+                    tw[k] = B->vw[e0 + (B->vsp[e0 + k] >> bits)];
+                memcpy(B->vw + e0, tw, (size_t)nnz * sizeof(float));
+            }
+            free(tw);
+        }
+    }
+    if (oom)
+        return AG_ERR_OOM;
+    if (bad) {   /* unreachable while tiles keep to one block */
+        free(B->vsp);
+        free(B->tbase);
+        B->vsp = B->tbase = NULL;
+        return AG_OK;
+    }
+    B->sorted = 1;
+    return AG_OK;
 }
 
 /* Pass-1 items of every slice: per block, greedy tiles of consecutive virtual rows (at most
@@ -873,6 +969,12 @@ static int build_block_plan(ag_plan *p, const char **err)
             }
         }
         free(vslot);
+    }
+
+    /* 6: tile_sort (not with hot segments, whose pass reads vw in virtual-row order) */
+    if (B->bits <= 20 && !H && q->tile_sort != AG_SORT_OFF && sort_tile_ids(B) != AG_OK) {
+        oom = 1;
+        goto done;
     }
 
 done:

@@ -318,7 +318,7 @@ def ppr_section(ag, dev, scale: int, k: int, cpu: bool) -> dict:
     return res
 
 
-def spmv_section(ag, dev, scale: int, reps: int, cpu: bool) -> dict:
+def spmv_section(ag, dev, scale: int, reps: int, cpu: bool, pkw: dict) -> dict:
     """SpMV pull on a weighted RMAT graph: w = (k+1) 2^-24 and x = k 2^-24 with k uniform on
     [0, 2^24) from a seeded stream; FP32 (binary32 products, FP32 sums) and the RTL's Q16.16
     fixed-point datapath (u64 sums). Algorithmic bytes 12M + 8N + 4 (FP32), 12M + 12N + 4
@@ -328,10 +328,10 @@ def spmv_section(ag, dev, scale: int, reps: int, cpu: bool) -> dict:
     rng = np.random.default_rng(ag.DEFAULT_SEED)
     g.set_weights(((rng.integers(0, 1 << 24, g.m) + 1) * 2.0 ** -24).astype(np.float32))
     x = (rng.integers(0, 1 << 24, g.n) * 2.0 ** -24).astype(np.float32)
-    dg = dev.upload(g, weights=True)
+    dg = dev.upload(g, weights=True, params=ag.plan_params(**pkw) if pkw else None)
     info = dg.info()
     out = {"graph": f"RMAT scale {scale} (n={g.n}, m={g.m}), weighted",
-           "src_block_bits": info.src_block_bits}
+           "src_block_bits": info.src_block_bits, "sorted_tiles": bool(info.sorted_tiles)}
     xq = dev.fixed_encode(32, x).astype(np.uint32)
     for name, fn, arg, nb in (("f32", dg.spmv, x, 12 * g.m + 8 * g.n + 4),
                               ("fixed32", dg.spmv_fixed, xq, 12 * g.m + 12 * g.n + 4)):
@@ -502,6 +502,8 @@ def main() -> int:
                     help="plan parameter src_block_bits for the bench graphs (0 = auto)")
     ap.add_argument("--top-table", type=int, default=0,
                     help="plan parameter top_table for the bench graphs (0 auto, 1 off, 2 on)")
+    ap.add_argument("--tile-sort", type=int, default=0,
+                    help="plan parameter tile_sort for the bench graphs (0 auto, 1 off, 2 on)")
     ap.add_argument("--no-rows", action="store_true",
                     help="skip the fixed-32, quantised, push and data-driven rows")
     args = ap.parse_args()
@@ -559,6 +561,8 @@ def main() -> int:
         pkw["src_block_bits"] = args.src_block_bits
     if args.top_table:
         pkw["top_table"] = args.top_table
+    if args.tile_sort:
+        pkw["tile_sort"] = args.tile_sort
     dg = dev.upload(g, params=ag.plan_params(**pkw) if pkw else None)
     info = dg.info()
     log(f"uploaded + planned in {time.perf_counter() - t0:.1f} s; bins rows="
@@ -613,7 +617,8 @@ def main() -> int:
         if t and t.get("hot_segments") == info.hot_segments and \
                 t.get("src_block_bits") == info.src_block_bits and \
                 t.get("top_sources") == info.top_sources and \
-                t.get("virtual_rows") == info.virtual_rows:
+                t.get("virtual_rows") == info.virtual_rows and \
+                t.get("sorted_tiles", 0) == info.sorted_tiles:
             traffic = t["traffic_bytes_per_launch"]
             traffic_src = (f"profiled, not measured in this run: {t['profile']} "
                            f"(rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a copy kernel)")
@@ -645,6 +650,7 @@ def main() -> int:
                      "hot_edge_fraction": round(info.hot_edges / max(m, 1), 4),
                      "lds_top_sources": info.top_sources,
                      "lds_top_edge_fraction": round(info.top_edges / max(m, 1), 4),
+                     "sorted_tiles": bool(info.sorted_tiles),
                      "measured_copy_gb_per_s": round(copy_gbs, 1)},
     }
 
@@ -688,7 +694,7 @@ def main() -> int:
         log(f"ppr: {out['ppr_64']} ({time.perf_counter() - t0:.1f} s)")
     if not args.no_spmv:
         t0 = time.perf_counter()
-        out["spmv"] = spmv_section(ag, dev, args.spmv_scale, 20, not args.no_cpu)
+        out["spmv"] = spmv_section(ag, dev, args.spmv_scale, 20, not args.no_cpu, pkw)
         log(f"spmv: {out['spmv']} ({time.perf_counter() - t0:.1f} s)")
     if not args.no_spmv:
         t0 = time.perf_counter()

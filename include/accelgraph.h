@@ -139,11 +139,21 @@ typedef struct ag_plan_params {
                                   the edges, 2.9% faster; FP32 SpMV measured no gain). AG_TOP_AUTO (default): when they
                                   receive at least 2% of the blocked in-edges; AG_TOP_OFF;
                                   AG_TOP_ON                                                  */
+    uint32_t tile_sort;        /* with source blocking of at most 2^20 sources per block and
+                                  tiles of at most 4096 in-edges, the first pass reads each
+                                  tile's source ids sorted by source, each packed with the
+                                  edge's position in the tile (one 32-bit word, no extra
+                                  bytes), so the lanes of a wavefront gather neighbouring
+                                  sources and share more L2 lines; every term still lands at
+                                  its position, so the sums are the same bit for bit. Not
+                                  with hot segments. AG_SORT_AUTO (default), AG_SORT_OFF,
+                                  AG_SORT_ON (the same while it is the default)              */
 } ag_plan_params;
 
 enum { AG_ORDER_FAST = 0, AG_ORDER_REFERENCE = 1 };
 enum { AG_TILES_AUTO = 0, AG_TILES_ROUND_ROBIN = 1, AG_TILES_TICKET = 2 };
 enum { AG_TOP_AUTO = 0, AG_TOP_OFF = 1, AG_TOP_ON = 2 };
+enum { AG_SORT_AUTO = 0, AG_SORT_OFF = 1, AG_SORT_ON = 2 };
 
 #define AG_SRC_BLOCK_AUTO 0u
 #define AG_SRC_BLOCK_OFF 255u
@@ -170,6 +180,7 @@ typedef struct ag_graph_info {
     uint64_t hot_edges;        /* in-edges whose source lies in a hot segment               */
     uint32_t top_sources;      /* sources of the first-pass LDS table (top_table; 0: none)  */
     uint64_t top_edges;        /* blocked in-edges whose source lies in that table          */
+    uint32_t sorted_tiles;     /* 1: first-pass tiles read sorted by source (tile_sort)     */
 } ag_graph_info;
 
 /* Upload one directed graph. Arrays are host arrays in the reference's layout:

@@ -53,7 +53,7 @@ def test_c2_spmv_rmat22(device):
     g.set_weights(((rng.integers(0, 1 << 24, g.m) + 1) * 2.0 ** -24).astype(np.float32))
     x = (rng.integers(0, 1 << 24, g.n) * 2.0 ** -24).astype(np.float32)
     dg = device.upload(g, weights=True)
-    assert dg.info().src_block_bits == 19
+    assert dg.info().src_block_bits == 19 and dg.info().sorted_tiles == 1
     y, _ = dg.spmv(x, reps=2)
     # binary32 products (one rounding, the same on every path), then a sum in any order:
     # |y - sum p| <= gamma_(d-1) sum |p|
@@ -77,6 +77,14 @@ def test_c2_spmv_rmat22(device):
     np.testing.assert_array_equal(yq, want)
     say("  Q16.16 bit-exact")
     dg.close()
+    # first-pass ids (and weights) in stream order instead of sorted by source: the same
+    # sums bit for bit
+    un = device.upload(g, weights=True, params=ag.plan_params(tile_sort=ag.SORT_OFF))
+    assert un.info().sorted_tiles == 0
+    np.testing.assert_array_equal(un.spmv(x, reps=1)[0].view(np.uint32), y.view(np.uint32))
+    np.testing.assert_array_equal(un.spmv_fixed(xq, reps=1)[0], yq)
+    say("  unsorted tiles: identical")
+    un.close()
     g.close()
 
 
@@ -96,6 +104,7 @@ def test_c3_er25(device):
     info = dg.info()
     # evenly spread sources: every in-edge its own piece (the first pass a plain gather)
     assert info.src_block_bits == 19 and info.virtual_rows == g.m and info.top_sources == 0
+    assert info.sorted_tiles == 1
     kw = dict(max_iterations=4, stop_rule=ag.STOP_NONE)
     say("C3: fixed-32 pull, 4 iterations")
     fx = dg.pagerank(kind=32, history=False, **kw)

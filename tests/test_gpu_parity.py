@@ -42,9 +42,14 @@ PARAMS = {"default": None, "small": SMALL, "small-order1": dict(SMALL, order=1),
                                tile_order=ag.TILES_TICKET),
           # the last 4096 sources read from an LDS table in the FP32 first pass, and not
           "blocked-top": dict(src_block_bits=10, blocked_min_bin=0, top_table=ag.TOP_ON),
-          "blocked-notop": dict(src_block_bits=10, blocked_min_bin=0, top_table=ag.TOP_OFF)}
+          "blocked-notop": dict(src_block_bits=10, blocked_min_bin=0, top_table=ag.TOP_OFF),
+          # first-pass ids in stream order rather than sorted by source (sorted is the
+          # default without hot segments), with multi-edge and with single-edge pieces
+          "blocked-unsorted": dict(src_block_bits=10, blocked_min_bin=1, tile_sort=ag.SORT_OFF),
+          "blocked-edge-unsorted": dict(src_block_bits=9, blocked_min_bin=0, piece_nnz=1,
+                                        tile_sort=ag.SORT_OFF)}
 BLOCKED = ["blocked", "blocked-all", "blocked-small", "blocked-nohot", "blocked-edge",
-           "blocked-top", "blocked-notop"]
+           "blocked-top", "blocked-notop", "blocked-unsorted", "blocked-edge-unsorted"]
 
 
 def say(*a):
@@ -665,15 +670,21 @@ def test_pagerank_blocked_auto_rmat22(device):
     assert err <= FP32_TOL
     f32_row_bars(dg, g, 20)
     dg.close()
-    # pass-1 tiles by ticket instead of round-robin, and without the LDS table of the last
-    # sources (on by default here): the same ranks bit for bit
+    # pass-1 tiles by ticket instead of round-robin, without the LDS table of the last
+    # sources (on by default here), and with their ids in stream order instead of sorted by
+    # source (the default): the same ranks bit for bit
     assert info.top_sources == 4096 and info.top_edges * 50 >= g.m
-    for prm in (dict(tile_order=ag.TILES_TICKET), dict(top_table=ag.TOP_OFF)):
+    assert info.sorted_tiles == 1
+    for prm in (dict(tile_order=ag.TILES_TICKET), dict(top_table=ag.TOP_OFF),
+                dict(tile_sort=ag.SORT_OFF)):
         other = device.upload(g, params=ag.plan_params(**prm))
         np.testing.assert_array_equal(other.pagerank(**kw).ranks.view(np.uint32),
                                       run.ranks.view(np.uint32))
         if "top_table" in prm:
             assert other.info().top_sources == 0
+        if "tile_sort" in prm:
+            assert other.info().sorted_tiles == 0
+            np.testing.assert_array_equal(other.pagerank(kind=32, **kw).ranks, fx.ranks)
         other.close()
     plain = device.upload(g, params=ag.plan_params(src_block_bits=ag.SRC_BLOCK_OFF))
     assert plain.info().src_block_bits == 0

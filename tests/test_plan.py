@@ -24,7 +24,8 @@ class BlockPlan(C.Structure):
                 ("hot_bits", C.c_uint32), ("n_hot", C.c_uint32), ("hot_edges", C.c_uint64),
                 ("hot_seg", u32p), ("hot_end", C.c_uint32 * 8), ("n_hgrp", C.c_uint32),
                 ("hgrp", u32p), ("hgrp_off", C.c_uint32 * 9), ("piece", C.c_uint32),
-                ("top0", C.c_uint32), ("top_edges", C.c_uint64)]
+                ("top0", C.c_uint32), ("top_edges", C.c_uint64), ("sorted", C.c_uint32),
+                ("vsp", u32p), ("tbase", u32p)]
 
 
 class Plan(C.Structure):
@@ -212,6 +213,34 @@ def block_keys(B, n):
     return keys
 
 
+def check_sorted_tiles(p, it):
+    """tile_sort: with blocks of at most 2^20 sources (and the sort not switched off) every
+    pass-1 tile's packed words decode to a permutation of its positions whose sources are
+    the tile's ids in stream order, in ascending source order with equal sources in position
+    order; tbase is the block start of the tile's sources."""
+    B, prm = p.blk, p.params
+    want = B.bits <= 20 and prm.tile_sort != ag.SORT_OFF and B.n_hot == 0
+    assert B.sorted == int(want)
+    if not B.sorted:
+        return
+    voff = view(B.voff, B.nv + 1).astype(np.int64)
+    vsrc = view(B.vsrc, B.mb).astype(np.int64)
+    vsp = view(B.vsp, B.mb).astype(np.int64)
+    tb = view(B.tbase, B.n_items).astype(np.int64)
+    mask = (1 << B.bits) - 1
+    for i, (first, kc) in enumerate(it):
+        if kc & 15 != ITEM_VTILE:
+            continue
+        e0, e1 = voff[first], voff[first + (kc >> 4)]
+        w = vsp[e0:e1]
+        pos, s = w >> B.bits, tb[i] + (w & mask)
+        assert tb[i] == (vsrc[e0] >> B.bits) << B.bits
+        assert np.array_equal(np.sort(pos), np.arange(e1 - e0))
+        assert np.array_equal(s, vsrc[e0 + pos])
+        key = s * 4096 + pos
+        assert np.all(np.diff(key) > 0)
+
+
 def check_block_plan(g, p):
     """Every blocked row's in-edges appear exactly once among its virtual rows, grouped by
     source block in stream order; virtual rows are ordered (slice, block, row, piece); the
@@ -339,6 +368,7 @@ def check_block_plan(g, p):
         assert nxt == he
     assert np.all(covered == 1)
     assert sorted(reg_seen) == list(range(reg_total))
+    check_sorted_tiles(p, it)
     # regular schedule of the unblocked rows: bins below blocked_min_bin only
     blk_counts = list(p.bin_blocks)
     rb = [blk_counts[b] if b < prm.blocked_min_bin else 0 for b in range(5)]
@@ -444,6 +474,24 @@ def test_block_plan_auto_piece_on_even_sources():
         ag.lib().ag_plan_free(C.byref(p))
 
 
+@pytest.mark.parametrize("sort", ["auto", "on", "off"])
+@pytest.mark.parametrize("bits,piece", [(8, 0), (8, 1), (21, 0)])
+def test_block_plan_tile_sort(sort, bits, piece):
+    """Sorted pass-1 ids (tile_sort) on a skewed and an even graph, with multi-edge and
+    single-edge pieces; blocks above 2^20 sources leave the tiles unsorted."""
+    mode = {"auto": ag.SORT_AUTO, "on": ag.SORT_ON, "off": ag.SORT_OFF}[sort]
+    for g in (ag.Graph.rmat(12), ag.Graph.erdos_renyi(1 << 12, 1 << 16)):
+        st, p, err = build(g, ag.plan_params(src_block_bits=bits, blocked_min_bin=0,
+                                             piece_nnz=piece, tile_sort=mode))
+        assert st == 0, err
+        try:
+            assert p.blk.bits == bits
+            assert p.blk.sorted == int(bits <= 20 and mode != ag.SORT_OFF)
+            check_block_plan(g, p)
+        finally:
+            ag.lib().ag_plan_free(C.byref(p))
+
+
 def test_block_plan_long_pieces():
     # one row with 10000 in-edges from one source block is cut into pieces of 4096
     n = 10002
@@ -485,18 +533,30 @@ def test_block_plan_rejects_bad_bits():
     assert st == 1 and str(cap).encode() in err
 
 
-def test_block_plan_weights_follow_sources():
+@pytest.mark.parametrize("sort", [ag.SORT_OFF, ag.SORT_ON], ids=["unsorted", "sorted"])
+def test_block_plan_weights_follow_sources(sort):
     """Weighted graph: the virtual-row weights are the row's weights regrouped exactly like its
-    sources (SpMV reads vw[j] beside vsrc[j])."""
+    sources (SpMV reads vw[j] beside vsrc[j]); with sorted tiles they follow the packed ids
+    (vw[j] beside vsp[j])."""
     g = ag.Graph.rmat(11)
     w = (np.arange(g.m, dtype=np.float64) + 1).astype(np.float32)   # unique per inverse edge
-    st, p, err = build(g, ag.plan_params(src_block_bits=8, blocked_min_bin=0), weights=w)
+    st, p, err = build(g, ag.plan_params(src_block_bits=8, blocked_min_bin=0, tile_sort=sort),
+                       weights=w)
     assert st == 0, err
     try:
         B, n = p.blk, g.n
         d2o = view(p.d2o, n)
         voff, vsrc = view(B.voff, B.nv + 1), view(B.vsrc, B.mb)
         vw = np.ctypeslib.as_array(B.vw, shape=(B.mb,)).copy()
+        assert B.sorted == (sort == ag.SORT_ON)
+        if B.sorted:   # back to virtual-row order through the packed positions
+            vsp = view(B.vsp, B.mb).astype(np.int64)
+            vr = np.full_like(vw, np.nan)
+            for first, kc in view(B.item, 2 * B.n_items).reshape(-1, 2).astype(np.int64):
+                assert kc & 15 == ITEM_VTILE
+                e0, e1 = voff[first], voff[first + (kc >> 4)]
+                vr[e0 + (vsp[e0:e1] >> B.bits)] = vw[e0:e1]
+            vw = vr
         coff, cidx = view(B.coff, n + 1), view(B.cidx, B.nv)
         for r in range(B.row0, n):
             v = d2o[r]
