@@ -400,7 +400,8 @@ __device__ __forceinline__ void stage_edges(const uint32_t *src, uint32_t e0, ui
 // most AG_LANE_MAX elements is summed by one lane in element order (the first element starts
 // the sum); a longer row by a wavefront (lane-strided sums combined in a fixed tree). A row's
 // value depends only on its elements, whichever lane or wavefront computes it.
-template <class P>
+// kReady: the caller has zeroed *s_nlong and passed a barrier since the values were staged.
+template <class P, bool kReady = false>
 __device__ __forceinline__ void mixed_reduce(const uint32_t *bnd, uint32_t base, uint32_t r0,
                                              uint32_t r1, const P &pol, Red &red,
                                              const typename P::Acc *s_vals, uint32_t *s_long,
@@ -408,9 +409,11 @@ __device__ __forceinline__ void mixed_reduce(const uint32_t *bnd, uint32_t base,
 {
     using Acc = typename P::Acc;
     const uint32_t tid = threadIdx.x;
-    if (tid == 0)
-        *s_nlong = 0u;
-    __syncthreads();
+    if (!kReady) {
+        if (tid == 0)
+            *s_nlong = 0u;
+        __syncthreads();
+    }
     for (uint32_t r = r0 + tid; r < r1; r += AG_WG) {
         const uint32_t b = bnd[r] - base, e = bnd[r + 1] - base;
         if (e - b > AG_LANE_MAX) {
@@ -487,59 +490,16 @@ __device__ __forceinline__ void gather_singletons(const uint32_t *src, uint32_t 
     }
 }
 
-// stage_edges over a tile whose ids are stored sorted by source: word w holds the source's
-// offset from `base` in its low `bits` bits and the edge's position in the tile above them.
-// Each term lands at its position, so s_vals ends as stage_edges leaves it; the lanes of a
-// wavefront gather neighbouring sources, which share more L2 lines than the in-edges of
-// consecutive rows do. Per-edge data (weights) is stored in the same sorted order and read
-// at the stream index, coalesced.
-template <class P, class TT>
-__device__ __forceinline__ void stage_sorted(const uint32_t *vsp, uint32_t base, uint32_t bits,
-                                             uint32_t e0, uint32_t nnz, const P &pol,
-                                             typename P::Acc *s_vals, const TT &tt)
-{
-    const uint32_t tid = threadIdx.x, mask = (1u << bits) - 1u;
-    for (uint32_t k = tid; k < nnz; k += AG_TILE_UNROLL * AG_WG) {
-        uint32_t w[AG_TILE_UNROLL];
-#pragma unroll
-        for (int u = 0; u < AG_TILE_UNROLL; u++) {
-            const uint32_t kk = k + u * AG_WG;
-            w[u] = kk < nnz ? ld_stream(&vsp[e0 + kk]) : 0u;
-        }
-#pragma unroll
-        for (int u = 0; u < AG_TILE_UNROLL; u++) {
-            const uint32_t kk = k + u * AG_WG;
-            if (kk < nnz) {
-                const uint32_t pos = w[u] >> bits;   // < nnz: a permutation of the tile
-                s_vals[pos] = tt.term(pol, e0 + kk, base + (w[u] & mask));
-            }
-        }
-    }
-}
-
-// One pass-1 tile (item `it`), virtual rows [first, last). When they are all single edges
-// (as many rows as edges) the tile is a plain gather; otherwise its edges are staged in LDS
-// and reduced row by row, ending with a barrier. Either way each partial is its piece's terms
-// added in element order, so the forms agree bit for bit. With sorted ids (B.sbits) the
-// terms are staged in LDS either way, and single edges are then written out in order.
+// One pass-1 tile, virtual rows [first, last). When they are all single edges (as many
+// rows as edges) the tile is a plain gather; otherwise its edges are staged in LDS and
+// reduced row by row, ending with a barrier. Either way each partial is its piece's terms
+// added in element order, so the two forms agree bit for bit.
 template <class PP, class TT>
 __device__ __forceinline__ void blocked_tile(const BlockGeom &B, const PP &pp, Red &red,
-                                             typename PP::Acc *s_vals, BlkLds &L, uint32_t it,
+                                             typename PP::Acc *s_vals, BlkLds &L,
                                              uint32_t first, uint32_t last, const TT &tt)
 {
     const uint32_t e0 = B.V.off[first], nnz = B.V.off[last] - e0;
-    if (B.sbits) {
-        stage_sorted(B.vsp, B.tbase[it], B.sbits, e0, nnz, pp, s_vals, tt);
-        __syncthreads();
-        if (nnz == last - first) {
-            for (uint32_t k = threadIdx.x; k < nnz; k += AG_WG)
-                pp.finish(first + k, s_vals[k], red);
-            __syncthreads();   // s_vals is reused by the next tile
-            return;
-        }
-        mixed_reduce(B.V.off, e0, first, last, pp, red, s_vals, L.lng, &L.nlong);
-        return;
-    }
     if (nnz == last - first) {
         gather_singletons(B.V.src, e0, first, nnz, pp, red, tt);
         return;
@@ -547,6 +507,131 @@ __device__ __forceinline__ void blocked_tile(const BlockGeom &B, const PP &pp, R
     stage_edges(B.V.src, e0, nnz, pp, s_vals, tt);
     __syncthreads();
     mixed_reduce(B.V.off, e0, first, last, pp, red, s_vals, L.lng, &L.nlong);
+}
+
+// A pass-1 item of a plan with sorted tiles (B.sbits): item `it` of the slice (none when
+// it >= end: valid 0). Uniform across the workgroup.
+struct SortedTile {
+    uint32_t valid, reg, first, last, e0, nnz, base;
+};
+
+__device__ __forceinline__ SortedTile sorted_tile_at(const BlockGeom &B, uint32_t it,
+                                                     uint32_t end)
+{
+    SortedTile t{};
+    if (it >= end)
+        return t;
+    const uint32_t kc = B.item[2 * it + 1];
+    t.valid = 1u;
+    t.first = B.item[2 * it];
+    if ((kc & 15u) == AG_ITEM_REG) {
+        t.reg = 1u;   // nnz 0: no ids to load
+        return t;
+    }
+    t.last = t.first + (kc >> 4);
+    t.e0 = B.V.off[t.first];
+    t.nnz = B.V.off[t.last] - t.e0;
+    t.base = B.tbase[it];
+    return t;
+}
+
+static_assert(AG_TILE_NNZ_MAX <= AG_TILE_UNROLL * AG_WG,
+              "a lane holds all of its ids of a sorted tile at once");
+
+// The packed ids of a sorted tile, AG_TILE_UNROLL per lane: the whole tile.
+__device__ __forceinline__ void load_sorted_ids(const uint32_t *vsp, const SortedTile &t,
+                                                uint32_t (&w)[AG_TILE_UNROLL])
+{
+#pragma unroll
+    for (int u = 0; u < AG_TILE_UNROLL; u++) {
+        const uint32_t kk = threadIdx.x + u * AG_WG;
+        w[u] = kk < t.nnz ? ld_stream(&vsp[t.e0 + kk]) : 0u;
+    }
+}
+
+// Pass 1 over sorted tiles, software-pipelined: word w of a tile holds the source's offset
+// from the tile's block start in its low sbits bits and the edge's position in the tile
+// above them, so the lanes of a wavefront gather neighbouring sources (they share more L2
+// lines than the in-edges of consecutive rows do) and each term lands at its position in
+// s_vals, which then holds what an unsorted tile stages; per-edge data (weights) is stored
+// in the same sorted order and read at the stream index. After a tile's gathers and barrier,
+// the next tile's item, offsets and ids are loaded while this tile's rows are summed, so a
+// workgroup waits on one dependent round of loads per tile (the gathers) instead of four.
+// Tiles go round-robin or, with B.dyn, by ticket, as in pull_blocked_items; a tile's
+// partials are the same either way and the same as with unsorted ids.
+template <class P, class PU, class TT>
+__device__ __forceinline__ void pull_sorted_items(const PullGeom &U, const BlockGeom &B,
+                                                  const P &pol, const PU &upol, Red &red,
+                                                  typename P::Acc *s_vals,
+                                                  typename P::Acc *s_wave, BlkLds &L,
+                                                  const TT &tt)
+{
+    using Acc = typename P::Acc;
+    const PartPol<P> pp{pol, (Acc *)B.part};
+    const uint32_t x = blockIdx.x % AG_NXCD, step = gridDim.x / AG_NXCD, tid = threadIdx.x;
+    const uint32_t lo = B.item_lo[x], end = B.item_off[x + 1];
+    const uint32_t bits = B.sbits, mask = (1u << bits) - 1u;
+    uint32_t *tick = B.tick + x, *left = B.tick + AG_NXCD + x;
+    uint32_t i;
+    if (B.dyn) {
+        if (tid == 0)
+            L.ticket[0] = atomicAdd(tick, 1u);
+        __syncthreads();
+        i = lo + L.ticket[0];
+    } else {
+        i = lo + blockIdx.x / AG_NXCD;
+    }
+    SortedTile cur = sorted_tile_at(B, i, end);
+    uint32_t w[AG_TILE_UNROLL];
+    load_sorted_ids(B.vsp, cur, w);
+    uint32_t k = 0;
+    while (cur.valid) {
+        uint32_t next = 0;
+        if (B.dyn && tid == 0)
+            next = atomicAdd(tick, 1u);   // in flight during the gathers
+        if (cur.reg) {
+            pull_one_item(U, upol, cur.first, red, s_vals, s_wave);
+        } else {
+#pragma unroll
+            for (int u = 0; u < AG_TILE_UNROLL; u++) {
+                const uint32_t kk = tid + u * AG_WG;
+                if (kk < cur.nnz) {
+                    const uint32_t pos = w[u] >> bits;   // < nnz: a permutation of the tile
+                    s_vals[pos] = tt.term(pp, cur.e0 + kk, cur.base + (w[u] & mask));
+                }
+            }
+        }
+        // the ticket word written here was last read before the previous tile's final
+        // barrier, and so was nlong
+        k ^= 1u;
+        if (tid == 0) {
+            L.ticket[k] = next;
+            L.nlong = 0u;
+        }
+        __syncthreads();
+        i = B.dyn ? lo + L.ticket[k] : i + step;
+        const SortedTile nxt = sorted_tile_at(B, i, end);
+        load_sorted_ids(B.vsp, nxt, w);   // in flight while this tile's rows are summed
+        if (!cur.reg) {
+            if (cur.nnz == cur.last - cur.first) {   // single edges: partials in order
+                for (uint32_t q = tid; q < cur.nnz; q += AG_WG)
+                    pp.finish(cur.first + q, s_vals[q], red);
+                __syncthreads();   // s_vals is reused by the next tile
+            } else {
+                mixed_reduce<PartPol<P>, true>(B.V.off, cur.e0, cur.first, cur.last, pp, red,
+                                               s_vals, L.lng, &L.nlong);
+            }
+        }
+        cur = nxt;
+    }
+    if (B.dyn && tid == 0) {
+        __threadfence();
+        // every workgroup of the slice has taken its last ticket once `left` reaches step
+        if (atomicAdd(left, 1u) == step - 1u) {
+            atomicExch(tick, 0u);
+            atomicExch(left, 0u);
+        }
+    }
 }
 
 template <class P, class PU, class TT = NoTop>
@@ -557,6 +642,10 @@ __device__ __forceinline__ void pull_blocked_items(const PullGeom &U, const Bloc
                                                    const TT &tt = TT())
 {
     using Acc = typename P::Acc;
+    if (B.sbits) {
+        pull_sorted_items(U, B, pol, upol, red, s_vals, s_wave, L, tt);
+        return;
+    }
     const PartPol<P> pp{pol, (Acc *)B.part};
     const uint32_t x = blockIdx.x % AG_NXCD;
     const uint32_t step = gridDim.x / AG_NXCD;   // the grid is a multiple of AG_NXCD
@@ -577,7 +666,7 @@ __device__ __forceinline__ void pull_blocked_items(const PullGeom &U, const Bloc
                 next = atomicAdd(tick, 1u);   // in flight while this tile runs
             const uint32_t i = lo + t;
             const uint32_t first = B.item[2 * i], last = first + (B.item[2 * i + 1] >> 4);
-            blocked_tile(B, pp, red, s_vals, L, i, first, last, tt);
+            blocked_tile(B, pp, red, s_vals, L, first, last, tt);
             k ^= 1u;
             if (tid == 0)
                 L.ticket[k] = next;
@@ -599,7 +688,7 @@ __device__ __forceinline__ void pull_blocked_items(const PullGeom &U, const Bloc
         if ((kc & 15u) == AG_ITEM_REG) {
             pull_one_item(U, upol, first, red, s_vals, s_wave);
         } else {
-            blocked_tile(B, pp, red, s_vals, L, i, first, first + (kc >> 4), tt);
+            blocked_tile(B, pp, red, s_vals, L, first, first + (kc >> 4), tt);
         }
     }
 }
