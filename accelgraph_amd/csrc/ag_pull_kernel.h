@@ -130,6 +130,10 @@ __device__ __forceinline__ uint32_t ld_stream(const uint32_t *p)
 }
 
 #define AG_UNROLL 4
+// 8-byte partials per lane in flight when a combine tile is staged in LDS. Measured against
+// one at a time: RMAT-25 fixed-32 iteration 2.65 -> 2.57 ms; the same for 4-byte partials
+// made the FP32 iteration slower (RMAT-25 2.36 -> 2.43 ms, ER-25 3.99 -> 4.34 ms)
+#define AG_CMB_UNROLL 8
 // source-id loads (then gathers) in flight per lane in an LDS tile: 16 covers a full tile
 // (AG_TILE_NNZ_MAX / AG_WG) in one round. Measured on RMAT-25 / ER-25 pass 1: 2 / 4 / 8 / 16
 // gave 2.96 / 2.76 / 2.63 / 2.57 ms and 5.11 / 4.63 / 4.52 / 4.43 ms per iteration
@@ -798,6 +802,10 @@ __device__ __forceinline__ void pull_combine_items(const BlockGeom &B, const P &
                                                    typename P::Acc *s_wave, BlkLds &L)
 {
     using Acc = typename P::Acc;
+    // 8-byte partials (the fixed-point and quantised kinds, whose combine fits 4 workgroups
+    // per CU in LDS) keep AG_CMB_UNROLL loads in flight per lane; 4-byte ones (8 workgroups
+    // per CU, the FP32 combine near the copy rate) one
+    constexpr int kU = sizeof(Acc) > 4 ? AG_CMB_UNROLL : 1;
     uint32_t *s_pre = L.pre, *s_start = L.start;
     const CombPol<P> cp{pol, (const Acc *)B.part};
     const uint32_t nb = B.nblocks, tid = threadIdx.x;
@@ -817,18 +825,32 @@ __device__ __forceinline__ void pull_combine_items(const BlockGeom &B, const P &
         for (uint32_t b = tid; b < nb; b += AG_WG)
             s_start[b] = start[b];
         __syncthreads();
-        for (uint32_t k = tid; k < np; k += AG_WG) {
-            // block of the k-th partial: s_pre[lo] <= k < s_pre[lo + 1] (s_pre[nb] = np)
-            uint32_t lo = 0, hi = nb;
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (s_pre[mid] <= k)
-                    lo = mid;
-                else
-                    hi = mid;
+        // kU partials per lane in flight at once (their block searches are independent),
+        // then their LDS writes
+        for (uint32_t k0 = tid; k0 < np; k0 += kU * AG_WG) {
+            Acc val[kU];
+            uint32_t slot[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                const uint32_t k = k0 + u * AG_WG;
+                if (k < np) {
+                    // block of the k-th partial: s_pre[lo] <= k < s_pre[lo + 1] (s_pre[nb] = np)
+                    uint32_t lo = 0, hi = nb;
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (s_pre[mid] <= k)
+                            lo = mid;
+                        else
+                            hi = mid;
+                    }
+                    val[u] = cp.part[s_start[lo] + (k - s_pre[lo])];
+                    slot[u] = B.lpos[p0 + k];
+                }
             }
-            const uint32_t v = s_start[lo] + (k - s_pre[lo]);
-            s_vals[B.lpos[p0 + k]] = cp.part[v];
+#pragma unroll
+            for (int u = 0; u < kU; u++)
+                if (k0 + u * AG_WG < np)
+                    s_vals[slot[u]] = val[u];
         }
         __syncthreads();
         mixed_reduce(B.C.off, p0, r0, r1, cp, red, s_vals, L.lng, &L.nlong);
