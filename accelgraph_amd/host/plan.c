@@ -750,11 +750,15 @@ static int build_block_plan(ag_plan *p, const char **err)
     }
     free(nvr);
     const uint32_t nv = B->nv;
-    /* slices: contiguous blocked rows of about equal pass-1 cost, 10 per in-edge (its source
-     * id and LDS traffic), 17 per gathered line (an L2 request) and 11 per virtual row (its
-     * offset reads and partial write) -- weights fitted to the measured pass-1 time of each
-     * slice of RMAT-25 (within 3%); an edge count alone left the slice of light rows 30%
-     * faster than the others */
+    /* slices: contiguous blocked rows of about equal pass-1 cost, per in-edge (its source id,
+     * its gather and LDS traffic), per gathered line of a piece (an L2 request) and per
+     * virtual row (its offset reads and partial write), weights fitted to the measured
+     * pass-1 time of each slice of RMAT-25. Tiles read in stream order: 10 / 17 / 11 (slices
+     * within 3%; an edge count alone left the slice of light rows 30% faster). Tiles sorted
+     * by source (their ids then share lines across rows, so a piece's own line count no
+     * longer predicts its requests): 100 / 0 / 46 -- 10 / 17 / 11 left the two slices of
+     * the heaviest rows 7% slower than the rest (RMAT-25 iteration 2.357 ms; 100 / 0 / 46:
+     * 2.317 ms, and 30 or 62 per virtual row 2.37 / 2.35 ms). */
     {
         uint64_t *cpre = (uint64_t *)malloc(((size_t)(n - row0) + 1) * sizeof(uint64_t));
         if (!cpre) {
@@ -766,9 +770,11 @@ static int build_block_plan(ag_plan *p, const char **err)
             return AG_ERR_OOM;
         }
         cpre[0] = 0;
+        const int sorted = bits <= 20 && !H && q->tile_sort != AG_SORT_OFF;   /* as step 6 */
+        const uint64_t wE = sorted ? 100 : 10, wL = sorted ? 0 : 17, wP = sorted ? 46 : 11;
         for (uint32_t r = row0; r < n; r++)
-            cpre[r - row0 + 1] = cpre[r - row0] + 10ull * (p->off[r + 1] - p->off[r]) +
-                                 17ull * lnr[r - row0] + 11ull * (B->coff[r + 1] - B->coff[r]);
+            cpre[r - row0 + 1] = cpre[r - row0] + wE * (p->off[r + 1] - p->off[r]) +
+                                 wL * lnr[r - row0] + wP * (B->coff[r + 1] - B->coff[r]);
         const uint64_t total = cpre[n - row0];
         B->slice_row[0] = row0;
         for (uint32_t x = 1; x < AG_NXCD; x++) {
