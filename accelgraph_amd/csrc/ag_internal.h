@@ -105,7 +105,7 @@ struct PullGeom {
 
 // Source-blocked pull: virtual rows, the combine schedule and the work items of each slice.
 struct BlockGeom {
-    PullGeom V;              // off = voff, src = vsrc (virtual rows)
+    PullGeom V;              // off = voff, src = vsrc (virtual rows; null with sorted tiles)
     PullGeom C;              // off = coff, src = cidx (a row's partial slots)
     const uint32_t *item;    // [2 * n_items] pass-1 items
     const uint32_t *c_item;  // [2 * c_n_items] combine items

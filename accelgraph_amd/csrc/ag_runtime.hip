@@ -440,7 +440,9 @@ extern "C" int ag_graph_upload(ag_device *dev, uint32_t n, uint32_t m, const uin
         g->blk.hot_seg = NULL;   // host-only array of the plan, freed with it
         const size_t nt = B.c_n_tiles, nb = B.nblocks;
         st = dev_upload(g, &g->blk.voff, B.voff, (size_t)B.nv + 1);
-        if (st == AG_OK) st = dev_upload(g, &g->blk.vsrc, B.vsrc, B.mb);
+        // with sorted tiles pass 1 reads only the packed ids (vsp): the ids in virtual-row
+        // order stay on the host (4 bytes per blocked in-edge less in HBM)
+        if (st == AG_OK && !B.sorted) st = dev_upload(g, &g->blk.vsrc, B.vsrc, B.mb);
         if (st == AG_OK && B.vw) st = dev_upload(g, &g->blk.vw, B.vw, B.mb);
         if (st == AG_OK) st = dev_upload(g, &g->blk.item, B.item, (size_t)B.n_items * 2);
         if (st == AG_OK) st = dev_upload(g, &g->blk.coff, B.coff, (size_t)plan.n + 1);
