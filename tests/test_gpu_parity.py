@@ -686,6 +686,20 @@ def test_pagerank_blocked_auto_rmat22(device):
             assert other.info().sorted_tiles == 0
             np.testing.assert_array_equal(other.pagerank(kind=32, **kw).ranks, fx.ranks)
         other.close()
+    # 2^20-source blocks fill the packed word (20 bits of source offset, 12 of position):
+    # sorted and unsorted tiles give the same ranks bit for bit
+    wide = [device.upload(g, params=ag.plan_params(src_block_bits=20, tile_sort=s))
+            for s in (ag.SORT_ON, ag.SORT_OFF)]
+    assert [w.info().sorted_tiles for w in wide] == [1, 0]
+    assert all(w.info().src_block_bits == 20 for w in wide)
+    r_on, r_off = (w.pagerank(**kw).ranks for w in wide)
+    np.testing.assert_array_equal(r_on.view(np.uint32), r_off.view(np.uint32))
+    f_on, f_off = (w.pagerank(kind=32, **kw).ranks for w in wide)
+    np.testing.assert_array_equal(f_on, fx.ranks)
+    np.testing.assert_array_equal(f_off, fx.ranks)
+    assert np.abs(r_on - run.ranks).max() <= FP32_TOL
+    for w in wide:
+        w.close()
     plain = device.upload(g, params=ag.plan_params(src_block_bits=ag.SRC_BLOCK_OFF))
     assert plain.info().src_block_bits == 0
     run2 = plain.pagerank(**kw)

@@ -492,6 +492,23 @@ def test_block_plan_tile_sort(sort, bits, piece):
             ag.lib().ag_plan_free(C.byref(p))
 
 
+def test_block_plan_tile_sort_full_word():
+    """2^20-source blocks and 4096-edge tiles fill the packed word: positions up to 4095 above
+    20 bits of source offset (RMAT-21, 2 M vertices)."""
+    g = ag.Graph.rmat(21)
+    st, p, err = build(g, ag.plan_params(src_block_bits=20, blocked_min_bin=0))
+    assert st == 0, err
+    try:
+        B = p.blk
+        assert B.bits == 20 and B.sorted == 1
+        it = view(B.item, 2 * B.n_items).reshape(-1, 2).astype(np.int64)
+        check_sorted_tiles(p, it)
+        vsp = view(B.vsp, B.mb).astype(np.int64)
+        assert (vsp >> 20).max() == 4095 and (vsp & 0xFFFFF).max() == (1 << 20) - 1
+    finally:
+        ag.lib().ag_plan_free(C.byref(p))
+
+
 def test_block_plan_long_pieces():
     # one row with 10000 in-edges from one source block is cut into pieces of 4096
     n = 10002
