@@ -130,10 +130,14 @@ __device__ __forceinline__ uint32_t ld_stream(const uint32_t *p)
 }
 
 #define AG_UNROLL 4
-// 8-byte partials per lane in flight when a combine tile is staged in LDS. Measured against
-// one at a time: RMAT-25 fixed-32 iteration 2.65 -> 2.57 ms; the same for 4-byte partials
-// made the FP32 iteration slower (RMAT-25 2.36 -> 2.43 ms, ER-25 3.99 -> 4.34 ms)
+// Partials per lane in flight when a combine tile is staged in LDS: 8 of 8 bytes (the
+// fixed-point and quantised kinds, 4 workgroups per CU for their LDS; RMAT-25 fixed-32
+// iteration 2.65 -> 2.57 ms against one at a time, 4 measured the same), 4 of 4 bytes (FP32,
+// 8 workgroups per CU). For 4-byte partials 1 / 2 / 4 / 6 measured RMAT-25 2.322 / 2.297 /
+// 2.286 / 2.285 ms and ER-25 4.00 / 3.83 / 3.77 / 3.74 ms, and 8 was slower than 1
+// (2.43 / 4.34 ms; 60 VGPRs, the same occupancy)
 #define AG_CMB_UNROLL 8
+#define AG_CMB_UNROLL_4B 4
 // source-id loads (then gathers) in flight per lane in an LDS tile: 16 covers a full tile
 // (AG_TILE_NNZ_MAX / AG_WG) in one round. Measured on RMAT-25 / ER-25 pass 1: 2 / 4 / 8 / 16
 // gave 2.96 / 2.76 / 2.63 / 2.57 ms and 5.11 / 4.63 / 4.52 / 4.43 ms per iteration
@@ -802,10 +806,7 @@ __device__ __forceinline__ void pull_combine_items(const BlockGeom &B, const P &
                                                    typename P::Acc *s_wave, BlkLds &L)
 {
     using Acc = typename P::Acc;
-    // 8-byte partials (the fixed-point and quantised kinds, whose combine fits 4 workgroups
-    // per CU in LDS) keep AG_CMB_UNROLL loads in flight per lane; 4-byte ones (8 workgroups
-    // per CU, the FP32 combine near the copy rate) one
-    constexpr int kU = sizeof(Acc) > 4 ? AG_CMB_UNROLL : 1;
+    constexpr int kU = sizeof(Acc) > 4 ? AG_CMB_UNROLL : AG_CMB_UNROLL_4B;
     uint32_t *s_pre = L.pre, *s_start = L.start;
     const CombPol<P> cp{pol, (const Acc *)B.part};
     const uint32_t nb = B.nblocks, tid = threadIdx.x;
